@@ -1,0 +1,229 @@
+"""MI355X replica-exchange Monte Carlo docking engine (GeauxDock hot path).
+
+Python mirror of the engine's C ABI (include/geauxdock.h). `run()` follows the
+reference's Run() (src/run.h:10-22, src/run.cu:33-289): it takes one complex,
+runs the MC launch loop and returns the accepted-step records per replica and
+the McLog figures (t0 compute time, t1 wall time, acceptance ratio,
+steps_total) that PrintSummary reports (src/util.C:1347-1364).
+
+The library is the in-tree libgeauxdock.so; there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgeauxdock.so")
+
+__all__ = ["Engine", "GeauxDockError", "load_library", "run", "abi"]
+
+
+class GeauxDockError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{abi.STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def load_library():
+    """Load the HIP engine. Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with geaxdock_gpu_amd.build.build()")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    lib.gd_create.argtypes = [P(vp), P(abi.Opts)]
+    lib.gd_destroy.argtypes = [vp]
+    lib.gd_destroy.restype = None
+    lib.gd_last_error.argtypes = [vp]
+    lib.gd_last_error.restype = C.c_char_p
+    lib.gd_add_complex.argtypes = [vp, P(abi.ComplexDesc), P(C.c_int)]
+    lib.gd_run.argtypes = [vp, C.c_int64, C.c_int, abi.RECORD_SINK, vp, P(abi.RunStats)]
+    lib.gd_get_state.argtypes = [vp, C.c_int, P(C.c_float), vp, vp, P(C.c_int32), P(C.c_int32)]
+    lib.gd_checkpoint.argtypes = [vp, vp, P(C.c_size_t)]
+    lib.gd_restore.argtypes = [vp, vp, C.c_size_t]
+    lib.gd_calc_energy.argtypes = [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_int), P(C.c_float), vp]
+    lib.gd_accept.argtypes = [vp, C.c_int, P(C.c_float), P(C.c_float), P(C.c_float),
+                              P(C.c_float), P(C.c_int32)]
+    lib.gd_exchange.argtypes = [vp, C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_float),
+                                P(C.c_float), P(C.c_int32), P(C.c_int32)]
+    lib.gd_philox.argtypes = [vp, C.c_int, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+    lib.gd_num_replicas.argtypes = [vp, C.c_int]
+    lib.gd_num_replicas.restype = C.c_int
+    for f in ("gd_create", "gd_add_complex", "gd_run", "gd_get_state", "gd_checkpoint",
+              "gd_restore", "gd_calc_energy", "gd_accept", "gd_exchange", "gd_philox"):
+        getattr(lib, f).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _u(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+class Engine:
+    """One engine context on one device (gd_create ... gd_destroy)."""
+
+    def __init__(self, seed=0, device=0, steps_per_dump=2000, record=True, lanes_per_replica=0,
+                 atoms_per_lane=0, verbose=False):
+        self.lib = load_library()
+        o = abi.Opts()
+        o.device, o.seed, o.steps_per_dump = device, seed, steps_per_dump
+        o.record, o.verbose = int(record), int(verbose)
+        o.lanes_per_replica, o.atoms_per_lane = lanes_per_replica, atoms_per_lane
+        self._ctx = C.c_void_p()
+        st = self.lib.gd_create(C.byref(self._ctx), C.byref(o))
+        if st != 0:
+            msg = self._err()
+            self.close()
+            raise GeauxDockError(st, msg)
+        self._descs = []
+        self.complexes = []
+
+    def _err(self):
+        if not self._ctx:
+            return "no context"
+        return self.lib.gd_last_error(self._ctx).decode()
+
+    def _check(self, st):
+        if st != 0:
+            raise GeauxDockError(st, self._err())
+
+    def close(self):
+        if self._ctx:
+            self.lib.gd_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_complex(self, cx):
+        d = cx.desc()
+        cid = C.c_int(-1)
+        self._check(self.lib.gd_add_complex(self._ctx, C.byref(d), C.byref(cid)))
+        self._descs.append(d)
+        self.complexes.append(cx)
+        return cid.value
+
+    def run(self, n_steps, steps_per_exchange=0, sink=None):
+        """Advance every replica n_steps. sink(cid, records ndarray) gets each
+        dump window's accepted-state records."""
+        st = abi.RunStats()
+        if sink is None:
+            cb = abi.RECORD_SINK()
+        else:
+            def _cb(user, cid, recs, n):
+                buf = C.cast(recs, C.POINTER(C.c_char * (n * abi.RECORD_DTYPE.itemsize))).contents
+                sink(cid, np.frombuffer(bytes(buf), dtype=abi.RECORD_DTYPE).copy())
+            cb = abi.RECORD_SINK(_cb)
+        self._check(self.lib.gd_run(self._ctx, int(n_steps), int(steps_per_exchange), cb, None,
+                                    C.byref(st)))
+        return {f: getattr(st, f) for f, _ in abi.RunStats._fields_}
+
+    def num_replicas(self, cid):
+        return self.lib.gd_num_replicas(self._ctx, cid)
+
+    def state(self, cid):
+        n = self.num_replicas(cid)
+        mm = np.zeros((n, 6), np.float32)
+        e = np.zeros(n, abi.ENERGY_DTYPE)
+        rep = np.zeros((n, 4), np.int32)
+        acs_mc = np.zeros(n, np.int32)
+        acs_x = np.zeros(n, np.int32)
+        self._check(self.lib.gd_get_state(self._ctx, cid, _f(mm), e.ctypes.data, rep.ctypes.data,
+                                          _i(acs_mc), _i(acs_x)))
+        return dict(movematrix=mm, energy=e, replica=rep, acs_mc=acs_mc, acs_x=acs_x)
+
+    def checkpoint(self):
+        sz = C.c_size_t(0)
+        self._check(self.lib.gd_checkpoint(self._ctx, None, C.byref(sz)))
+        buf = C.create_string_buffer(sz.value)
+        self._check(self.lib.gd_checkpoint(self._ctx, buf, C.byref(sz)))
+        return bytes(buf.raw[: sz.value])
+
+    def restore(self, blob):
+        buf = C.create_string_buffer(blob, len(blob))
+        self._check(self.lib.gd_restore(self._ctx, buf, len(blob)))
+
+    def calc_energy(self, cid, idx_lig, idx_prt, movematrix):
+        idx_lig = np.ascontiguousarray(idx_lig, np.int32)
+        idx_prt = np.ascontiguousarray(idx_prt, np.int32)
+        mm = np.ascontiguousarray(movematrix, np.float32).reshape(-1, 6)
+        n = mm.shape[0]
+        out = np.zeros(n, abi.ENERGY_DTYPE)
+        self._check(self.lib.gd_calc_energy(self._ctx, cid, n, _i(idx_lig), _i(idx_prt), _f(mm),
+                                            out.ctypes.data))
+        return out
+
+    def accept(self, e_new, e_old, minus_beta, u):
+        a = [np.ascontiguousarray(v, np.float32) for v in (e_new, e_old, minus_beta, u)]
+        out = np.zeros(len(a[0]), np.int32)
+        self._check(self.lib.gd_accept(self._ctx, len(a[0]), _f(a[0]), _f(a[1]), _f(a[2]),
+                                       _f(a[3]), _i(out)))
+        return out
+
+    def exchange(self, n_tmp, mode_t, etot, minus_beta, u, idx_tmp, acs):
+        etot = np.ascontiguousarray(etot, np.float32)
+        mb = np.ascontiguousarray(minus_beta, np.float32)
+        u = np.ascontiguousarray(u, np.float32)
+        it = np.array(idx_tmp, np.int32, copy=True)
+        ac = np.array(acs, np.int32, copy=True)
+        n_groups = it.size // n_tmp
+        self._check(self.lib.gd_exchange(self._ctx, n_groups, n_tmp, mode_t, _f(etot), _f(mb),
+                                         _f(u), _i(it), _i(ac)))
+        return it, ac
+
+    def philox(self, ctr, key):
+        ctr = np.ascontiguousarray(ctr, np.uint32).reshape(-1, 4)
+        key = np.ascontiguousarray(key, np.uint32).reshape(-1, 2)
+        out = np.zeros_like(ctr)
+        self._check(self.lib.gd_philox(self._ctx, ctr.shape[0], _u(ctr), _u(key), _u(out)))
+        return out
+
+
+def run(cx, steps_total, steps_per_exchange=20, seed=0, steps_per_dump=2000, device=0):
+    """Run() for one complex: returns (records by replica, mclog).
+
+    mclog holds t0 (MC-kernel seconds), t1 (wall seconds), ar (accepted
+    records / (n_rep * steps_total)) and steps_total, as McLog does.
+    """
+    records = {}
+
+    def sink(cid, recs):
+        for r in np.unique(recs["replica"][:, 0]):
+            records.setdefault(int(r), []).append(recs[recs["replica"][:, 0] == r])
+
+    with Engine(seed=seed, device=device, steps_per_dump=steps_per_dump) as eng:
+        eng.add_complex(cx)
+        st = eng.run(steps_total, steps_per_exchange, sink=sink)
+    recs = {k: np.concatenate(v) for k, v in records.items()}
+    n_rec = sum(len(v) for v in recs.values())
+    mclog = dict(t0=st["t_compute"], t1=st["t_wall"], steps_total=steps_total,
+                 ar=n_rec / float(cx.n_rep * max(steps_total, 1)),
+                 mc_sweeps_per_s=steps_total * cx.n_rep / st["t_compute"] if st["t_compute"] else 0.0)
+    return recs, mclog

@@ -1,0 +1,989 @@
+// gd_engine.hip -- host side of the C ABI in include/geauxdock.h.
+//
+// Replaces Run() and its single-GPU launcher (src/run.cu:33-289,
+// src/kernel_cuda_lancher_siglegpu.cu:5-69): uploads complexes once, keeps
+// all replica state resident in HBM, and advances every replica of every
+// complex with one fused kernel launch per record window.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gd_mc.hip"
+
+using namespace gd;
+
+namespace {
+
+struct HostComplex {
+  gd_complex_desc d;  // sizes, scalars (pointers are not kept)
+  int n_rep = 0, rep_base = 0;
+  int lpr = 0, apl = 0, n_chunks = 0;
+  bool ref_built = false;
+  DevComplex dev{};
+  std::vector<void *> bufs;  // device allocations owned by this complex
+  size_t lds_bytes = 0;
+};
+
+}  // namespace
+
+struct gd_ctx {
+  gd_opts opts{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  int n_cu = 0;
+  size_t max_lds = 65536;
+  std::vector<HostComplex> cx;
+  DevComplex *d_cx = nullptr;
+  bool cx_dirty = true;
+  int *d_overflow = nullptr;
+  // replica state
+  bool state_alloc = false, initialized = false;
+  int n_rep = 0;
+  int64_t step = 0;
+  float *d_mm = nullptr;
+  gd_energy *d_eold = nullptr;
+  int *d_idx_tmp = nullptr, *d_acs_mc = nullptr, *d_acs_x = nullptr, *d_rep_flat = nullptr;
+  gd_record *d_rec = nullptr;
+  int *d_rec_cnt = nullptr;
+  int rec_cap = 0;
+  // packs of the MC launch
+  Pack *d_packs = nullptr;
+  int *d_slots = nullptr;
+  int n_packs = 0, block = 0;
+  size_t lds = 0;
+  KParams lay{};  // LDS layout of the MC launch
+};
+
+namespace {
+
+gd_status fail(gd_ctx *c, gd_status s, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return s;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(ctx, GD_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+  } while (0)
+
+template <class T>
+gd_status dev_upload(gd_ctx *ctx, HostComplex &h, T **dst, const T *src, size_t n) {
+  void *p = nullptr;
+  const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  HIPCHK(hipMalloc(&p, bytes));
+  h.bufs.push_back(p);
+  if (n) HIPCHK(hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  *dst = static_cast<T *>(p);
+  return GD_OK;
+}
+
+#define TRY(expr)                      \
+  do {                                 \
+    gd_status s_ = (expr);             \
+    if (s_ != GD_OK) return s_;        \
+  } while (0)
+
+// Lanes per replica and atoms per lane: the fewest lane slots lpr*apl that
+// hold lna atoms with apl <= 4, preferring 2..3 atoms per lane on ties (more
+// independent work per lane), with a whole ladder within 512 lanes.
+void choose_shape(int lna, int n_tmp, int force_lpr, int force_apl, int *lpr, int *apl) {
+  int best_l = 64, best_a = (lna + 63) / 64, best_cost = 1 << 30;
+  const int cands[4] = {8, 16, 32, 64};
+  for (int ci = 0; ci < 4; ++ci) {
+    const int l = cands[ci];
+    if (force_lpr && l != force_lpr) continue;
+    const int a = (lna + l - 1) / l;
+    if (a > 4 || (force_apl && a > force_apl)) continue;
+    const int aa = force_apl ? force_apl : a;
+    if (l * n_tmp > 512 && !force_lpr) continue;
+    const int cost = l * aa * 8 + std::abs(aa - 2);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best_l = l;
+      best_a = aa;
+    }
+  }
+  *lpr = best_l;
+  *apl = best_a;
+}
+
+size_t lds_need(const HostComplex &h, int slots) {
+  const gd_complex_desc &d = h.d;
+  size_t b = (size_t)d.pnp * 16 + (size_t)d.pnp * 8 + PAR_FLOATS * 4 + (size_t)d.pnk * 16 +
+             NLT * 8 + (size_t)h.n_chunks * 16 + (size_t)slots * 16 + 256;
+  return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gd_last_error(const gd_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+gd_status gd_create(gd_ctx **out, const gd_opts *opts) {
+  if (!out) return GD_ERR_INVALID;
+  *out = nullptr;
+  gd_ctx *ctx = new (std::nothrow) gd_ctx();
+  if (!ctx) return GD_ERR_NOMEM;
+  if (opts) ctx->opts = *opts;
+  if (ctx->opts.steps_per_dump <= 0) ctx->opts.steps_per_dump = 2000;
+  ctx->device = ctx->opts.device;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    *out = ctx;
+    return fail(ctx, GD_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  }
+  *out = ctx;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+  ctx->n_cu = prop.multiProcessorCount;
+  ctx->max_lds = prop.sharedMemPerBlock;
+  if (prop.maxSharedMemoryPerMultiProcessor > 0)
+    ctx->max_lds = std::max<size_t>(ctx->max_lds, std::min<size_t>(prop.maxSharedMemoryPerMultiProcessor, 160 * 1024));
+  if (ctx->opts.verbose)
+    fprintf(stderr, "[geauxdock] %s: %d CUs, LDS/block %zu, LDS/CU %zu, regs/block %d\n",
+            prop.gcnArchName, prop.multiProcessorCount, (size_t)prop.sharedMemPerBlock,
+            (size_t)prop.maxSharedMemoryPerMultiProcessor, prop.regsPerBlock);
+  HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&ctx->ev0));
+  HIPCHK(hipEventCreate(&ctx->ev1));
+  HIPCHK(hipMalloc(&ctx->d_overflow, sizeof(int)));
+  return GD_OK;
+}
+
+void gd_destroy(gd_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->stream) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+  }
+  for (auto &h : ctx->cx)
+    for (void *p : h.bufs) (void)hipFree(p);
+  void *ps[] = {ctx->d_cx, ctx->d_overflow, ctx->d_mm, ctx->d_eold, ctx->d_idx_tmp,
+                ctx->d_acs_mc, ctx->d_acs_x, ctx->d_rep_flat, ctx->d_rec, ctx->d_rec_cnt,
+                ctx->d_packs, ctx->d_slots};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int gd_num_replicas(const gd_ctx *ctx, int cid) {
+  if (!ctx || cid < 0 || cid >= (int)ctx->cx.size()) return -1;
+  return ctx->cx[cid].n_rep;
+}
+
+gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id) {
+  if (!ctx || !d) return GD_ERR_INVALID;
+  if (ctx->state_alloc) return fail(ctx, GD_ERR_STATE, "complexes must be added before gd_run");
+  if (d->n_lig <= 0 || d->n_prt <= 0 || d->n_tmp <= 0 || d->lna <= 0 || d->pnp <= 0 ||
+      d->pnk < 0 || d->pos < 0)
+    return fail(ctx, GD_ERR_INVALID, "non-positive size in complex description");
+  if (d->lna > GD_MAX_LNA) return fail(ctx, GD_ERR_CAPACITY, "lna %d > %d", d->lna, GD_MAX_LNA);
+  if (d->n_tmp > GD_MAX_NTMP)
+    return fail(ctx, GD_ERR_CAPACITY, "n_tmp %d > %d", d->n_tmp, GD_MAX_NTMP);
+  if (!d->lig_x || !d->lig_y || !d->lig_z || !d->lig_t || !d->lig_c || !d->lig_n || !d->prt_x ||
+      !d->prt_y || !d->prt_z || !d->prt_t || !d->prt_c || !d->prt_ele || !d->prt_seq3r ||
+      !d->prt_c0_and_d12_or_c2 || !d->prt_hpp || !d->enepara || !d->temp)
+    return fail(ctx, GD_ERR_INVALID, "null array in complex description");
+  if (d->pnk > 0 && (!d->kde_x || !d->kde_y || !d->kde_z || !d->kde_t))
+    return fail(ctx, GD_ERR_INVALID, "null KDE array");
+  if (d->pos > 0 && (!d->mcs_x || !d->mcs_y || !d->mcs_z || !d->mcs_tcc || d->n_mcs <= 0))
+    return fail(ctx, GD_ERR_INVALID, "null MCS array");
+  const int lna = d->lna, pnp = d->pnp, pnk = d->pnk, pos = d->pos;
+  for (int l = 0; l < lna; ++l) {
+    if (d->lig_t[l] < 0 || d->lig_t[l] >= NLT)
+      return fail(ctx, GD_ERR_INVALID, "ligand atom %d type %d out of range", l, d->lig_t[l]);
+    if (pos > 0 && (d->lig_n[l] + 1 < 0 || d->lig_n[l] + 1 >= d->n_mcs))
+      return fail(ctx, GD_ERR_INVALID, "ligand atom %d number %d outside MCS fields", l, d->lig_n[l]);
+  }
+  bool any_psp = false;
+  for (int p = 0; p < pnp; ++p) {
+    if (d->prt_t[p] < 0 || d->prt_t[p] >= NPT)
+      return fail(ctx, GD_ERR_INVALID, "receptor point %d type %d out of range", p, d->prt_t[p]);
+    if (d->prt_c[p] == 2) {
+      any_psp = true;
+      if (d->prt_seq3r[p] < 0 || d->prt_seq3r[p] >= d->n_res)
+        return fail(ctx, GD_ERR_INVALID, "receptor point %d seq3r %d outside [0,%d)", p,
+                    d->prt_seq3r[p], d->n_res);
+    }
+  }
+  if (any_psp && !d->psp) return fail(ctx, GD_ERR_INVALID, "null PSP table");
+  for (int k = 0; k < pnk; ++k)
+    if (d->kde_t[k] < 0 || d->kde_t[k] >= NLT)
+      return fail(ctx, GD_ERR_INVALID, "KDE point %d type out of range", k);
+
+  HIPCHK(hipSetDevice(ctx->device));
+  ctx->cx.emplace_back();
+  HostComplex &h = ctx->cx.back();
+  const int cid = (int)ctx->cx.size() - 1;
+  h.d = *d;
+  h.n_rep = d->n_lig * d->n_prt * d->n_tmp;
+  choose_shape(lna, d->n_tmp, ctx->opts.lanes_per_replica, ctx->opts.atoms_per_lane, &h.lpr,
+               &h.apl);
+  if (h.lpr * h.apl < lna || h.apl < 1 || h.apl > 4)
+    return fail(ctx, GD_ERR_INVALID, "no lane shape for lna %d (lpr %d apl %d)", lna, h.lpr, h.apl);
+
+  // receptor sorted by (type, kind); kind: 2 = class 2, 1 = hydrophobic flag
+  std::vector<int> kind(pnp), perm(pnp);
+  for (int p = 0; p < pnp; ++p) {
+    kind[p] = d->prt_c[p] == 2 ? KIND_PSP : (d->prt_c0_and_d12_or_c2[p] == 1 ? KIND_HPC : KIND_PLAIN);
+    perm[p] = p;
+  }
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
+    if (d->prt_t[a] != d->prt_t[b]) return d->prt_t[a] < d->prt_t[b];
+    return kind[a] < kind[b];
+  });
+  std::vector<float4> xyze((size_t)d->n_prt * pnp);
+  for (int c = 0; c < d->n_prt; ++c)
+    for (int i = 0; i < pnp; ++i) {
+      const int p = perm[i];
+      const size_t o = (size_t)c * pnp;
+      xyze[o + i] = make_float4(d->prt_x[o + p], d->prt_y[o + p], d->prt_z[o + p], d->prt_ele[p]);
+    }
+  std::vector<float> hpp(pnp);
+  std::vector<int> pspi(pnp);
+  for (int i = 0; i < pnp; ++i) {
+    const int p = perm[i];
+    hpp[i] = d->prt_c0_and_d12_or_c2[p] == 1 ? d->prt_hpp[p] : 0.0f;
+    pspi[i] = d->prt_c[p] == 2 ? d->prt_seq3r[p] : -1;
+  }
+  std::vector<int4> chunks;
+  for (int i = 0; i < pnp;) {
+    int j = i;
+    const int t = d->prt_t[perm[i]], k = kind[perm[i]];
+    while (j < pnp && d->prt_t[perm[j]] == t && kind[perm[j]] == k) ++j;
+    chunks.push_back(make_int4(i, j - i, t, k));
+    i = j;
+  }
+  h.n_chunks = (int)chunks.size();
+
+  // type-pair parameter records [receptor type][ligand type]
+  const gd_enepara &ep = *d->enepara;
+  std::vector<float> par(PAR_FLOATS, 0.0f);
+  const float hsc = sqrtf(0.5f * 1.44269504088896340736f);
+  const float sqrt_2_pi_m1 = -1.0f / sqrtf(2.0f * 3.1415926535f);
+  for (int pt = 0; pt < NPT; ++pt)
+    for (int lt = 0; lt < NLT; ++lt) {
+      float *r = &par[(pt * NLT + lt) * NPAR];
+      r[P_P1A] = ep.p1a[lt][pt];
+      r[P_P2A] = ep.p2a[lt][pt];
+      r[P_PMF0] = ep.pmf0[lt][pt];
+      r[P_PMF1] = ep.pmf1[lt][pt];
+      const float h0 = ep.hdb0[lt][pt], h1 = ep.hdb1[lt][pt];
+      const bool on = h0 > 0.1f;
+      r[P_HA] = on ? h1 * hsc : 0.0f;
+      r[P_HB] = on ? -h0 * h1 * hsc : 0.0f;
+      r[P_HC] = on ? sqrt_2_pi_m1 * h1 : 0.0f;
+    }
+
+  // ligand conformers [n_lig][3][lna]
+  std::vector<float> orig((size_t)d->n_lig * 3 * lna);
+  for (int c = 0; c < d->n_lig; ++c)
+    for (int l = 0; l < lna; ++l) {
+      const size_t o = (size_t)c * lna + l;
+      orig[(size_t)c * 3 * lna + l] = d->lig_x[o];
+      orig[(size_t)c * 3 * lna + lna + l] = d->lig_y[o];
+      orig[(size_t)c * 3 * lna + 2 * lna + l] = d->lig_z[o];
+    }
+  // MCS fields by atom: [pos][lna]
+  std::vector<float4> mcs((size_t)std::max(pos, 1) * lna);
+  std::vector<float> mcnt(std::max(pos, 1), 0.0f), tcc(std::max(pos, 1), 0.0f);
+  for (int m = 0; m < pos; ++m) {
+    tcc[m] = d->mcs_tcc[m];
+    for (int l = 0; l < lna; ++l) {
+      const size_t o = (size_t)m * d->n_mcs + d->lig_n[l] + 1;
+      const bool ok = d->mcs_x[o] != GD_MCS_INVALID_COORD;
+      mcs[(size_t)m * lna + l] = make_float4(d->mcs_x[o], d->mcs_y[o], d->mcs_z[o], ok ? 1.0f : 0.0f);
+      mcnt[m] += ok ? 1.0f : 0.0f;
+    }
+  }
+  // KDE sorted by type with per-type ranges
+  std::vector<int> kperm(pnk);
+  for (int k = 0; k < pnk; ++k) kperm[k] = k;
+  std::stable_sort(kperm.begin(), kperm.end(), [&](int a, int b) { return d->kde_t[a] < d->kde_t[b]; });
+  std::vector<float4> kde(std::max(pnk, 1));
+  std::vector<int2> krange(NLT, make_int2(0, 0));
+  for (int i = 0; i < pnk; ++i) {
+    const int k = kperm[i];
+    kde[i] = make_float4(d->kde_x[k], d->kde_y[k], d->kde_z[k], 0.0f);
+  }
+  for (int t = 0; t < NLT; ++t) {
+    int b = 0;
+    while (b < pnk && d->kde_t[kperm[b]] < t) ++b;
+    int e = b;
+    while (e < pnk && d->kde_t[kperm[e]] == t) ++e;
+    krange[t] = make_int2(b, e);
+  }
+
+  DevComplex &D = h.dev;
+  D.lna = lna;
+  D.pnp = pnp;
+  D.pnk = pnk;
+  D.pos = pos;
+  D.n_res = d->n_res;
+  D.n_lig = d->n_lig;
+  D.n_prt = d->n_prt;
+  D.n_tmp = d->n_tmp;
+  D.n_chunks = h.n_chunks;
+  D.lpr = h.lpr;
+  D.apl = h.apl;
+  D.move_kind = d->move_kind;
+  D.ref_total = 0;
+  D.cid = cid;
+  for (int i = 0; i < 3; ++i) D.pocket[i] = d->pocket_center[i];
+  for (int i = 0; i < 6; ++i) D.move_scale[i] = d->move_scale[i];
+  D.lj0 = ep.lj0;
+  D.lj01 = ep.lj0 * ep.lj1;
+  D.el0 = ep.el0;
+  D.el1 = ep.el1;
+  D.inv_el1 = 1.0f / ep.el1;
+  D.a1 = ep.a1;
+  D.b1 = ep.b1;
+  D.kde2l = ep.kde2 * 1.44269504088896340736f;
+  D.kde3 = ep.kde3;
+  for (int i = 0; i < GD_MAXWEI; ++i) {
+    D.w[i] = ep.w[i];
+    D.a_para[i] = ep.a_para[i];
+    D.b_para[i] = ep.b_para[i];
+  }
+  for (int i = 0; i < NLT; ++i) {
+    D.hpl0[i] = ep.hpl0[i];
+    D.hpl1[i] = ep.hpl1[i];
+    D.hpl2[i] = ep.hpl2[i];
+  }
+  for (int t = 0; t < GD_MAX_NTMP; ++t) D.minus_beta[t] = t < d->n_tmp ? d->temp[t].minus_beta : 0.0f;
+
+  const float *psp_src = d->psp;
+  std::vector<float> psp_zero;
+  size_t psp_n = (size_t)NLT * std::max(d->n_res, 1);
+  if (!psp_src) {
+    psp_zero.assign(psp_n, 0.0f);
+    psp_src = psp_zero.data();
+  }
+  TRY(dev_upload(ctx, h, (float4 **)&D.prt_xyze, xyze.data(), xyze.size()));
+  TRY(dev_upload(ctx, h, (float **)&D.prt_hpp, hpp.data(), hpp.size()));
+  TRY(dev_upload(ctx, h, (int **)&D.prt_psp, pspi.data(), pspi.size()));
+  TRY(dev_upload(ctx, h, (int4 **)&D.chunks, chunks.data(), chunks.size()));
+  TRY(dev_upload(ctx, h, (float **)&D.params, par.data(), par.size()));
+  TRY(dev_upload(ctx, h, (float **)&D.lig_orig, orig.data(), orig.size()));
+  TRY(dev_upload(ctx, h, (int **)&D.lig_t, d->lig_t, (size_t)lna));
+  TRY(dev_upload(ctx, h, (float **)&D.lig_c, d->lig_c, (size_t)lna));
+  TRY(dev_upload(ctx, h, (float4 **)&D.mcs, mcs.data(), mcs.size()));
+  TRY(dev_upload(ctx, h, (float **)&D.mcs_tcc, tcc.data(), tcc.size()));
+  TRY(dev_upload(ctx, h, (float **)&D.mcs_cnt, mcnt.data(), mcnt.size()));
+  TRY(dev_upload(ctx, h, (float4 **)&D.kde, kde.data(), kde.size()));
+  TRY(dev_upload(ctx, h, (int2 **)&D.kde_range, krange.data(), krange.size()));
+  TRY(dev_upload(ctx, h, (float **)&D.psp, psp_src, psp_n));
+  TRY(dev_upload<int2>(ctx, h, &D.ref, nullptr, 0));
+  {
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, (size_t)lna * KMAX * sizeof(int2)));
+    h.bufs.push_back(p);
+    D.ref = (int2 *)p;
+    HIPCHK(hipMalloc(&p, (size_t)lna * sizeof(int)));
+    h.bufs.push_back(p);
+    D.ref_cnt = (int *)p;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  h.lds_bytes = lds_need(h, 512 / h.lpr);
+  if (h.lds_bytes > ctx->max_lds)
+    return fail(ctx, GD_ERR_CAPACITY, "complex needs %zu bytes of LDS, device has %zu",
+                h.lds_bytes, ctx->max_lds);
+  ctx->cx_dirty = true;
+  if (complex_id) *complex_id = cid;
+  return GD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+gd_status sync_complexes(gd_ctx *ctx) {
+  if (!ctx->cx_dirty) return GD_OK;
+  if (ctx->d_cx) HIPCHK(hipFree(ctx->d_cx));
+  ctx->d_cx = nullptr;
+  std::vector<DevComplex> v;
+  for (auto &h : ctx->cx) v.push_back(h.dev);
+  HIPCHK(hipMalloc(&ctx->d_cx, std::max<size_t>(v.size(), 1) * sizeof(DevComplex)));
+  HIPCHK(hipMemcpyAsync(ctx->d_cx, v.data(), v.size() * sizeof(DevComplex), hipMemcpyHostToDevice,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->cx_dirty = false;
+  return GD_OK;
+}
+
+// Contact map of replica 0's start pose for each complex that lacks one.
+gd_status ensure_ref(gd_ctx *ctx) {
+  TRY(sync_complexes(ctx));
+  bool any = false;
+  for (int c = 0; c < (int)ctx->cx.size(); ++c) {
+    HostComplex &h = ctx->cx[c];
+    if (h.ref_built) continue;
+    HIPCHK(hipMemsetAsync(ctx->d_overflow, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(gd_ref_kernel, dim3((h.d.lna + 63) / 64), dim3(64), 0, ctx->stream,
+                       ctx->d_cx, c, ctx->d_overflow);
+    HIPCHK(hipGetLastError());
+    int ovf = 0;
+    std::vector<int> cnt(h.d.lna);
+    HIPCHK(hipMemcpyAsync(&ovf, ctx->d_overflow, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(cnt.data(), h.dev.ref_cnt, cnt.size() * sizeof(int), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ovf) return fail(ctx, GD_ERR_CAPACITY, "complex %d: more than %d reference contacts for one ligand atom", c, KMAX);
+    int tot = 0;
+    for (int v : cnt) tot += v;
+    h.dev.ref_total = tot;
+    h.ref_built = true;
+    any = true;
+  }
+  if (any) {
+    ctx->cx_dirty = true;
+    TRY(sync_complexes(ctx));
+  }
+  return GD_OK;
+}
+
+// LDS layout for a launch covering packs with these maxima.
+void layout(KParams &P, size_t *bytes, int pnp_max, int pnk_max, int nch_max, int slots_max) {
+  auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
+  size_t o = al((size_t)pnp_max * 16);
+  P.off_hpp = (int)o;
+  o = al(o + (size_t)pnp_max * 4);
+  P.off_psp = (int)o;
+  o = al(o + (size_t)pnp_max * 4);
+  P.off_par = (int)o;
+  o = al(o + (size_t)PAR_FLOATS * 4);
+  P.off_kde = (int)o;
+  o = al(o + (size_t)std::max(pnk_max, 1) * 16);
+  P.off_kr = (int)o;
+  o = al(o + NLT * 8);
+  P.off_ch = (int)o;
+  o = al(o + (size_t)std::max(nch_max, 1) * 16);
+  P.off_xe = (int)o;
+  o = al(o + (size_t)slots_max * 4);
+  P.off_xi = (int)o;
+  o = al(o + (size_t)slots_max * 4);
+  P.off_xa = (int)o;
+  o = al(o + (size_t)slots_max * 4);
+  P.off_xo = (int)o;
+  o = al(o + (size_t)slots_max * 4);
+  *bytes = o;
+}
+
+struct PackSet {
+  std::vector<Pack> packs;
+  std::vector<int> slots;
+  int block = 64, pnp_max = 0, pnk_max = 0, nch_max = 0, slots_max = 1;
+};
+
+void pack_finish(const gd_ctx *ctx, PackSet &ps) {
+  // longest packs first so the hardware dispatcher fills the CUs in LPT order
+  std::vector<int> order(ps.packs.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  auto cost = [&](const Pack &p) {
+    const HostComplex &h = ctx->cx[p.cx];
+    return (double)h.d.pnp * (double)((p.n_slots * h.lpr + 63) / 64) * h.apl;
+  };
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return cost(ps.packs[a]) > cost(ps.packs[b]); });
+  std::vector<Pack> sorted;
+  for (int i : order) sorted.push_back(ps.packs[i]);
+  ps.packs.swap(sorted);
+  for (const Pack &p : ps.packs) {
+    const HostComplex &h = ctx->cx[p.cx];
+    ps.block = std::max(ps.block, ((p.n_slots * h.lpr + 63) / 64) * 64);
+    ps.pnp_max = std::max(ps.pnp_max, h.d.pnp);
+    ps.pnk_max = std::max(ps.pnk_max, h.d.pnk);
+    ps.nch_max = std::max(ps.nch_max, h.n_chunks);
+    ps.slots_max = std::max(ps.slots_max, p.n_slots);
+  }
+  ps.pnp_max = (ps.pnp_max + 3) & ~3;
+}
+
+// MC packs: whole ladders of one (complex, protein conformer), about 256
+// lanes per workgroup and at most 512.
+PackSet build_mc_packs(const gd_ctx *ctx) {
+  PackSet ps;
+  for (int c = 0; c < (int)ctx->cx.size(); ++c) {
+    const HostComplex &h = ctx->cx[c];
+    const int n_tmp = h.d.n_tmp, n_lig = h.d.n_lig;
+    const int glanes = n_tmp * h.lpr;
+    int G = std::max(1, 256 / glanes);
+    while (G > 1 && G * glanes > 512) --G;
+    for (int p = 0; p < h.d.n_prt; ++p)
+      for (int l0 = 0; l0 < n_lig; l0 += G) {
+        const int ng = std::min(G, n_lig - l0);
+        Pack pk;
+        pk.cx = c;
+        pk.prt = p;
+        pk.n_groups = ng;
+        pk.n_slots = ng * n_tmp;
+        pk.slot_off = (int)ps.slots.size();
+        for (int g = 0; g < ng; ++g)
+          for (int t = 0; t < n_tmp; ++t)
+            ps.slots.push_back(h.rep_base + n_tmp * n_lig * p + n_lig * t + (l0 + g));
+        ps.packs.push_back(pk);
+      }
+  }
+  pack_finish(ctx, ps);
+  return ps;
+}
+
+gd_status alloc_state(gd_ctx *ctx) {
+  if (ctx->state_alloc) return GD_OK;
+  TRY(ensure_ref(ctx));
+  int n = 0;
+  for (auto &h : ctx->cx) {
+    h.rep_base = n;
+    n += h.n_rep;
+  }
+  if (n == 0) return fail(ctx, GD_ERR_STATE, "no complexes added");
+  ctx->n_rep = n;
+  HIPCHK(hipMalloc(&ctx->d_mm, (size_t)n * 6 * sizeof(float)));
+  HIPCHK(hipMalloc(&ctx->d_eold, (size_t)n * sizeof(gd_energy)));
+  HIPCHK(hipMalloc(&ctx->d_idx_tmp, (size_t)n * sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_acs_mc, (size_t)n * sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_acs_x, (size_t)n * sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_rep_flat, (size_t)n * sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_rec_cnt, (size_t)n * sizeof(int)));
+  std::vector<int> flat(n), itmp(n);
+  for (auto &h : ctx->cx)
+    for (int f = 0; f < h.n_rep; ++f) {
+      flat[h.rep_base + f] = f;
+      itmp[h.rep_base + f] = (f / h.d.n_lig) % h.d.n_tmp;  // SetReplica, src/util.C:846-852
+    }
+  HIPCHK(hipMemcpyAsync(ctx->d_rep_flat, flat.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->d_idx_tmp, itmp.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_mm, 0, (size_t)n * 6 * sizeof(float), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_eold, 0, (size_t)n * sizeof(gd_energy), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_acs_mc, 0, (size_t)n * sizeof(int), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_acs_x, 0, (size_t)n * sizeof(int), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_rec_cnt, 0, (size_t)n * sizeof(int), ctx->stream));
+  if (ctx->opts.record) {
+    ctx->rec_cap = ctx->opts.steps_per_dump + 1;
+    HIPCHK(hipMalloc(&ctx->d_rec, (size_t)n * ctx->rec_cap * sizeof(gd_record)));
+  }
+  PackSet ps = build_mc_packs(ctx);
+  HIPCHK(hipMalloc(&ctx->d_packs, ps.packs.size() * sizeof(Pack)));
+  HIPCHK(hipMalloc(&ctx->d_slots, ps.slots.size() * sizeof(int)));
+  HIPCHK(hipMemcpyAsync(ctx->d_packs, ps.packs.data(), ps.packs.size() * sizeof(Pack),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->d_slots, ps.slots.data(), ps.slots.size() * sizeof(int),
+                        hipMemcpyHostToDevice, ctx->stream));
+  ctx->n_packs = (int)ps.packs.size();
+  ctx->block = ps.block;
+  layout(ctx->lay, &ctx->lds, ps.pnp_max, ps.pnk_max, ps.nch_max, ps.slots_max);
+  if (ctx->lds > ctx->max_lds)
+    return fail(ctx, GD_ERR_CAPACITY, "launch needs %zu bytes of LDS, device has %zu", ctx->lds,
+                ctx->max_lds);
+  if (ctx->block > 512)
+    return fail(ctx, GD_ERR_CAPACITY, "a temperature ladder needs %d lanes (> 512)", ctx->block);
+  (void)hipFuncSetAttribute((const void *)gd_mc_kernel,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->lds);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->state_alloc = true;
+  if (ctx->opts.verbose)
+    fprintf(stderr, "[geauxdock] %d replicas, %d packs, block %d, LDS %zu B\n", n, ctx->n_packs,
+            ctx->block, ctx->lds);
+  return GD_OK;
+}
+
+KParams base_params(gd_ctx *ctx) {
+  KParams P = ctx->lay;
+  P.cxs = ctx->d_cx;
+  P.packs = ctx->d_packs;
+  P.slots = ctx->d_slots;
+  P.mm = ctx->d_mm;
+  P.e_old = ctx->d_eold;
+  P.idx_tmp = ctx->d_idx_tmp;
+  P.acs_mc = ctx->d_acs_mc;
+  P.acs_x = ctx->d_acs_x;
+  P.rep_flat = ctx->d_rep_flat;
+  P.rec = ctx->d_rec;
+  P.rec_cnt = ctx->d_rec_cnt;
+  P.rec_cap = ctx->rec_cap;
+  P.k0 = (uint32_t)ctx->opts.seed;
+  P.k1 = (uint32_t)(ctx->opts.seed >> 32);
+  return P;
+}
+
+gd_status drain_records(gd_ctx *ctx, gd_record_sink sink, void *user) {
+  if (!ctx->d_rec) return GD_OK;
+  const int n = ctx->n_rep;
+  std::vector<int> cnt(n);
+  HIPCHK(hipMemcpyAsync(cnt.data(), ctx->d_rec_cnt, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (sink) {
+    std::vector<int64_t> off(n + 1, 0);
+    for (int i = 0; i < n; ++i) off[i + 1] = off[i] + cnt[i];
+    const int64_t tot = off[n];
+    if (tot > 0) {
+      int64_t *d_off = nullptr;
+      gd_record *d_out = nullptr;
+      HIPCHK(hipMalloc(&d_off, (n + 1) * sizeof(int64_t)));
+      HIPCHK(hipMalloc(&d_out, tot * sizeof(gd_record)));
+      HIPCHK(hipMemcpyAsync(d_off, off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+      hipLaunchKernelGGL(gd_compact_kernel, dim3(n), dim3(256), 0, ctx->stream, n, ctx->d_rec,
+                         ctx->d_rec_cnt, d_off, ctx->rec_cap, d_out);
+      std::vector<gd_record> host(tot);
+      HIPCHK(hipMemcpyAsync(host.data(), d_out, tot * sizeof(gd_record), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      (void)hipFree(d_off);
+      (void)hipFree(d_out);
+      for (int c = 0; c < (int)ctx->cx.size(); ++c) {
+        const HostComplex &h = ctx->cx[c];
+        const int64_t b = off[h.rep_base], e = off[h.rep_base + h.n_rep];
+        if (e > b) sink(user, c, host.data() + b, e - b);
+      }
+    }
+  }
+  HIPCHK(hipMemsetAsync(ctx->d_rec_cnt, 0, n * sizeof(int), ctx->stream));
+  return GD_OK;
+}
+
+gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float *ms) {
+  KParams P = base_params(ctx);
+  P.mode = mode;
+  P.step0 = step0;
+  P.n_steps = n_steps;
+  P.nc = nc;
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  hipLaunchKernelGGL(gd_mc_kernel, dim3(ctx->n_packs), dim3(ctx->block), ctx->lds, ctx->stream, P);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->ev1));
+  if (ms) HIPCHK(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+  return GD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gd_status gd_run(gd_ctx *ctx, int64_t n_steps, int steps_per_exchange, gd_record_sink sink,
+                 void *user, gd_run_stats *stats) {
+  if (!ctx) return GD_ERR_INVALID;
+  if (n_steps < 0 || steps_per_exchange < 0) return fail(ctx, GD_ERR_INVALID, "negative step count");
+  if (ctx->step + n_steps > 2000000000LL) return fail(ctx, GD_ERR_CAPACITY, "step counter overflow");
+  HIPCHK(hipSetDevice(ctx->device));
+  const auto t_start = std::chrono::steady_clock::now();
+  TRY(alloc_state(ctx));
+  gd_run_stats st{};
+  std::vector<int> acs0(ctx->n_rep), acx0(ctx->n_rep);
+  HIPCHK(hipMemcpyAsync(acs0.data(), ctx->d_acs_mc, ctx->n_rep * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(acx0.data(), ctx->d_acs_x, ctx->n_rep * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  if (!ctx->initialized) {
+    float ms = 0.f;
+    TRY(launch_mc(ctx, MODE_INIT, (int)ctx->step, 0, 0, &ms));
+    ctx->initialized = true;
+  }
+  int64_t left = n_steps;
+  double t_ms = 0.0;
+  while (left > 0) {
+    const int win = (int)std::min<int64_t>(left, ctx->opts.steps_per_dump);
+    float ms = 0.f;
+    TRY(launch_mc(ctx, MODE_MC, (int)ctx->step, win, steps_per_exchange, &ms));
+    t_ms += ms;
+    st.kernel_launches += 1;
+    ctx->step += win;
+    left -= win;
+    TRY(drain_records(ctx, sink, user));
+  }
+  if (n_steps == 0) TRY(drain_records(ctx, sink, user));
+  std::vector<int> acs1(ctx->n_rep), acx1(ctx->n_rep);
+  HIPCHK(hipMemcpyAsync(acs1.data(), ctx->d_acs_mc, ctx->n_rep * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(acx1.data(), ctx->d_acs_x, ctx->n_rep * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < ctx->n_rep; ++i) {
+    st.accepted += acs1[i] - acs0[i];
+    st.exchanges_accepted += acx1[i] - acx0[i];
+  }
+  st.t_compute = t_ms * 1e-3;
+  st.steps = n_steps;
+  st.replica_steps = n_steps * (int64_t)ctx->n_rep;
+  st.kernel_ms_avg = st.kernel_launches ? t_ms / st.kernel_launches : 0.0;
+  st.t_wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  if (stats) *stats = st;
+  return GD_OK;
+}
+
+gd_status gd_get_state(gd_ctx *ctx, int cid, float *movematrix, gd_energy *energy_old,
+                       gd_replica *replica, int32_t *acs_mc, int32_t *acs_x) {
+  if (!ctx || cid < 0 || cid >= (int)ctx->cx.size()) return GD_ERR_INVALID;
+  if (!ctx->state_alloc) return fail(ctx, GD_ERR_STATE, "no state before gd_run");
+  HIPCHK(hipSetDevice(ctx->device));
+  const HostComplex &h = ctx->cx[cid];
+  const size_t b = h.rep_base, n = h.n_rep;
+  if (movematrix)
+    HIPCHK(hipMemcpyAsync(movematrix, ctx->d_mm + b * 6, n * 6 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  if (energy_old)
+    HIPCHK(hipMemcpyAsync(energy_old, ctx->d_eold + b, n * sizeof(gd_energy), hipMemcpyDeviceToHost, ctx->stream));
+  if (acs_mc) HIPCHK(hipMemcpyAsync(acs_mc, ctx->d_acs_mc + b, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  if (acs_x) HIPCHK(hipMemcpyAsync(acs_x, ctx->d_acs_x + b, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<int> it(n);
+  HIPCHK(hipMemcpyAsync(it.data(), ctx->d_idx_tmp + b, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (replica) {
+    for (size_t f = 0; f < n; ++f) {
+      replica[f].idx_rep = (int)f;
+      replica[f].idx_prt = (int)(f / (h.d.n_tmp * h.d.n_lig));
+      replica[f].idx_tmp = it[f];
+      replica[f].idx_lig = (int)(f % h.d.n_lig);
+    }
+  }
+  return GD_OK;
+}
+
+static const uint32_t kCkptMagic = 0x47444b31u;  // "GDK1"
+
+gd_status gd_checkpoint(gd_ctx *ctx, void *buf, size_t *size) {
+  if (!ctx || !size) return GD_ERR_INVALID;
+  if (!ctx->state_alloc) return fail(ctx, GD_ERR_STATE, "no state before gd_run");
+  const size_t n = ctx->n_rep;
+  const size_t need = 32 + n * (6 * sizeof(float) + sizeof(gd_energy) + 3 * sizeof(int));
+  if (!buf || *size < need) {
+    *size = need;
+    return buf ? fail(ctx, GD_ERR_INVALID, "checkpoint buffer too small") : GD_OK;
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  char *o = (char *)buf;
+  uint32_t hdr[8] = {kCkptMagic, (uint32_t)n, (uint32_t)ctx->initialized, 0,
+                     (uint32_t)(ctx->step & 0xffffffffu), (uint32_t)(ctx->step >> 32),
+                     (uint32_t)ctx->cx.size(), 0};
+  memcpy(o, hdr, 32);
+  o += 32;
+  HIPCHK(hipMemcpyAsync(o, ctx->d_mm, n * 6 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  o += n * 6 * sizeof(float);
+  HIPCHK(hipMemcpyAsync(o, ctx->d_eold, n * sizeof(gd_energy), hipMemcpyDeviceToHost, ctx->stream));
+  o += n * sizeof(gd_energy);
+  HIPCHK(hipMemcpyAsync(o, ctx->d_idx_tmp, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  o += n * sizeof(int);
+  HIPCHK(hipMemcpyAsync(o, ctx->d_acs_mc, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  o += n * sizeof(int);
+  HIPCHK(hipMemcpyAsync(o, ctx->d_acs_x, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *size = need;
+  return GD_OK;
+}
+
+gd_status gd_restore(gd_ctx *ctx, const void *buf, size_t size) {
+  if (!ctx || !buf) return GD_ERR_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  TRY(alloc_state(ctx));
+  const size_t n = ctx->n_rep;
+  const size_t need = 32 + n * (6 * sizeof(float) + sizeof(gd_energy) + 3 * sizeof(int));
+  uint32_t hdr[8];
+  if (size < need) return fail(ctx, GD_ERR_INVALID, "checkpoint too small");
+  memcpy(hdr, buf, 32);
+  if (hdr[0] != kCkptMagic || hdr[1] != n || hdr[6] != ctx->cx.size())
+    return fail(ctx, GD_ERR_INVALID, "checkpoint does not match this context");
+  const char *o = (const char *)buf + 32;
+  HIPCHK(hipMemcpyAsync(ctx->d_mm, o, n * 6 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  o += n * 6 * sizeof(float);
+  HIPCHK(hipMemcpyAsync(ctx->d_eold, o, n * sizeof(gd_energy), hipMemcpyHostToDevice, ctx->stream));
+  o += n * sizeof(gd_energy);
+  HIPCHK(hipMemcpyAsync(ctx->d_idx_tmp, o, n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  o += n * sizeof(int);
+  HIPCHK(hipMemcpyAsync(ctx->d_acs_mc, o, n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  o += n * sizeof(int);
+  HIPCHK(hipMemcpyAsync(ctx->d_acs_x, o, n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_rec_cnt, 0, n * sizeof(int), ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->initialized = hdr[2] != 0;
+  ctx->step = (int64_t)hdr[4] | ((int64_t)hdr[5] << 32);
+  return GD_OK;
+}
+
+gd_status gd_calc_energy(gd_ctx *ctx, int cid, int n, const int *idx_lig, const int *idx_prt,
+                         const float *movematrix, gd_energy *out) {
+  if (!ctx || cid < 0 || cid >= (int)ctx->cx.size() || n < 0) return GD_ERR_INVALID;
+  if (n == 0) return GD_OK;
+  if (!idx_lig || !idx_prt || !movematrix || !out) return GD_ERR_INVALID;
+  HostComplex &h = ctx->cx[cid];
+  for (int i = 0; i < n; ++i)
+    if (idx_lig[i] < 0 || idx_lig[i] >= h.d.n_lig || idx_prt[i] < 0 || idx_prt[i] >= h.d.n_prt)
+      return fail(ctx, GD_ERR_INVALID, "pose %d: conformer index out of range", i);
+  HIPCHK(hipSetDevice(ctx->device));
+  TRY(ensure_ref(ctx));
+  PackSet ps;
+  const int per = std::max(1, 256 / h.lpr);
+  for (int p = 0; p < h.d.n_prt; ++p) {
+    std::vector<int> mine;
+    for (int i = 0; i < n; ++i)
+      if (idx_prt[i] == p) mine.push_back(i);
+    for (size_t b = 0; b < mine.size(); b += per) {
+      Pack pk;
+      pk.cx = cid;
+      pk.prt = p;
+      pk.n_slots = (int)std::min<size_t>(per, mine.size() - b);
+      pk.n_groups = 0;
+      pk.slot_off = (int)ps.slots.size();
+      for (int j = 0; j < pk.n_slots; ++j) ps.slots.push_back(mine[b + j]);
+      ps.packs.push_back(pk);
+    }
+  }
+  pack_finish(ctx, ps);
+  KParams P{};
+  size_t lds = 0;
+  layout(P, &lds, ps.pnp_max, ps.pnk_max, ps.nch_max, ps.slots_max);
+  if (lds > ctx->max_lds) return fail(ctx, GD_ERR_CAPACITY, "LDS %zu > %zu", lds, ctx->max_lds);
+  Pack *d_packs = nullptr;
+  int *d_slots = nullptr, *d_lig = nullptr;
+  float *d_mm = nullptr;
+  gd_energy *d_out = nullptr;
+  HIPCHK(hipMalloc(&d_packs, ps.packs.size() * sizeof(Pack)));
+  HIPCHK(hipMalloc(&d_slots, ps.slots.size() * sizeof(int)));
+  HIPCHK(hipMalloc(&d_lig, n * sizeof(int)));
+  HIPCHK(hipMalloc(&d_mm, (size_t)n * 6 * sizeof(float)));
+  HIPCHK(hipMalloc(&d_out, (size_t)n * sizeof(gd_energy)));
+  HIPCHK(hipMemcpyAsync(d_packs, ps.packs.data(), ps.packs.size() * sizeof(Pack), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(d_slots, ps.slots.data(), ps.slots.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(d_lig, idx_lig, n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(d_mm, movematrix, (size_t)n * 6 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  P.cxs = ctx->d_cx;
+  P.packs = d_packs;
+  P.slots = d_slots;
+  P.ev_mm = d_mm;
+  P.ev_lig = d_lig;
+  P.ev_out = d_out;
+  P.mode = MODE_EVAL;
+  if (ps.block > 512) return fail(ctx, GD_ERR_CAPACITY, "pose pack of %d lanes (> 512)", ps.block);
+  (void)hipFuncSetAttribute((const void *)gd_mc_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(gd_mc_kernel, dim3(ps.packs.size()), dim3(ps.block), lds, ctx->stream, P);
+  hipError_t le = hipGetLastError();
+  if (le == hipSuccess)
+    le = hipMemcpyAsync(out, d_out, (size_t)n * sizeof(gd_energy), hipMemcpyDeviceToHost, ctx->stream);
+  if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d_packs);
+  (void)hipFree(d_slots);
+  (void)hipFree(d_lig);
+  (void)hipFree(d_mm);
+  (void)hipFree(d_out);
+  if (ctx->state_alloc)
+    (void)hipFuncSetAttribute((const void *)gd_mc_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->lds);
+  if (le != hipSuccess) return fail(ctx, GD_ERR_HIP, "energy evaluation: %s", hipGetErrorString(le));
+  return GD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+template <class T>
+gd_status tmp_dev(gd_ctx *ctx, std::vector<void *> &keep, T **p, const T *src, size_t n) {
+  HIPCHK(hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
+  keep.push_back(*p);
+  if (src && n) HIPCHK(hipMemcpyAsync(*p, src, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return GD_OK;
+}
+
+void free_all(std::vector<void *> &keep) {
+  for (void *p : keep) (void)hipFree(p);
+  keep.clear();
+}
+}  // namespace
+
+extern "C" {
+
+gd_status gd_accept(gd_ctx *ctx, int n, const float *e_new, const float *e_old,
+                    const float *minus_beta, const float *u, int32_t *accept) {
+  if (!ctx || n < 0) return GD_ERR_INVALID;
+  if (n == 0) return GD_OK;
+  if (!e_new || !e_old || !minus_beta || !u || !accept) return GD_ERR_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<void *> keep;
+  float *a, *b, *c, *d;
+  int *r;
+  gd_status s = GD_OK;
+  if ((s = tmp_dev(ctx, keep, &a, e_new, n)) || (s = tmp_dev(ctx, keep, &b, e_old, n)) ||
+      (s = tmp_dev(ctx, keep, &c, minus_beta, n)) || (s = tmp_dev(ctx, keep, &d, u, n)) ||
+      (s = tmp_dev<int>(ctx, keep, &r, nullptr, n))) {
+    free_all(keep);
+    return s;
+  }
+  hipLaunchKernelGGL(gd_accept_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, a, b, c, d, r);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(accept, r, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  free_all(keep);
+  if (e != hipSuccess) return fail(ctx, GD_ERR_HIP, "gd_accept: %s", hipGetErrorString(e));
+  return GD_OK;
+}
+
+gd_status gd_exchange(gd_ctx *ctx, int n_groups, int n_tmp, int mode_t, const float *etot,
+                      const float *minus_beta, const float *u, int32_t *idx_tmp, int32_t *acs) {
+  if (!ctx || n_groups < 0 || n_tmp <= 0) return GD_ERR_INVALID;
+  if (n_tmp > GD_MAX_NTMP) return fail(ctx, GD_ERR_CAPACITY, "n_tmp %d > %d", n_tmp, GD_MAX_NTMP);
+  if (mode_t != 0 && mode_t != 1) return fail(ctx, GD_ERR_INVALID, "mode_t must be 0 or 1");
+  if (n_groups == 0) return GD_OK;
+  if (!etot || !minus_beta || !u || !idx_tmp || !acs) return GD_ERR_INVALID;
+  const size_t ns = (size_t)n_groups * n_tmp;
+  for (size_t i = 0; i < ns; ++i)
+    if (idx_tmp[i] < 0 || idx_tmp[i] >= n_tmp)
+      return fail(ctx, GD_ERR_INVALID, "idx_tmp[%zu] = %d outside [0,%d)", i, idx_tmp[i], n_tmp);
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<void *> keep;
+  float *de, *dm, *du;
+  int *di, *da;
+  gd_status s = GD_OK;
+  const size_t nu = (size_t)n_groups * (n_tmp / 2);
+  if ((s = tmp_dev(ctx, keep, &de, etot, ns)) || (s = tmp_dev(ctx, keep, &dm, minus_beta, (size_t)n_tmp)) ||
+      (s = tmp_dev(ctx, keep, &du, u, nu)) || (s = tmp_dev<int>(ctx, keep, &di, idx_tmp, ns)) ||
+      (s = tmp_dev<int>(ctx, keep, &da, acs, ns))) {
+    free_all(keep);
+    return s;
+  }
+  hipLaunchKernelGGL(gd_exchange_kernel, dim3((n_groups + 63) / 64), dim3(64), 0, ctx->stream,
+                     n_groups, n_tmp, mode_t, de, dm, du, di, da);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(idx_tmp, di, ns * sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(acs, da, ns * sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  free_all(keep);
+  if (e != hipSuccess) return fail(ctx, GD_ERR_HIP, "gd_exchange: %s", hipGetErrorString(e));
+  return GD_OK;
+}
+
+gd_status gd_philox(gd_ctx *ctx, int n, const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+  if (!ctx || n < 0) return GD_ERR_INVALID;
+  if (n == 0) return GD_OK;
+  if (!ctr || !key || !out) return GD_ERR_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<void *> keep;
+  uint32_t *dc, *dk, *dout;
+  gd_status s = GD_OK;
+  if ((s = tmp_dev(ctx, keep, &dc, ctr, (size_t)4 * n)) || (s = tmp_dev(ctx, keep, &dk, key, (size_t)2 * n)) ||
+      (s = tmp_dev<uint32_t>(ctx, keep, &dout, nullptr, (size_t)4 * n))) {
+    free_all(keep);
+    return s;
+  }
+  hipLaunchKernelGGL(gd_philox_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, dc, dk, dout);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, (size_t)4 * n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  free_all(keep);
+  if (e != hipSuccess) return fail(ctx, GD_ERR_HIP, "gd_philox: %s", hipGetErrorString(e));
+  return GD_OK;
+}
+
+}  // extern "C"

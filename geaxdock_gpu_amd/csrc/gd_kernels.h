@@ -1,0 +1,90 @@
+// gd_kernels.h -- data layout shared by the host side and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/geauxdock.h"
+
+namespace gd {
+
+constexpr int NLT = GD_MAXTP2;            // ligand atom types
+constexpr int NPT = GD_MAXTP1;            // receptor point types
+constexpr int NPAR = 8;                   // per (receptor type, ligand type) record
+constexpr int PAR_FLOATS = NPT * NLT * NPAR;
+constexpr int KMAX = GD_MAX_REFCONTACT;
+
+// per (receptor type, ligand type) parameter record
+enum { P_P1A = 0, P_P2A, P_PMF0, P_PMF1, P_HA, P_HB, P_HC, P_PAD };
+
+// Receptor point kinds after sorting: which optional terms a point can
+// contribute. 0: neither, 1: hydrophobic contact (c0_and_d12_or_c2), 2: also
+// pocket-specific potential (class 2).
+enum { KIND_PLAIN = 0, KIND_HPC = 1, KIND_PSP = 2 };
+
+// One complex on the device. Receptor points are stored sorted by
+// (type, kind) so each chunk has one parameter record per ligand atom.
+struct DevComplex {
+  int lna, pnp, pnk, pos, n_res, n_lig, n_prt, n_tmp;
+  int n_chunks, lpr, apl, move_kind;
+  int ref_total;  // contacts of the reference contact map
+  int cid;        // complex id (random stream id)
+  float pocket[3];
+  float move_scale[6];
+  float lj0, lj01, el0, el1, inv_el1, a1, b1, kde2l, kde3;
+  float w[GD_MAXWEI], a_para[GD_MAXWEI], b_para[GD_MAXWEI];
+  float hpl0[NLT], hpl1[NLT], hpl2[NLT];
+  float minus_beta[GD_MAX_NTMP];
+  const float4 *prt_xyze;  // [n_prt * pnp] x, y, z, ele (sorted order)
+  const float *prt_hpp;    // [pnp] hpp where c0_and_d12_or_c2, else 0
+  const int *prt_psp;      // [pnp] seq3r where class 2, else -1
+  const int4 *chunks;      // [n_chunks] start, length, type, kind
+  const float *params;     // [NPT * NLT * NPAR]
+  const float *lig_orig;   // [n_lig][3][lna]
+  const int *lig_t;        // [lna]
+  const float *lig_c;      // [lna]
+  const float4 *mcs;       // [pos * lna] x, y, z, valid
+  const float *mcs_tcc;    // [pos]
+  const float *mcs_cnt;    // [pos] valid atoms per template
+  const float4 *kde;       // [pnk] sorted by type
+  const int2 *kde_range;   // [NLT] begin, end
+  const float *psp;        // [NLT * n_res]
+  int2 *ref;               // [lna * KMAX] sorted point index, pmf0 bits
+  int *ref_cnt;            // [lna]
+};
+
+// One workgroup's work: replicas (or poses) of one complex that share one
+// protein conformer; slots are ordered ladder by ladder, temperature slot
+// fastest, so slots [g*n_tmp, (g+1)*n_tmp) form exchange ladder g.
+struct Pack {
+  int cx, prt, n_slots, n_groups, slot_off;
+};
+
+enum { MODE_MC = 0, MODE_INIT = 1, MODE_EVAL = 2 };
+
+struct KParams {
+  const DevComplex *cxs;
+  const Pack *packs;
+  const int *slots;
+  // replica state (MODE_MC / MODE_INIT), indexed by global replica id
+  float *mm;
+  gd_energy *e_old;
+  int *idx_tmp;
+  int *acs_mc;
+  int *acs_x;
+  const int *rep_flat;
+  gd_record *rec;
+  int *rec_cnt;
+  int rec_cap;
+  // poses (MODE_EVAL), indexed by slot value
+  const float *ev_mm;
+  const int *ev_lig;
+  gd_energy *ev_out;
+  int mode;
+  int step0, n_steps, nc;
+  uint32_t k0, k1;
+  // LDS layout (bytes)
+  int off_hpp, off_psp, off_par, off_kde, off_kr, off_ch, off_xe, off_xi, off_xa, off_xo;
+};
+
+}  // namespace gd

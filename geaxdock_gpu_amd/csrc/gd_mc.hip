@@ -1,0 +1,608 @@
+// gd_mc.hip -- the MC step kernels for CDNA4 (gfx950).
+//
+// One workgroup owns a pack: whole temperature ladders of one complex that
+// share one protein conformer. The receptor conformer, the type-pair
+// parameter records and the KDE points are staged into LDS once per launch;
+// each replica is `lpr` lanes of one wave (8..64), each lane owns `APL`
+// ligand atoms, and every replica-level sum is a butterfly over its lanes, so
+// the MC step needs no workgroup barrier. Barriers occur only at replica-
+// exchange points, where one thread per ladder performs the swap sweep.
+//
+// Reference step order (src/kernel_cuda_l1_montecarlo.cu:95-119): Move_d,
+// CalcRmsd_d, CalcMcc_d, CalcEnergy_d (+CombineEnergy_d), Accept_d,
+// RecordLigand_d; ExchangeReplicas_d between segments
+// (src/kernel_cuda_lancher_siglegpu.cu:25-31).
+#include "gd_device.h"
+#include "gd_kernels.h"
+
+namespace gd {
+
+struct Lds {
+  const float4 *xyze;
+  const float *hpp;
+  const int *psp;
+  const float *par;
+  const float4 *kde;
+  const int2 *kr;
+  const int4 *ch;
+  float *xe;
+  int *xi;
+  int *xa;
+  int *xo;
+};
+
+__device__ __forceinline__ float seg_sum(float v, int lpr) {
+  for (int m = 1; m < lpr; m <<= 1) v += __shfl_xor(v, m);
+  return v;
+}
+__device__ __forceinline__ int seg_sum_i(int v, int lpr) {
+  for (int m = 1; m < lpr; m <<= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+// HPC switching polynomial in d^2 (src/kernel_cuda_l2_calcenergy.cu:116-121):
+// 1 - (3.5 x - 4.5 x^2 + 2.5 x^3 - 0.5 x^4), x = d^2 / 81.
+constexpr float H1 = -3.5f / 81.0f;
+constexpr float H2 = 4.5f / (81.0f * 81.0f);
+constexpr float H3 = -2.5f / (81.0f * 81.0f * 81.0f);
+constexpr float H4 = 0.5f / (81.0f * 81.0f * 81.0f * 81.0f);
+constexpr float LOG2E = 1.44269504088896340736f;
+
+struct PairConst {
+  float lj0, lj01, el0, el1, inv_el1, a1, b1;
+};
+
+// Receptor points [beg, end) of one chunk against the lane's APL atoms.
+// All terms of src/kernel_cuda_l2_calcenergy.cu:105-184 plus the contact
+// count of CalcMcc_d:
+//   vdW  (p1 - p2) / (p1 lj0 (1 + lj1 d^2) + 1), p1 = p1a/d^9, p2 = p2a/d^6,
+//        evaluated multiplied through by d^9: (p1a - p2a d^3)/(p1a t + d^9);
+//   ele  ele_p * g(el1 d) (the ligand charge is applied per atom later);
+//   pmf  pmf1 / (1 + exp((6 - d/2)(d - pmf0)));
+//   hdb  -hdb1/sqrt(2 pi) exp(-((d - hdb0) hdb1)^2 / 2) (zero record if hdb0 <= 0.1);
+//   hpc  hpp * switch(d^2) for d^2 <= 81 (KIND >= 1);
+//   psp  psp[lig_t][seq3r] when class 2 and d <= pmf0 (KIND == 2);
+//   contacts  d <= pmf0.
+template <int APL, int KIND>
+__device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, const PairConst &K,
+                                            const float (&x)[APL], const float (&y)[APL],
+                                            const float (&z)[APL], const float (&prm)[APL][7],
+                                            const float *const (&psprow)[APL],
+                                            float (&vdw)[APL], float (&ele)[APL],
+                                            float (&pmf)[APL], float (&psp)[APL],
+                                            float (&hdb)[APL], float (&hpc)[APL],
+                                            int (&cnt)[APL]) {
+  const float m05l = -0.5f * LOG2E, c6l = 6.0f * LOG2E;
+#pragma unroll 2
+  for (int p = beg; p < end; ++p) {
+    const float4 P = S.xyze[p];
+    float hp = 0.0f;
+    int ps = 0;
+    if (KIND >= KIND_HPC) hp = S.hpp[p];
+    if (KIND == KIND_PSP) ps = S.psp[p];
+#pragma unroll
+    for (int k = 0; k < APL; ++k) {
+      const float d2 = pair_d2(x[k], y[k], z[k], P.x, P.y, P.z);
+      const float rd = fast_rsq(d2);
+      const float d = d2 * rd;
+      // soft Lennard-Jones
+      const float d3 = d2 * d;
+      const float d6 = d3 * d3;
+      const float d9 = d6 * d3;
+      const float num = fmaf(-prm[k][P_P2A], d3, prm[k][P_P1A]);
+      const float t = fmaf(K.lj01, d2, K.lj0);
+      const float den = fmaf(prm[k][P_P1A], t, d9);
+      vdw[k] = fmaf(num, fast_rcp(den), vdw[k]);
+      // soft electrostatics
+      const float s = K.el1 * d;
+      const float gn = fmaf(s * s, fmaf(K.b1, s, K.a1), K.el0);
+      const float gf = rd * K.inv_el1;
+      ele[k] = fmaf(P.w, (s < 1.0f) ? gn : gf, ele[k]);
+      // contact PMF
+      const float v = d - prm[k][P_PMF0];
+      const float wv = fmaf(m05l, d, c6l) * v;
+      pmf[k] = fmaf(prm[k][P_PMF1], fast_rcp(1.0f + fast_exp2(wv)), pmf[k]);
+      // hydrogen bond
+      const float h = fmaf(d, prm[k][P_HA], prm[k][P_HB]);
+      hdb[k] = fmaf(prm[k][P_HC], fast_exp2(-(h * h)), hdb[k]);
+      // contacts for MCC
+      const bool c = v <= 0.0f;
+      cnt[k] += c ? 1 : 0;
+      if (KIND >= KIND_HPC) {
+        const float poly = fmaf(d2, fmaf(d2, fmaf(d2, fmaf(d2, H4, H3), H2), H1), 1.0f);
+        hpc[k] += (d2 <= 81.0f) ? hp * poly : 0.0f;
+      }
+      if (KIND == KIND_PSP) {
+        if (c) psp[k] += psprow[k][ps];
+      }
+    }
+  }
+}
+
+// Energy, RMSD and MCC of one pose held by `lpr` lanes (CalcEnergy_d +
+// CombineEnergy_d + CalcRmsd_d + CalcMcc_d). Every lane of the replica ends
+// with the same bit pattern in E.
+template <int APL>
+__device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, int lpr, const float (&x)[APL],
+                            const float (&y)[APL], const float (&z)[APL],
+                            const float (&ox)[APL], const float (&oy)[APL],
+                            const float (&oz)[APL], const int (&lt)[APL],
+                            const float (&q)[APL], const int (&li)[APL],
+                            const bool (&val)[APL], const float mm[6], gd_energy &E) {
+  const int n_chunks = __builtin_amdgcn_readfirstlane(C.n_chunks);
+  PairConst K;
+  K.lj0 = C.lj0;
+  K.lj01 = C.lj01;
+  K.el0 = C.el0;
+  K.el1 = C.el1;
+  K.inv_el1 = C.inv_el1;
+  K.a1 = C.a1;
+  K.b1 = C.b1;
+
+  float vdw[APL], ele[APL], pmf[APL], psp[APL], hdb[APL], hpc[APL];
+  int cnt[APL];
+  const float *psprow[APL];
+#pragma unroll
+  for (int k = 0; k < APL; ++k) {
+    vdw[k] = ele[k] = pmf[k] = psp[k] = hdb[k] = hpc[k] = 0.0f;
+    cnt[k] = 0;
+    psprow[k] = C.psp + (size_t)lt[k] * C.n_res;
+  }
+
+  for (int ci = 0; ci < n_chunks; ++ci) {
+    const int4 ch = S.ch[ci];
+    const int beg = __builtin_amdgcn_readfirstlane(ch.x);
+    const int end = beg + __builtin_amdgcn_readfirstlane(ch.y);
+    const int type = __builtin_amdgcn_readfirstlane(ch.z);
+    const int kind = __builtin_amdgcn_readfirstlane(ch.w);
+    float prm[APL][7];
+#pragma unroll
+    for (int k = 0; k < APL; ++k) {
+      const float *r = S.par + (type * NLT + lt[k]) * NPAR;
+      const float4 a = *reinterpret_cast<const float4 *>(r);
+      const float4 b = *reinterpret_cast<const float4 *>(r + 4);
+      prm[k][0] = a.x;
+      prm[k][1] = a.y;
+      prm[k][2] = a.z;
+      prm[k][3] = a.w;
+      prm[k][4] = b.x;
+      prm[k][5] = b.y;
+      prm[k][6] = b.z;
+    }
+    if (kind == KIND_PLAIN)
+      chunk_pairs<APL, KIND_PLAIN>(S, beg, end, K, x, y, z, prm, psprow, vdw, ele, pmf, psp, hdb,
+                                   hpc, cnt);
+    else if (kind == KIND_HPC)
+      chunk_pairs<APL, KIND_HPC>(S, beg, end, K, x, y, z, prm, psprow, vdw, ele, pmf, psp, hdb,
+                                 hpc, cnt);
+    else
+      chunk_pairs<APL, KIND_PSP>(S, beg, end, K, x, y, z, prm, psprow, vdw, ele, pmf, psp, hdb,
+                                 hpc, cnt);
+  }
+
+  // per-atom terms
+  float t_vdw = 0.f, t_ele = 0.f, t_pmf = 0.f, t_psp = 0.f, t_hdb = 0.f, t_hpc = 0.f;
+  float t_kde = 0.f, t_rms = 0.f;
+  int t_cnt = 0, t_tp = 0;
+  const float kde2l = C.kde2l;
+#pragma unroll
+  for (int k = 0; k < APL; ++k) {
+    if (!val[k]) continue;
+    t_vdw += vdw[k];
+    t_ele += q[k] * ele[k];
+    t_pmf += pmf[k];
+    t_psp += psp[k];
+    t_hdb += hdb[k];
+    // hydrophobic restraint, src/kernel_cuda_l2_calcenergy.cu:200-205
+    const float h2 = (hpc[k] - C.hpl0[lt[k]]) / C.hpl1[lt[k]];
+    t_hpc += 0.5f * h2 * h2 - C.hpl2[lt[k]];
+    t_cnt += cnt[k];
+    // KDE: mean over same-type points, src/kernel_cuda_l2_calcenergy.cu:224-253
+    const int2 kr = S.kr[lt[k]];
+    float ks = 0.0f;
+    for (int j = kr.x; j < kr.y; ++j) {
+      const float4 Q = S.kde[j];
+      ks += fast_exp2(kde2l * pair_d2(x[k], y[k], z[k], Q.x, Q.y, Q.z));
+    }
+    if (kr.y > kr.x) t_kde += ks / (float)(kr.y - kr.x);
+    // true positives against the reference contact list
+    const int nr = C.ref_cnt[li[k]];
+    const int2 *rl = C.ref + (size_t)li[k] * KMAX;
+    for (int j = 0; j < nr; ++j) {
+      const int2 r = rl[j];
+      const float4 P = S.xyze[r.x];
+      const float d2 = pair_d2(x[k], y[k], z[k], P.x, P.y, P.z);
+      const float d = d2 * fast_rsq(d2);
+      t_tp += (d - __int_as_float(r.y) <= 0.0f) ? 1 : 0;
+    }
+    // RMSD against the start pose
+    const float dx = x[k] - (ox[k] + C.pocket[0]);
+    const float dy = y[k] - (oy[k] + C.pocket[1]);
+    const float dz = z[k] - (oz[k] + C.pocket[2]);
+    t_rms += dx * dx + dy * dy + dz * dz;
+  }
+
+  // MCS restraints, src/kernel_cuda_l2_calcenergy.cu:274-313
+  float elhm = 0.0f;
+  const int pos = C.pos;
+  for (int m = 0; m < pos; ++m) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < APL; ++k) {
+      if (!val[k]) continue;
+      const float4 M = C.mcs[(size_t)m * C.lna + li[k]];
+      if (M.w != 0.0f) s += pair_d2(x[k], y[k], z[k], M.x, M.y, M.z);
+    }
+    s = seg_sum(s, lpr);
+    const float n = C.mcs_cnt[m];
+    if (n > 0.0f) elhm += C.mcs_tcc[m] * sqrtf(s / n);
+  }
+
+  t_vdw = seg_sum(t_vdw, lpr);
+  t_ele = seg_sum(t_ele, lpr);
+  t_pmf = seg_sum(t_pmf, lpr);
+  t_psp = seg_sum(t_psp, lpr);
+  t_hdb = seg_sum(t_hdb, lpr);
+  t_hpc = seg_sum(t_hpc, lpr);
+  t_kde = seg_sum(t_kde, lpr);
+  t_rms = seg_sum(t_rms, lpr);
+  t_cnt = seg_sum_i(t_cnt, lpr);
+  t_tp = seg_sum_i(t_tp, lpr);
+
+  // src/kernel_cuda_l2_calcenergy.cu:334-359, src/kernel_cuda_l3_combineenergy.cu:28-37
+  const float lna = (float)C.lna;
+  float e[GD_MAXWEI];
+  e[0] = t_vdw / lna;
+  e[1] = t_ele / lna;
+  e[2] = t_pmf / lna;
+  e[3] = t_psp / lna;
+  e[4] = t_hdb / lna;
+  e[5] = t_hpc / lna;
+  e[6] = (t_kde / C.kde3) / lna;
+  e[7] = pos > 0 ? logf(elhm / (float)pos) : 0.0f;
+  {
+    const float cx = C.pocket[0] + mm[0], cy = C.pocket[1] + mm[1], cz = C.pocket[2] + mm[2];
+    const float dx = cx - C.pocket[0], dy = cy - C.pocket[1], dz = cz - C.pocket[2];
+    e[8] = sqrtf(dx * dx + dy * dy + dz * dz);
+  }
+  float total = 0.0f;
+#pragma unroll
+  for (int i = 0; i < GD_MAXWEI - 1; ++i) {
+    e[i] = C.a_para[i] * e[i] + C.b_para[i];
+    E.e[i] = e[i];
+    total += e[i] * C.w[i];
+  }
+  E.e[GD_MAXWEI - 1] = total;
+  E.rmsd = sqrtf(t_rms / lna);
+  // Matthews correlation of the contact maps, src/kernel_cuda_l2_calcmcc.cu:194-206
+  const int tp = t_tp, fp = t_cnt - t_tp, fn = C.ref_total - t_tp;
+  const int tn = C.lna * C.pnp - C.ref_total - fp;
+  const float tp0 = (float)tp, fn0 = (float)fn, fp0 = (float)fp, tn0 = (float)tn;
+  const float dv = sqrtf((tp0 + fp0) * (tp0 + fn0) * (tn0 + fp0) * (tn0 + fn0));
+  E.cms = dv != 0.0f ? (tp0 * tn0 - fp0 * fn0) / dv : 0.0f;
+}
+
+// ExchangeReplicas_d for one ladder (src/kernel_cuda_l1_exchangereplicas.cu:33-78):
+// temperature i1 = 2 pair + mode_t and i1 + 1 trade slots when
+// exp((mb1 - mb2)(E2 - E1)) > U.
+template <class U>
+__device__ __forceinline__ void exchange_ladder(int n_tmp, int mode_t, const float *et, const float *mb,
+                                int *temps, int *acs, int *order, U &&uni) {
+  for (int t = 0; t < n_tmp; ++t) order[temps[t]] = t;
+  const int maxpair = n_tmp / 2 - ((n_tmp % 2 == 0) && (mode_t == 1));
+  for (int pair = 0; pair < maxpair; ++pair) {
+    const int i1 = (pair << 1) + mode_t;
+    const int i2 = i1 + 1;
+    const int o1 = order[i1];
+    const int o2 = order[i2];
+    const int tt = temps[o1];
+    const float e1 = et[o1], e2 = et[o2];
+    const float mb1 = mb[temps[o1]], mb2 = mb[temps[o2]];
+    const float d1 = mb1 - mb2;
+    const float d2 = e2 - e1;
+    const float delta = d1 * d2;
+    const float prob = decision_expf(delta);
+    if (prob > uni(pair)) {
+      temps[o1] = temps[o2];
+      temps[o2] = tt;
+      acs[o1] += 1;
+      acs[o2] += 1;
+    }
+  }
+}
+
+template <int APL>
+__device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const DevComplex &C, const Lds &S,
+                         int lpr) {
+  const int tid = threadIdx.x;
+  const int slot = tid / lpr;
+  const int a = tid - slot * lpr;
+  const int n_slots = pk.n_slots;
+  const bool active = slot < n_slots;
+  const int sl = active ? slot : 0;
+  const int sv = P.slots[pk.slot_off + sl];  // replica id or pose index
+  const int lna = C.lna;
+  const int n_lig = C.n_lig;
+
+  int flat = 0, ilig;
+  float mo[6];
+  gd_energy eo;
+  if (P.mode == MODE_EVAL) {
+    ilig = P.ev_lig[sv];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) mo[i] = P.ev_mm[(size_t)sv * 6 + i];
+  } else {
+    flat = P.rep_flat[sv];
+    ilig = flat % n_lig;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) mo[i] = P.mm[(size_t)sv * 6 + i];
+    eo = P.e_old[sv];
+  }
+
+  // the lane's atoms: l = a + k*lpr
+  float ox[APL], oy[APL], oz[APL], q[APL];
+  int lt[APL], li[APL];
+  bool val[APL];
+  const float *orig = C.lig_orig + (size_t)ilig * 3 * lna;
+#pragma unroll
+  for (int k = 0; k < APL; ++k) {
+    const int l = a + k * lpr;
+    val[k] = l < lna;
+    const int lc = val[k] ? l : 0;
+    li[k] = lc;
+    ox[k] = orig[lc];
+    oy[k] = orig[lna + lc];
+    oz[k] = orig[2 * lna + lc];
+    lt[k] = C.lig_t[lc];
+    q[k] = C.lig_c[lc];
+  }
+
+  float x[APL], y[APL], z[APL];
+  auto place = [&](const float m[6]) {
+    const Rot R = euler(m[3], m[4], m[5]);
+#pragma unroll
+    for (int k = 0; k < APL; ++k) {
+      x[k] = R.r00 * ox[k] + R.r01 * oy[k] + R.r02 * oz[k] + m[0] + C.pocket[0];
+      y[k] = R.r10 * ox[k] + R.r11 * oy[k] + R.r12 * oz[k] + m[1] + C.pocket[1];
+      z[k] = R.r20 * ox[k] + R.r21 * oy[k] + R.r22 * oz[k] + m[2] + C.pocket[2];
+    }
+  };
+
+  if (P.mode != MODE_MC) {
+    place(mo);
+    gd_energy E;
+    pose_energy<APL>(C, S, lpr, x, y, z, ox, oy, oz, lt, q, li, val, mo, E);
+    if (active && a == 0) {
+      if (P.mode == MODE_EVAL) {
+        P.ev_out[sv] = E;
+      } else {
+        P.e_old[sv] = E;
+        if (P.rec) {
+          const int rc = P.rec_cnt[sv];
+          if (rc < P.rec_cap) {
+            gd_record &r = P.rec[(size_t)sv * P.rec_cap + rc];
+            r.replica.idx_rep = flat;
+            r.replica.idx_prt = pk.prt;
+            r.replica.idx_tmp = P.idx_tmp[sv];
+            r.replica.idx_lig = ilig;
+            r.energy = E;
+            for (int i = 0; i < 6; ++i) r.movematrix[i] = mo[i];
+            r.step = P.step0;
+            P.rec_cnt[sv] = rc + 1;
+          }
+        }
+      }
+    }
+    return;
+  }
+
+  // ---- MC steps ------------------------------------------------------------
+  const int cid = C.cid;
+  const int nc = P.nc;
+  float mb = C.minus_beta[S.xi[sl]];
+  int acs = 0;
+  int rc = P.rec ? P.rec_cnt[sv] : 0;
+  const bool wave_active = __ballot(active) != 0;
+  float scale[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) scale[i] = C.move_scale[i];
+
+  for (int s = P.step0; s < P.step0 + P.n_steps; ++s) {
+    if (wave_active) {
+      float u[8], dm[6], mn[6];
+      mc_uniforms(P.k0, P.k1, cid, flat, s, u);
+      move_offsets(u, scale, C.move_kind, dm);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) mn[i] = mo[i] + dm[i];
+      place(mn);
+      gd_energy E;
+      pose_energy<APL>(C, S, lpr, x, y, z, ox, oy, oz, lt, q, li, val, mn, E);
+      if (metropolis(E.e[GD_MAXWEI - 1], eo.e[GD_MAXWEI - 1], mb, u[6])) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) mo[i] = mn[i];
+        eo = E;
+        acs += 1;
+        if (P.rec && active) {
+          if (a == 0 && rc < P.rec_cap) {
+            gd_record &r = P.rec[(size_t)sv * P.rec_cap + rc];
+            r.replica.idx_rep = flat;
+            r.replica.idx_prt = pk.prt;
+            r.replica.idx_tmp = S.xi[sl];
+            r.replica.idx_lig = ilig;
+            r.energy = E;
+            for (int i = 0; i < 6; ++i) r.movematrix[i] = mo[i];
+            r.step = s;
+          }
+          rc += 1;
+        }
+      }
+    }
+    if (nc > 0 && ((s + 1) % nc) == 0) {
+      if (active && a == 0) S.xe[slot] = eo.e[GD_MAXWEI - 1];
+      __syncthreads();
+      if (tid < pk.n_groups) {
+        const int n_tmp = C.n_tmp;
+        const int seg = (s + 1) / nc - 1;
+        const int mode_t = (seg % 2 == 0) ? 1 : 0;
+        const int f0 = P.rep_flat[P.slots[pk.slot_off + tid * n_tmp]];
+        const int group = (f0 / (n_tmp * n_lig)) * n_lig + (f0 % n_lig);
+        const int b = tid * n_tmp;
+        exchange_ladder(n_tmp, mode_t, S.xe + b, C.minus_beta, S.xi + b, S.xa + b, S.xo + b,
+                        [&](int pair) {
+                          return exchange_uniform(P.k0, P.k1, cid, group, seg, pair);
+                        });
+      }
+      __syncthreads();
+      mb = C.minus_beta[S.xi[sl]];
+    }
+  }
+
+  if (active && a == 0) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) P.mm[(size_t)sv * 6 + i] = mo[i];
+    P.e_old[sv] = eo;
+    P.idx_tmp[sv] = S.xi[slot];
+    P.acs_mc[sv] += acs;
+    P.acs_x[sv] = S.xa[slot];
+    if (P.rec) P.rec_cnt[sv] = rc < P.rec_cap ? rc : P.rec_cap;
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(512) gd_mc_kernel(KParams P) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  const Pack pk = P.packs[blockIdx.x];
+  const DevComplex &C = P.cxs[pk.cx];
+  Lds S;
+  S.xyze = reinterpret_cast<const float4 *>(lds);
+  S.hpp = reinterpret_cast<const float *>(lds + P.off_hpp);
+  S.psp = reinterpret_cast<const int *>(lds + P.off_psp);
+  S.par = reinterpret_cast<const float *>(lds + P.off_par);
+  S.kde = reinterpret_cast<const float4 *>(lds + P.off_kde);
+  S.kr = reinterpret_cast<const int2 *>(lds + P.off_kr);
+  S.ch = reinterpret_cast<const int4 *>(lds + P.off_ch);
+  S.xe = reinterpret_cast<float *>(lds + P.off_xe);
+  S.xi = reinterpret_cast<int *>(lds + P.off_xi);
+  S.xa = reinterpret_cast<int *>(lds + P.off_xa);
+  S.xo = reinterpret_cast<int *>(lds + P.off_xo);
+
+  // stage the pack's receptor conformer and tables
+  const int pnp = C.pnp;
+  const int nt = blockDim.x;
+  {
+    float4 *xyze = reinterpret_cast<float4 *>(lds);
+    float *hpp = reinterpret_cast<float *>(lds + P.off_hpp);
+    int *psp = reinterpret_cast<int *>(lds + P.off_psp);
+    float *par = reinterpret_cast<float *>(lds + P.off_par);
+    float4 *kde = reinterpret_cast<float4 *>(lds + P.off_kde);
+    int2 *kr = reinterpret_cast<int2 *>(lds + P.off_kr);
+    int4 *ch = reinterpret_cast<int4 *>(lds + P.off_ch);
+    const float4 *src = C.prt_xyze + (size_t)pk.prt * pnp;
+    for (int i = threadIdx.x; i < pnp; i += nt) {
+      xyze[i] = src[i];
+      hpp[i] = C.prt_hpp[i];
+      psp[i] = C.prt_psp[i];
+    }
+    for (int i = threadIdx.x; i < PAR_FLOATS; i += nt) par[i] = C.params[i];
+    for (int i = threadIdx.x; i < C.pnk; i += nt) kde[i] = C.kde[i];
+    for (int i = threadIdx.x; i < NLT; i += nt) kr[i] = C.kde_range[i];
+    for (int i = threadIdx.x; i < C.n_chunks; i += nt) ch[i] = C.chunks[i];
+    if (P.mode == MODE_MC) {
+      for (int i = threadIdx.x; i < pk.n_slots; i += nt) {
+        const int g = P.slots[pk.slot_off + i];
+        S.xi[i] = P.idx_tmp[g];
+        S.xa[i] = P.acs_x[g];
+      }
+    }
+  }
+  __syncthreads();
+
+  const int lpr = C.lpr;
+  switch (C.apl) {
+    case 1: run_pack<1>(P, pk, C, S, lpr); break;
+    case 2: run_pack<2>(P, pk, C, S, lpr); break;
+    case 3: run_pack<3>(P, pk, C, S, lpr); break;
+    default: run_pack<4>(P, pk, C, S, lpr); break;
+  }
+}
+
+// Reference contact map of a complex (InitRefMatrix_d,
+// src/kernel_cuda_l2_calcmcc.cu:57-83): contacts of replica 0 (ligand
+// conformer 0, protein conformer 0) in its start pose, as per-atom lists of
+// sorted receptor indices. One thread per ligand atom; run before any MCC.
+extern "C" __global__ void gd_ref_kernel(DevComplex *cxs, int cxi, int *overflow) {
+  DevComplex &C = cxs[cxi];
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= C.lna) return;
+  const int lna = C.lna;
+  const float m[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const Rot R = euler(m[3], m[4], m[5]);
+  const float ox = C.lig_orig[l], oy = C.lig_orig[lna + l], oz = C.lig_orig[2 * lna + l];
+  const float x = R.r00 * ox + R.r01 * oy + R.r02 * oz + m[0] + C.pocket[0];
+  const float y = R.r10 * ox + R.r11 * oy + R.r12 * oz + m[1] + C.pocket[1];
+  const float z = R.r20 * ox + R.r21 * oy + R.r22 * oz + m[2] + C.pocket[2];
+  const int lt = C.lig_t[l];
+  int n = 0;
+  for (int ci = 0; ci < C.n_chunks; ++ci) {
+    const int4 ch = C.chunks[ci];
+    const float pmf0 = C.params[(ch.z * NLT + lt) * NPAR + P_PMF0];
+    for (int p = ch.x; p < ch.x + ch.y; ++p) {
+      const float4 P = C.prt_xyze[p];
+      const float d2 = pair_d2(x, y, z, P.x, P.y, P.z);
+      const float d = d2 * fast_rsq(d2);
+      if (d - pmf0 <= 0.0f) {
+        if (n < KMAX) C.ref[(size_t)l * KMAX + n] = make_int2(p, __float_as_int(pmf0));
+        ++n;
+      }
+    }
+  }
+  if (n > KMAX) {
+    atomicAdd(overflow, 1);
+    n = KMAX;
+  }
+  C.ref_cnt[l] = n;
+}
+
+extern "C" __global__ void gd_accept_kernel(int n, const float *e_new, const float *e_old,
+                                            const float *mb, const float *u, int *acc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) acc[i] = metropolis(e_new[i], e_old[i], mb[i], u[i]) ? 1 : 0;
+}
+
+extern "C" __global__ void gd_exchange_kernel(int n_groups, int n_tmp, int mode_t,
+                                              const float *etot, const float *mb,
+                                              const float *u, int *idx_tmp, int *acs) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_groups) return;
+  int order[GD_MAX_NTMP];
+  const float *ug = u + (size_t)g * (n_tmp / 2);
+  exchange_ladder(n_tmp, mode_t, etot + (size_t)g * n_tmp, mb, idx_tmp + (size_t)g * n_tmp,
+                  acs + (size_t)g * n_tmp, order, [&](int pair) { return ug[pair]; });
+}
+
+extern "C" __global__ void gd_philox_kernel(int n, const uint32_t *ctr, const uint32_t *key,
+                                            uint32_t *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u32x4 r = philox4x32_10({ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3]},
+                                key[2 * i], key[2 * i + 1]);
+  out[4 * i] = r.x;
+  out[4 * i + 1] = r.y;
+  out[4 * i + 2] = r.z;
+  out[4 * i + 3] = r.w;
+}
+
+// Gathers each replica's window of records into one dense buffer
+// (replica order, then step), ahead of one device-to-host copy.
+extern "C" __global__ void gd_compact_kernel(int n_rep, const gd_record *rec, const int *cnt,
+                                             const int64_t *off, int cap, gd_record *out) {
+  const int r = blockIdx.x;
+  if (r >= n_rep) return;
+  const int n = cnt[r];
+  const int64_t o = off[r];
+  const int *src = reinterpret_cast<const int *>(rec + (size_t)r * cap);
+  int *dst = reinterpret_cast<int *>(out + o);
+  const int words = n * (int)(sizeof(gd_record) / 4);
+  for (int i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+}
+
+}  // namespace gd

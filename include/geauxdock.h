@@ -1,0 +1,249 @@
+/*
+ * geauxdock.h -- C ABI of the MI355X replica-exchange Monte Carlo docking engine.
+ *
+ * The reference (GeauxDock, geaux-computational-bio/geaxdock_gpu) exposes its
+ * MC hot path through one C++ function,
+ *     void Run(const Ligand*, const Protein*, const Psp*, const Kde*,
+ *              const Mcs*, const EnePara*, const Temp*, const Replica*,
+ *              const McPara*, McLog*, map<int, vector<LigRecordSingleStep>>&,
+ *              const ComplexSize)                      (src/run.h:10-22)
+ * that uploads the structures, runs the launch loop of
+ * src/kernel_cuda_lancher_siglegpu.cu:5-69 and returns accepted-step records.
+ * This header replaces that boundary with plain C entry points: sizes are
+ * runtime values (no MAXLIG/MAXPRO arrays), every call returns a status code
+ * instead of exit(), device state lives in an opaque context instead of
+ * __constant__ symbols, and many complexes can share one batched launch.
+ *
+ * Data layouts that must match the reference's meaning keep its field names
+ * and order: gd_enepara = EnePara (src/dock_soa.h:99-133), gd_temp = Temp
+ * (src/dock.h:288-293), gd_replica = Replica (src/dock.h:302-309), gd_energy =
+ * Energy (src/dock.h:108-128), gd_record = LigRecordSingleStep
+ * (src/dock.h:361-367).
+ */
+#ifndef GEAUXDOCK_H
+#define GEAUXDOCK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Type-table dimensions (src/size.h:95-113). */
+#define GD_MAXTP1 30 /* receptor point types */
+#define GD_MAXTP2 24 /* ligand atom types */
+#define GD_MAXTP3 50 /* electrostatic point classes */
+#define GD_MAXTP4 20 /* residue types */
+#define GD_MAXWEI 10 /* energy terms incl. total */
+/* MCS field value marking "atom absent in this template" (src/size.h:142-143). */
+#define GD_MCS_INVALID_COORD 728492.0f
+
+/* Engine capacity limits (runtime checks; each violation returns GD_ERR_CAPACITY). */
+#define GD_MAX_LNA 256     /* ligand atoms per complex */
+#define GD_MAX_NTMP 64     /* temperatures per ladder */
+#define GD_MAX_REFCONTACT 256 /* reference contacts per ligand atom (MCC map) */
+
+typedef enum {
+  GD_OK = 0,
+  GD_ERR_INVALID = 1,  /* bad argument or inconsistent description */
+  GD_ERR_CAPACITY = 2, /* a size exceeds an engine limit */
+  GD_ERR_HIP = 3,      /* HIP runtime error (message in gd_last_error) */
+  GD_ERR_STATE = 4,    /* call not valid in the current context state */
+  GD_ERR_NOMEM = 5
+} gd_status;
+
+/* Energy parameters after OptimizeEnepara (src/util.C:717-768).
+ * Two-index tables are [ligand type][receptor type]. */
+typedef struct gd_enepara {
+  float p1a[GD_MAXTP2][GD_MAXTP1];
+  float p2a[GD_MAXTP2][GD_MAXTP1];
+  float lj0, lj1;
+  float el0;
+  float el1;
+  float ele[GD_MAXTP3];
+  float a1; /* 4 - 3*el0 */
+  float b1; /* 2*el0 - 3 */
+  float pmf0[GD_MAXTP2][GD_MAXTP1];
+  float pmf1[GD_MAXTP2][GD_MAXTP1];
+  float hdb0[GD_MAXTP2][GD_MAXTP1];
+  float hdb1[GD_MAXTP2][GD_MAXTP1];
+  float hpp[GD_MAXTP4];
+  float hpl0[GD_MAXTP2];
+  float hpl1[GD_MAXTP2];
+  float hpl2[GD_MAXTP2];
+  float kde2; /* -0.5 / bw^2 */
+  float kde3; /* (bw * sqrt(2 pi))^3 */
+  float w[GD_MAXWEI];
+  float a_para[GD_MAXWEI];
+  float b_para[GD_MAXWEI];
+} gd_enepara;
+
+typedef struct gd_temp {
+  float t;
+  float minus_beta; /* -1 / (k_B T), k_B = 1 */
+  int order;
+} gd_temp;
+
+typedef struct gd_replica {
+  int idx_rep; /* flat index n_tmp*n_lig*prt + n_lig*tmp + lig */
+  int idx_prt;
+  int idx_tmp; /* current temperature index (changes on exchange) */
+  int idx_lig;
+} gd_replica;
+
+/* e[0..8] hold the normalized, unweighted terms
+ * (vdw, ele, pmf, psp, hdb, hpc, kde, lhm, dst); e[9] the weighted total. */
+typedef struct gd_energy {
+  float e[GD_MAXWEI];
+  float cms;
+  float rmsd;
+} gd_energy;
+
+typedef struct gd_record {
+  gd_replica replica;
+  gd_energy energy;
+  float movematrix[6]; /* translation x y z, Euler angles x y z */
+  int step;
+} gd_record;
+
+/* Move proposal kinds. The reference draws scale*(2U-1) per DOF
+ * (src/kernel_cuda_l1_montecarlo.cu:100, src/kernel_cuda_l2_move.cu:27-30). */
+#define GD_MOVE_UNIFORM 0
+#define GD_MOVE_GAUSSIAN 1 /* scale * N(0,1) per DOF */
+
+/* One complex: every ligand conformer x protein conformer x temperature.
+ * All arrays are host pointers, copied by gd_add_complex.
+ * Per-point receptor attributes are shared by all protein conformers; only
+ * coordinates differ (conformers are models of one structure). */
+typedef struct gd_complex_desc {
+  int n_lig, n_prt, n_tmp; /* ComplexSize (src/dock.h:87-100) */
+  int lna, pnp, pnk, pos;
+  int n_res; /* PSP columns: residue sequence numbers seq3r lie in [0, n_res) */
+  int n_mcs; /* fields per MCS template, indexed by ligand atom number + 1 */
+
+  /* Ligand conformers in the ligand frame (Ligand.coord_orig, already centred). */
+  const float *lig_x, *lig_y, *lig_z; /* [n_lig * lna] */
+  const int *lig_t;                   /* [lna] atom type in [0, GD_MAXTP2) */
+  const float *lig_c;                 /* [lna] charge */
+  const int *lig_n;                   /* [lna] atom number; MCS index is n + 1 */
+  float pocket_center[3];
+
+  /* Protein conformers (Protein after OptimizeProtein, src/util.C:574-646). */
+  const float *prt_x, *prt_y, *prt_z; /* [n_prt * pnp] */
+  const int *prt_t;                   /* [pnp] point type in [0, GD_MAXTP1) */
+  const int *prt_c;                   /* [pnp] point class */
+  const float *prt_ele;               /* [pnp] electrostatic parameter */
+  const int *prt_seq3r;               /* [pnp] residue sequence number */
+  const int *prt_c0_and_d12_or_c2;    /* [pnp] hydrophobic-contact flag */
+  const float *prt_hpp;               /* [pnp] residue hydrophobicity */
+
+  const float *psp; /* [GD_MAXTP2 * n_res]: psp[lig_t * n_res + seq3r] */
+
+  const float *kde_x, *kde_y, *kde_z; /* [pnk] */
+  const int *kde_t;                   /* [pnk] */
+
+  const float *mcs_x, *mcs_y, *mcs_z; /* [pos * n_mcs] */
+  const float *mcs_tcc;               /* [pos] */
+
+  const gd_enepara *enepara;
+  const gd_temp *temp; /* [n_tmp] */
+
+  float move_scale[6]; /* McPara.move_scale: translation x y z, rotation x y z */
+  int move_kind;       /* GD_MOVE_UNIFORM or GD_MOVE_GAUSSIAN */
+} gd_complex_desc;
+
+/* Engine options. Zero-initialise, then set what you need. */
+typedef struct gd_opts {
+  int device;          /* HIP device ordinal */
+  uint64_t seed;       /* Philox4x32-10 key; the whole run is a function of it */
+  int steps_per_dump;  /* accepted-state records are drained every this many steps (0 -> 2000) */
+  int record;          /* 1 = keep accepted-state records (src/kernel_cuda_l3_util.cu:73-108) */
+  int lanes_per_replica; /* 0 = auto; else 8, 16, 32 or 64 */
+  int atoms_per_lane;    /* 0 = auto; else 1..4 */
+  int verbose;
+} gd_opts;
+
+typedef struct gd_run_stats {
+  double t_compute; /* seconds in MC kernels (HIP events), the reference's McLog.t0 */
+  double t_wall;    /* seconds including record transfer, McLog.t1 */
+  int64_t steps;    /* MC steps this call */
+  int64_t replica_steps;
+  int64_t accepted;          /* accepted MC moves this call */
+  int64_t exchanges_accepted;
+  int kernel_launches;
+  double kernel_ms_avg;      /* mean MC-kernel duration (HIP events) */
+} gd_run_stats;
+
+/* Record sink: called once per dump window and complex with that window's
+ * accepted-state records, ordered by replica then step. */
+typedef void (*gd_record_sink)(void *user, int complex_id, const gd_record *recs, int64_t n);
+
+typedef struct gd_ctx gd_ctx;
+
+/* Create a context on opts->device. Replaces the set-up half of Run()
+ * (src/run.cu:57-134). */
+gd_status gd_create(gd_ctx **out, const gd_opts *opts);
+void gd_destroy(gd_ctx *ctx);
+const char *gd_last_error(const gd_ctx *ctx);
+
+/* Copy one complex to the device (src/run.cu:138-223). Returns its id.
+ * Replicas are laid out as SetReplica does (src/util.C:842-861). */
+gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *desc, int *complex_id);
+
+/* Run n_steps MC steps for every replica of every complex, with a replica-
+ * exchange attempt every steps_per_exchange steps (0 = no exchange).
+ * The first call first evaluates every replica's start pose and builds each
+ * complex's reference contact map (MonteCarlo_Init_d,
+ * src/kernel_cuda_l1_montecarlo.cu:12-68). Replaces the launch loop of
+ * src/kernel_cuda_lancher_siglegpu.cu:5-69. sink may be NULL. */
+gd_status gd_run(gd_ctx *ctx, int64_t n_steps, int steps_per_exchange,
+                 gd_record_sink sink, void *user, gd_run_stats *stats);
+
+/* Per-replica state of one complex (n_rep = n_lig*n_prt*n_tmp entries each;
+ * any pointer may be NULL). acs_mc counts accepted MC moves, acs_x accepted
+ * exchanges per replica (src/kernel_cuda_l1_exchangereplicas.cu:65-66 keeps
+ * the same counts per temperature slot). */
+gd_status gd_get_state(gd_ctx *ctx, int complex_id, float *movematrix /*[n_rep*6]*/,
+                       gd_energy *energy_old, gd_replica *replica, int32_t *acs_mc,
+                       int32_t *acs_x);
+
+/* Checkpoint / restore of the whole MC state (poses, energies, temperature
+ * assignment, counters, step counter). The RNG is counter based, so a resumed
+ * run continues bit-identically. size: in = buffer size, out = bytes needed. */
+gd_status gd_checkpoint(gd_ctx *ctx, void *buf, size_t *size);
+gd_status gd_restore(gd_ctx *ctx, const void *buf, size_t size);
+
+/* Parity hook: energies of n poses of one complex, through the same device
+ * code as the MC kernel (CalcEnergy_d + CalcRmsd_d + CalcMcc_d,
+ * src/kernel_cuda_l2_calcenergy.cu:19-360). movematrix is [n*6]. Needs the
+ * complex's reference contact map (built here on first use). */
+gd_status gd_calc_energy(gd_ctx *ctx, int complex_id, int n, const int *idx_lig,
+                         const int *idx_prt, const float *movematrix, gd_energy *out);
+
+/* Parity hook: Metropolis decisions accept[i] = u[i] < exp((e_new[i]-e_old[i]) *
+ * minus_beta[i]) on the device (Accept_d, src/kernel_cuda_l2_accept.cu:16-42). */
+gd_status gd_accept(gd_ctx *ctx, int n, const float *e_new, const float *e_old,
+                    const float *minus_beta, const float *u, int32_t *accept);
+
+/* Parity hook: one temperature-exchange sweep over n_groups ladders of n_tmp
+ * slots on the device (ExchangeReplicas_d, src/kernel_cuda_l1_exchangereplicas.cu:29-78).
+ * etot, idx_tmp, acs: [n_groups*n_tmp] by slot; minus_beta: [n_tmp] by
+ * temperature; u: [n_groups*(n_tmp/2)] one uniform per attempted pair.
+ * idx_tmp and acs are updated in place. */
+gd_status gd_exchange(gd_ctx *ctx, int n_groups, int n_tmp, int mode_t, const float *etot,
+                      const float *minus_beta, const float *u, int32_t *idx_tmp,
+                      int32_t *acs);
+
+/* Parity hook: the device's Philox4x32-10 stream. For i in [0,n):
+ * out[4i..4i+3] = philox4x32_10(ctr[4i..4i+3], key[2i..2i+1]). */
+gd_status gd_philox(gd_ctx *ctx, int n, const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+
+/* Number of replicas of a complex (n_lig*n_prt*n_tmp). */
+int gd_num_replicas(const gd_ctx *ctx, int complex_id);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GEAUXDOCK_H */

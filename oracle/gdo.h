@@ -1,0 +1,93 @@
+/*
+ * gdo.h -- CPU oracle for the GeauxDock replica-exchange MC hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This code is the checker: tests/, the smoke check
+ * and the benchmark's cpu_baseline leg may call it; the engine never does.
+ *
+ * Contents
+ *   - Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw,
+ *     "Parallel random numbers: as easy as 1, 2, 3", SC'11), the stream layout
+ *     shared with the engine, uniform (0,1] and Box-Muller normals.
+ *   - gdo_expf: single-precision exp used by every Metropolis / exchange
+ *     decision (Cody-Waite reduction, degree-7 Taylor polynomial, explicit fma).
+ *   - FP64 per-term energy of one pose (the parity oracle) with per-term
+ *     magnitudes sum|contribution| for conditioned error bounds.
+ *   - FP32 CPU MC path (Move, RMSD, MCC, CalcEnergy, Accept, record,
+ *     ExchangeReplicas), OpenMP over replicas: the CPU baseline.
+ *
+ * Reference files followed are cited per function in gdo.c.
+ */
+#ifndef GDO_H
+#define GDO_H
+
+#include <stdint.h>
+#include "../include/geauxdock.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- random numbers ---------------------------------------------------- */
+void gdo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float gdo_uniform(uint32_t bits); /* ((bits >> 8) + 1) * 2^-24, in (0, 1] */
+/* Eight uniforms of MC step `step` of replica `flat` of complex `cid`. */
+void gdo_mc_uniforms(uint64_t seed, int cid, int flat, int step, float u[8]);
+/* Uniform of exchange pair `pair` of ladder `group` at segment `seg`. */
+float gdo_exchange_uniform(uint64_t seed, int cid, int group, int seg, int pair);
+/* Six move offsets for one MC step from u[0..5]. */
+void gdo_move_offsets(const float u[8], const float scale[6], int move_kind, float dm[6]);
+
+/* ---- decisions ----------------------------------------------------------- */
+float gdo_expf(float x);
+int gdo_accept(float e_new, float e_old, float minus_beta, float u);
+/* One exchange sweep of one ladder; etot/idx_tmp/acs by slot, mb by
+ * temperature, u by pair. */
+void gdo_exchange_group(int n_tmp, int mode_t, const float *etot, const float *mb,
+                        const float *u, int32_t *idx_tmp, int32_t *acs);
+
+/* ---- FP64 energy oracle ------------------------------------------------- */
+typedef struct gdo_energy64 {
+  double e[GD_MAXWEI];   /* normalized terms and total, as gd_energy */
+  double raw[GD_MAXWEI]; /* terms before a*e+b (after /lna, log, ...) */
+  double mag[GD_MAXWEI]; /* sum |contribution| per raw term (same scaling) */
+  double cms, rmsd;
+  int64_t tp, fn, fp, tn;
+  double min_margin;     /* min |d - pmf0| over all pairs (contact boundary) */
+} gdo_energy64;
+
+/* Reference contact map of replica 0's start pose (InitRefMatrix_d,
+ * src/kernel_cuda_l2_calcmcc.cu:57-83), [lna*pnp] bytes, FP64 distances. */
+void gdo_refmap_f64(const gd_complex_desc *cx, uint8_t *ref);
+void gdo_energy_f64(const gd_complex_desc *cx, const uint8_t *ref, int iprt, int ilig,
+                    const float mm[6], gdo_energy64 *out);
+/* Ligand coordinates of a pose in FP64 (Move_d, src/kernel_cuda_l2_move.cu:21-80). */
+void gdo_place_f64(const gd_complex_desc *cx, int ilig, const float mm[6], double *x,
+                   double *y, double *z);
+
+/* ---- FP32 CPU path ------------------------------------------------------ */
+void gdo_refmap_f32(const gd_complex_desc *cx, uint8_t *ref);
+void gdo_energy_f32(const gd_complex_desc *cx, const uint8_t *ref, int iprt, int ilig,
+                    const float mm[6], gd_energy *out);
+
+typedef struct gdo_mc_io {
+  float *mm;          /* [n_rep*6] movematrix_old */
+  gd_energy *e_old;   /* [n_rep] */
+  int32_t *idx_tmp;   /* [n_rep] */
+  int32_t *acs_mc;    /* [n_rep] */
+  int32_t *acs_x;     /* [n_rep] */
+  gd_record *rec;     /* [n_rep*rec_cap] or NULL */
+  int64_t rec_cap;    /* records per replica */
+  int64_t *rec_cnt;   /* [n_rep] */
+} gdo_mc_io;
+
+/* Runs steps [step0, step0+n_steps). init=1 first evaluates the start poses
+ * and records them at step0 (MonteCarlo_Init_d). nc = steps per exchange
+ * (0 = none). Returns 0 on success. */
+int gdo_mc_run(const gd_complex_desc *cx, uint64_t seed, int cid, int init, int step0,
+               int n_steps, int nc, gdo_mc_io *io, int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

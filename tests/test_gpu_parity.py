@@ -1,0 +1,223 @@
+"""GPU parity of the engine against the CPU oracle, through the C ABI.
+
+Energies: every term of gd_calc_energy (FP32, the MC kernel's device code)
+against the FP64 oracle (oracle/gdo.c, restating
+src/kernel_cuda_l2_calcenergy.cu). Tolerance, per term i and pose:
+    |gpu_i - ref_i| <= 1e-4 * mag_i          (mag_i = sum |pair contribution|)
+and, for terms without heavy cancellation (mag_i <= 10 |ref_i|),
+    |gpu_i - ref_i| <= 1e-4 * |ref_i|.
+Decisions (Metropolis, exchange) and the random stream: bit-exact.
+"""
+import numpy as np
+import pytest
+
+from geaxdock_gpu_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+TERMS = ["vdw", "ele", "pmf", "psp", "hdb", "hpc", "kde", "lhm", "dst", "total"]
+
+
+@pytest.fixture(scope="module")
+def eng_mod():
+    import geaxdock_gpu_amd as g
+    return g
+
+
+def random_poses(rng, n, t_sigma=1.5):
+    mm = np.zeros((n, 6), np.float32)
+    mm[:, :3] = rng.normal(0.0, t_sigma, size=(n, 3))
+    mm[:, 3:] = rng.uniform(-np.pi, np.pi, size=(n, 3))
+    mm[0] = 0.0
+    return mm
+
+
+def check_energy_parity(oracle, eng, cid, cx, mm, idx_lig, idx_prt, report):
+    d = cx.desc()
+    ref = oracle.refmap_f64(d, cx)
+    out = eng.calc_energy(cid, idx_lig, idx_prt, mm)
+    worst_rel = np.zeros(10)
+    worst_cond = np.zeros(10)
+    for i in range(len(mm)):
+        o = oracle.energy_f64(d, ref, int(idx_prt[i]), int(idx_lig[i]), mm[i])
+        g = out[i]
+        for t in range(10):
+            r, m = o.e[t], max(o.mag[t], 1e-30)
+            err = abs(float(g["e"][t]) - r)
+            cond = err / m
+            worst_cond[t] = max(worst_cond[t], cond)
+            assert cond <= 1e-4, (TERMS[t], i, float(g["e"][t]), r, m)
+            if m <= 10.0 * abs(r) and abs(r) > 0:
+                rel = err / abs(r)
+                worst_rel[t] = max(worst_rel[t], rel)
+                assert rel <= 1e-4, (TERMS[t], i, float(g["e"][t]), r)
+        assert abs(float(g["rmsd"]) - o.rmsd) <= 1e-4 * max(1.0, o.rmsd), (i, g["rmsd"], o.rmsd)
+        if o.min_margin > 1e-3:
+            assert abs(float(g["cms"]) - o.cms) <= 1e-5, (i, g["cms"], o.cms)
+    report.update({TERMS[t]: (worst_rel[t], worst_cond[t]) for t in range(10)})
+    return out
+
+
+@pytest.mark.parametrize("lpr,apl", [(8, 4), (16, 2), (16, 3), (32, 1), (64, 1)])
+def test_energy_terms_vs_fp64(oracle, eng_mod, lpr, apl):
+    cx = synth.config0(seed=3, identity_norm=True, void_radius=7.0)
+    rng = np.random.default_rng(11)
+    mm = random_poses(rng, 48)
+    with eng_mod.Engine(seed=1, lanes_per_replica=lpr, atoms_per_lane=apl) as eng:
+        cid = eng.add_complex(cx)
+        rep = {}
+        check_energy_parity(oracle, eng, cid, cx, mm, np.zeros(48, np.int32),
+                            np.zeros(48, np.int32), rep)
+    print("\nlpr", lpr, "apl", apl, "worst (rel, cond):",
+          {k: (f"{v[0]:.2e}", f"{v[1]:.2e}") for k, v in rep.items()})
+
+
+def test_energy_terms_weighted_multiconformer(oracle, eng_mod):
+    """Normalization a*e+b and weights of the reference parameter file, several
+    ligand and protein conformers, a 48-atom ligand."""
+    cx = synth.make_complex(5, lna=48, pnp=1500, pnk=150, pos=6, mcs_atoms=30, n_lig=3, n_prt=2,
+                            n_tmp=2, void_radius=8.0)
+    rng = np.random.default_rng(5)
+    n = 36
+    mm = random_poses(rng, n, t_sigma=2.0)
+    il = rng.integers(0, 3, size=n).astype(np.int32)
+    ip = rng.integers(0, 2, size=n).astype(np.int32)
+    il[0] = ip[0] = 0
+    with eng_mod.Engine(seed=2) as eng:
+        cid = eng.add_complex(cx)
+        rep = {}
+        check_energy_parity(oracle, eng, cid, cx, mm, il, ip, rep)
+    print("\nworst (rel, cond):", {k: (f"{v[0]:.2e}", f"{v[1]:.2e}") for k, v in rep.items()})
+
+
+def test_start_pose_mcc_is_one(oracle, eng_mod):
+    """Replica 0 in its start pose reproduces the reference contact map exactly:
+    cms = 1 whenever the map has contacts (src/kernel_cuda_l2_calcmcc.cu)."""
+    cx = synth.config0(seed=4, void_radius=5.5)
+    d = cx.desc()
+    ref = oracle.refmap_f64(d, cx)
+    assert ref.sum() > 0
+    with eng_mod.Engine(seed=3) as eng:
+        cid = eng.add_complex(cx)
+        out = eng.calc_energy(cid, [0], [0], np.zeros((1, 6), np.float32))
+    assert float(out[0]["cms"]) == pytest.approx(1.0, abs=1e-6)
+    assert float(out[0]["rmsd"]) == 0.0
+
+
+def test_philox_matches_oracle(oracle, eng_mod):
+    rng = np.random.default_rng(0)
+    ctr = rng.integers(0, 2**32, size=(257, 4), dtype=np.uint64).astype(np.uint32)
+    key = rng.integers(0, 2**32, size=(257, 2), dtype=np.uint64).astype(np.uint32)
+    ctr[0], key[0] = 0, 0
+    ctr[1], key[1] = 0xFFFFFFFF, 0xFFFFFFFF
+    with eng_mod.Engine() as eng:
+        got = eng.philox(ctr, key)
+    for i in range(len(ctr)):
+        assert np.array_equal(got[i], oracle.philox(ctr[i], key[i])), i
+    assert [hex(v) for v in got[0]] == ["0x6627e8d5", "0xe169c58d", "0xbc57ac4c", "0x9b00dbd8"]
+
+
+def test_accept_bit_exact(oracle, eng_mod):
+    rng = np.random.default_rng(1)
+    n = 200000
+    e_old = rng.normal(0, 3, n).astype(np.float32)
+    e_new = (e_old + rng.normal(0, 1, n)).astype(np.float32)
+    mb = (-1.0 / rng.uniform(0.2, 6.0, n)).astype(np.float32)
+    u = ((rng.integers(0, 2**24, n) + 1) * 2.0**-24).astype(np.float32)
+    # edge cases: dE = 0, u = 1, NaN / inf energies, huge dE
+    e_new[:8] = [e_old[0], np.nan, np.inf, -np.inf, 1e30, -1e30, e_old[6], e_old[7] + 1e-7]
+    u[:8] = [1.0, 0.5, 0.5, 0.5, 0.5, 0.5, 2.0**-24, 1.0]
+    with eng_mod.Engine() as eng:
+        got = eng.accept(e_new, e_old, mb, u)
+    ref = np.array([oracle.lib().gdo_accept(float(a), float(b), float(c), float(d))
+                    for a, b, c, d in zip(e_new, e_old, mb, u)], np.int32)
+    assert np.array_equal(got, ref)
+    assert got[0] == 1 and got[1] == 0 and got[2] == 0
+
+
+def test_exchange_bit_exact(oracle, eng_mod):
+    import ctypes as C
+    rng = np.random.default_rng(2)
+    for n_tmp in (2, 7, 8, 16, 40):
+        n_groups = 500
+        et = rng.normal(0, 2, size=(n_groups, n_tmp)).astype(np.float32)
+        it = np.array([rng.permutation(n_tmp) for _ in range(n_groups)], np.int32)
+        ac = rng.integers(0, 5, size=(n_groups, n_tmp)).astype(np.int32)
+        _, mb = synth.temperature_ladder(n_tmp, 0.5, 5.0)
+        u = rng.uniform(0, 1, size=(n_groups, n_tmp // 2)).astype(np.float32)
+        for mode_t in (0, 1):
+            with eng_mod.Engine() as eng:
+                git, gac = eng.exchange(n_tmp, mode_t, et, mb, u, it, ac)
+            for g in range(n_groups):
+                rit, rac = it[g].copy(), ac[g].copy()
+                oracle.lib().gdo_exchange_group(
+                    n_tmp, mode_t, et[g].ctypes.data_as(C.POINTER(C.c_float)),
+                    mb.ctypes.data_as(C.POINTER(C.c_float)),
+                    u[g].ctypes.data_as(C.POINTER(C.c_float)),
+                    rit.ctypes.data_as(C.POINTER(C.c_int32)),
+                    rac.ctypes.data_as(C.POINTER(C.c_int32)))
+                assert np.array_equal(git[g], rit) and np.array_equal(gac[g], rac), (n_tmp, mode_t, g)
+                assert sorted(git[g].tolist()) == list(range(n_tmp))
+
+
+def _run_gpu(eng_mod, cx, seed, n_steps, nc, **kw):
+    recs = []
+    with eng_mod.Engine(seed=seed, **kw) as eng:
+        cid = eng.add_complex(cx)
+        st = eng.run(n_steps, nc, sink=lambda c, r: recs.append(r))
+        state = eng.state(cid)
+    recs = np.concatenate(recs) if recs else np.zeros(0, abi.RECORD_DTYPE)
+    return recs, state, st
+
+
+def test_mc_run_tracks_cpu_path(oracle, eng_mod):
+    """Same seed, same random stream: the GPU engine and the FP32 CPU path make
+    the same accept / exchange decisions, step for step, unless an energy falls
+    within rounding of a threshold."""
+    cx = synth.config0(seed=0)
+    n_steps, nc = 400, 20
+    recs, state, st = _run_gpu(eng_mod, cx, 7, n_steps, nc)
+    d = cx.desc()
+    cpu = oracle.McState(cx, rec_cap=n_steps + 1)
+    oracle.mc_run(d, cpu, seed=7, init=True, step0=0, n_steps=0, nc=nc)
+    oracle.mc_run(d, cpu, seed=7, init=False, step0=0, n_steps=n_steps, nc=nc)
+    same = 0
+    for r in range(cx.n_rep):
+        g = recs[recs["replica"][:, 0] == r]
+        c = cpu.records(r)
+        k = min(len(g), len(c))
+        agree = k > 0 and np.array_equal(g["step"][:k], c["step"][:k])
+        if agree and len(g) == len(c):
+            same += 1
+            np.testing.assert_allclose(g["energy"]["e"][:, 9], c["energy"]["e"][:, 9], rtol=2e-4,
+                                       atol=2e-4)
+            np.testing.assert_allclose(g["movematrix"], c["movematrix"], rtol=1e-4, atol=1e-4)
+    assert same >= cx.n_rep - 1, same
+    assert np.array_equal(state["acs_mc"], cpu.acs_mc) or same < cx.n_rep
+    print("\nreplicas with identical decision sequences:", same, "of", cx.n_rep,
+          "acs_mc", state["acs_mc"], "acs_x", state["acs_x"])
+
+
+def test_checkpoint_resume_bit_exact(eng_mod):
+    cx = synth.config0(seed=1)
+    with eng_mod.Engine(seed=5, record=False) as eng:
+        cid = eng.add_complex(cx)
+        eng.run(60, 20)
+        blob = eng.checkpoint()
+        eng.run(80, 20)
+        a = eng.state(cid)
+        eng.restore(blob)
+        eng.run(80, 20)
+        b = eng.state(cid)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_run_deterministic_and_window_independent(eng_mod):
+    """A run is a function of the seed alone: record windows of 50 or 2000
+    steps give bit-identical states."""
+    cx = synth.config0(seed=2)
+    _, s1, _ = _run_gpu(eng_mod, cx, 9, 300, 20, steps_per_dump=50)
+    _, s2, _ = _run_gpu(eng_mod, cx, 9, 300, 20, steps_per_dump=2000)
+    for k in s1:
+        assert np.array_equal(s1[k], s2[k]), k
