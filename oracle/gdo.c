@@ -211,7 +211,8 @@ static void finalize64(const gd_complex_desc *cx, const double raw_in[9], const 
   double total = 0.0;
   for (int i = 0; i < 9; ++i) {
     out->raw[i] = raw_in[i];
-    out->mag[i] = mag_in[i];
+    /* magnitude of the normalized term a*e+b */
+    out->mag[i] = fabs((double)ep->a_para[i]) * mag_in[i] + fabs((double)ep->b_para[i]);
     out->e[i] = (double)ep->a_para[i] * raw_in[i] + (double)ep->b_para[i];
     total += (double)ep->w[i] * out->e[i];
   }

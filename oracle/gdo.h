@@ -49,7 +49,7 @@ void gdo_exchange_group(int n_tmp, int mode_t, const float *etot, const float *m
 typedef struct gdo_energy64 {
   double e[GD_MAXWEI];   /* normalized terms and total, as gd_energy */
   double raw[GD_MAXWEI]; /* terms before a*e+b (after /lna, log, ...) */
-  double mag[GD_MAXWEI]; /* sum |contribution| per raw term (same scaling) */
+  double mag[GD_MAXWEI]; /* |a| * sum|contribution| + |b| per term of e (condition scale) */
   double cms, rmsd;
   int64_t tp, fn, fp, tn;
   double min_margin;     /* min |d - pmf0| over all pairs (contact boundary) */

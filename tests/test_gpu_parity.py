@@ -124,15 +124,19 @@ def test_accept_bit_exact(oracle, eng_mod):
     e_new = (e_old + rng.normal(0, 1, n)).astype(np.float32)
     mb = (-1.0 / rng.uniform(0.2, 6.0, n)).astype(np.float32)
     u = ((rng.integers(0, 2**24, n) + 1) * 2.0**-24).astype(np.float32)
-    # edge cases: dE = 0, u = 1, NaN / inf energies, huge dE
-    e_new[:8] = [e_old[0], np.nan, np.inf, -np.inf, 1e30, -1e30, e_old[6], e_old[7] + 1e-7]
-    u[:8] = [1.0, 0.5, 0.5, 0.5, 0.5, 0.5, 2.0**-24, 1.0]
+    # edge cases: dE = 0 with U = 1 (strict U < exp(0) rejects) and U < 1,
+    # NaN / inf energies, huge dE
+    e_new[:9] = [e_old[0], np.nan, np.inf, -np.inf, 1e30, -1e30, e_old[6], e_old[7] + 1e-7,
+                 e_old[8]]
+    u[:9] = [1.0, 0.5, 0.5, 0.5, 0.5, 0.5, 2.0**-24, 1.0, 1.0 - 2.0**-24]
     with eng_mod.Engine() as eng:
         got = eng.accept(e_new, e_old, mb, u)
     ref = np.array([oracle.lib().gdo_accept(float(a), float(b), float(c), float(d))
                     for a, b, c, d in zip(e_new, e_old, mb, u)], np.int32)
     assert np.array_equal(got, ref)
-    assert got[0] == 1 and got[1] == 0 and got[2] == 0
+    assert got[0] == 0 and got[8] == 1  # dE = 0: accept unless U = 1
+    assert got[1] == 0 and got[2] == 0  # NaN and +inf never accepted
+    assert got[3] == 1 and got[5] == 1 and got[4] == 0
 
 
 def test_exchange_bit_exact(oracle, eng_mod):
