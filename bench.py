@@ -44,7 +44,23 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lpr", type=int, default=0)
     ap.add_argument("--apl", type=int, default=0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(mib, reps=5):
+    """Plain device-to-device copy of known size, for scaling FETCH_SIZE /
+    WRITE_SIZE counters to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4,
+                      "reps": reps}))
 
 
 def workload(args):
@@ -97,6 +113,9 @@ def cpu_baseline(cxs, args):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are "
                                    "not part of this build (shard whole temperature ladders or "
