@@ -233,6 +233,8 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
     if (d->kde_t[k] < 0 || d->kde_t[k] >= NLT)
       return fail(ctx, GD_ERR_INVALID, "KDE point %d type out of range", k);
 
+  if (!(d->enepara->el1 > 0.0f) || !(d->enepara->lj0 > 0.0f))
+    return fail(ctx, GD_ERR_INVALID, "el1 and lj0 must be positive");
   HIPCHK(hipSetDevice(ctx->device));
   ctx->cx.emplace_back();
   HostComplex &h = ctx->cx.back();
@@ -259,7 +261,9 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
     for (int i = 0; i < pnp; ++i) {
       const int p = perm[i];
       const size_t o = (size_t)c * pnp;
-      xyze[o + i] = make_float4(d->prt_x[o + p], d->prt_y[o + p], d->prt_z[o + p], d->prt_ele[p]);
+      // the electrostatic parameter is stored divided by el1 (far-field form)
+      xyze[o + i] = make_float4(d->prt_x[o + p], d->prt_y[o + p], d->prt_z[o + p],
+                                d->prt_ele[p] / d->enepara->el1);
     }
   std::vector<float> hpp(pnp);
   std::vector<int> pspi(pnp);

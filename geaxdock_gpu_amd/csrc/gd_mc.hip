@@ -49,74 +49,143 @@ constexpr float H4 = 0.5f / (81.0f * 81.0f * 81.0f * 81.0f);
 constexpr float LOG2E = 1.44269504088896340736f;
 
 struct PairConst {
-  float lj0, lj01, el0, el1, inv_el1, a1, b1;
+  float lj0, lj01, el0, el1, a1, b1;
 };
 
-// Receptor points [beg, end) of one chunk against the lane's APL atoms.
+// Per-lane parameters of one chunk (one receptor type) for one ligand atom.
+struct ChunkPar {
+  float p1a, p2a, pmf0, pmf1, pb, pc, ha, hb, hc;
+  float nr2;  // contacts and the near electrostatic form lie within sqrt(nr2)
+  float rf2;  // beyond sqrt(rf2) PMF equals pmf1 exactly and H-bond < 1e-18
+};
+
+// Receptor points [p, p + NP) of one chunk against the lane's APL atoms.
 // All terms of src/kernel_cuda_l2_calcenergy.cu:105-184 plus the contact
 // count of CalcMcc_d:
 //   vdW  (p1 - p2) / (p1 lj0 (1 + lj1 d^2) + 1), p1 = p1a/d^9, p2 = p2a/d^6,
-//        evaluated multiplied through by d^9: (p1a - p2a d^3)/(p1a t + d^9);
+//        evaluated multiplied through by d^9: num/den = (p1a - p2a d^3)/(p1a t + d^9);
 //   ele  ele_p * g(el1 d) (the ligand charge is applied per atom later);
-//   pmf  pmf1 / (1 + exp((6 - d/2)(d - pmf0)));
+//        g = 1/(el1 d) is accumulated for every pair (the point stores
+//        ele_p/el1); where el1 d < 1 the near form replaces it;
+//   pmf  pmf1 / (1 + exp((6 - d/2)(d - pmf0))), exponent in base 2 as
+//        A d^2 + B d + C; vdW and PMF share one reciprocal 1/(den (1 + e));
 //   hdb  -hdb1/sqrt(2 pi) exp(-((d - hdb0) hdb1)^2 / 2) (zero record if hdb0 <= 0.1);
 //   hpc  hpp * switch(d^2) for d^2 <= 81 (KIND >= 1);
 //   psp  psp[lig_t][seq3r] when class 2 and d <= pmf0 (KIND == 2);
 //   contacts  d <= pmf0.
+// Two wave-uniform branches skip work no lane of the wave needs: when every
+// lane is beyond its far radius the exponentials are not evaluated (1 + e
+// rounds to 1 there, so PMF is pmf1 bit for bit); contacts, the near
+// electrostatic form and PSP run only when some lane is within contact range.
+template <int APL, int KIND, int NP>
+__device__ __forceinline__ int pair_group(const Lds &S, int p, const PairConst &K,
+                                           const float (&x)[APL], const float (&y)[APL],
+                                           const float (&z)[APL], const ChunkPar (&cp)[APL],
+                                           const float *const (&psprow)[APL],
+                                           float (&vdw)[APL], float (&ele)[APL],
+                                           float (&pmf)[APL], float (&psp)[APL],
+                                           float (&hdb)[APL], float (&hpc)[APL],
+                                           int (&cnt)[APL]) {
+  const float pa = -0.5f * LOG2E;
+  int nfar = 0;
+  float4 P[NP];
+  float hp[NP];
+  int ps[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    P[j] = S.xyze[p + j];
+    hp[j] = (KIND >= KIND_HPC) ? S.hpp[p + j] : 0.0f;
+    ps[j] = (KIND == KIND_PSP) ? S.psp[p + j] : 0;
+  }
+  float d2[NP][APL], rd[NP][APL], dd[NP][APL], num[NP][APL], den[NP][APL];
+  bool mid = false, near = false;
+#pragma unroll
+  for (int j = 0; j < NP; ++j)
+#pragma unroll
+    for (int k = 0; k < APL; ++k) {
+      const float q2 = pair_d2(x[k], y[k], z[k], P[j].x, P[j].y, P[j].z);
+      const float r = fast_rsq(q2);
+      const float d = q2 * r;
+      d2[j][k] = q2;
+      rd[j][k] = r;
+      dd[j][k] = d;
+      mid |= q2 < cp[k].rf2;
+      near |= q2 <= cp[k].nr2;
+      const float d3 = q2 * d;
+      const float d6 = d3 * d3;
+      const float d9 = d6 * d3;
+      num[j][k] = fmaf(-cp[k].p2a, d3, cp[k].p1a);
+      den[j][k] = fmaf(cp[k].p1a, fmaf(K.lj01, q2, K.lj0), d9);
+      ele[k] = fmaf(P[j].w, r, ele[k]);
+    }
+  if (__ballot(mid) != 0) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+#pragma unroll
+      for (int k = 0; k < APL; ++k) {
+        const float q2 = d2[j][k], d = dd[j][k];
+        const float wv = fminf(fmaf(pa, q2, fmaf(cp[k].pb, d, cp[k].pc)), 64.0f);
+        const float e1 = 1.0f + fast_exp2(wv);
+        const float r = fast_rcp(den[j][k] * e1);
+        vdw[k] = fmaf(num[j][k] * e1, r, vdw[k]);
+        pmf[k] = fmaf(cp[k].pmf1 * den[j][k], r, pmf[k]);
+        const float h = fmaf(d, cp[k].ha, cp[k].hb);
+        hdb[k] = fmaf(cp[k].hc, fast_exp2(-(h * h)), hdb[k]);
+        if (KIND >= KIND_HPC) {
+          const float poly = fmaf(q2, fmaf(q2, fmaf(q2, fmaf(q2, H4, H3), H2), H1), 1.0f);
+          hpc[k] += (q2 <= 81.0f) ? hp[j] * poly : 0.0f;
+        }
+      }
+  } else {
+    // far field: PMF contributes pmf1 per point, added once per chunk
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+#pragma unroll
+      for (int k = 0; k < APL; ++k) vdw[k] = fmaf(num[j][k], fast_rcp(den[j][k]), vdw[k]);
+    nfar = NP;
+  }
+  if (__ballot(near) != 0) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+#pragma unroll
+      for (int k = 0; k < APL; ++k) {
+        const float d = dd[j][k];
+        const bool c = d <= cp[k].pmf0;
+        cnt[k] += c ? 1 : 0;
+        const float s = K.el1 * d;
+        if (s < 1.0f) {
+          const float gn = fmaf(s * s, fmaf(K.b1, s, K.a1), K.el0);
+          ele[k] = fmaf(P[j].w, fmaf(K.el1, gn, -rd[j][k]), ele[k]);
+        }
+        if (KIND == KIND_PSP) {
+          if (c) psp[k] += psprow[k][ps[j]];
+        }
+      }
+  }
+  return nfar;
+}
+
 template <int APL, int KIND>
 __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, const PairConst &K,
                                             const float (&x)[APL], const float (&y)[APL],
-                                            const float (&z)[APL], const float (&prm)[APL][7],
+                                            const float (&z)[APL], const ChunkPar (&cp)[APL],
                                             const float *const (&psprow)[APL],
                                             float (&vdw)[APL], float (&ele)[APL],
                                             float (&pmf)[APL], float (&psp)[APL],
                                             float (&hdb)[APL], float (&hpc)[APL],
                                             int (&cnt)[APL]) {
-  const float m05l = -0.5f * LOG2E, c6l = 6.0f * LOG2E;
-#pragma unroll 2
-  for (int p = beg; p < end; ++p) {
-    const float4 P = S.xyze[p];
-    float hp = 0.0f;
-    int ps = 0;
-    if (KIND >= KIND_HPC) hp = S.hpp[p];
-    if (KIND == KIND_PSP) ps = S.psp[p];
+  // points per group: enough independent pairs per lane to cover latency
+  constexpr int NP = APL >= 3 ? 1 : (APL == 2 ? 2 : 4);
+  int p = beg, nfar = 0;
+  for (; p + NP <= end; p += NP)
+    nfar += pair_group<APL, KIND, NP>(S, p, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc,
+                                      cnt);
+  for (; p < end; ++p)
+    nfar += pair_group<APL, KIND, 1>(S, p, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc,
+                                     cnt);
+  const float nf = (float)nfar;
 #pragma unroll
-    for (int k = 0; k < APL; ++k) {
-      const float d2 = pair_d2(x[k], y[k], z[k], P.x, P.y, P.z);
-      const float rd = fast_rsq(d2);
-      const float d = d2 * rd;
-      // soft Lennard-Jones
-      const float d3 = d2 * d;
-      const float d6 = d3 * d3;
-      const float d9 = d6 * d3;
-      const float num = fmaf(-prm[k][P_P2A], d3, prm[k][P_P1A]);
-      const float t = fmaf(K.lj01, d2, K.lj0);
-      const float den = fmaf(prm[k][P_P1A], t, d9);
-      vdw[k] = fmaf(num, fast_rcp(den), vdw[k]);
-      // soft electrostatics
-      const float s = K.el1 * d;
-      const float gn = fmaf(s * s, fmaf(K.b1, s, K.a1), K.el0);
-      const float gf = rd * K.inv_el1;
-      ele[k] = fmaf(P.w, (s < 1.0f) ? gn : gf, ele[k]);
-      // contact PMF
-      const float v = d - prm[k][P_PMF0];
-      const float wv = fmaf(m05l, d, c6l) * v;
-      pmf[k] = fmaf(prm[k][P_PMF1], fast_rcp(1.0f + fast_exp2(wv)), pmf[k]);
-      // hydrogen bond
-      const float h = fmaf(d, prm[k][P_HA], prm[k][P_HB]);
-      hdb[k] = fmaf(prm[k][P_HC], fast_exp2(-(h * h)), hdb[k]);
-      // contacts for MCC
-      const bool c = v <= 0.0f;
-      cnt[k] += c ? 1 : 0;
-      if (KIND >= KIND_HPC) {
-        const float poly = fmaf(d2, fmaf(d2, fmaf(d2, fmaf(d2, H4, H3), H2), H1), 1.0f);
-        hpc[k] += (d2 <= 81.0f) ? hp * poly : 0.0f;
-      }
-      if (KIND == KIND_PSP) {
-        if (c) psp[k] += psprow[k][ps];
-      }
-    }
-  }
+  for (int k = 0; k < APL; ++k) pmf[k] = fmaf(cp[k].pmf1, nf, pmf[k]);
 }
 
 // Energy, RMSD and MCC of one pose held by `lpr` lanes (CalcEnergy_d +
@@ -135,7 +204,6 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
   K.lj01 = C.lj01;
   K.el0 = C.el0;
   K.el1 = C.el1;
-  K.inv_el1 = C.inv_el1;
   K.a1 = C.a1;
   K.b1 = C.b1;
 
@@ -155,28 +223,42 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
     const int end = beg + __builtin_amdgcn_readfirstlane(ch.y);
     const int type = __builtin_amdgcn_readfirstlane(ch.z);
     const int kind = __builtin_amdgcn_readfirstlane(ch.w);
-    float prm[APL][7];
+    ChunkPar cp[APL];
 #pragma unroll
     for (int k = 0; k < APL; ++k) {
       const float *r = S.par + (type * NLT + lt[k]) * NPAR;
       const float4 a = *reinterpret_cast<const float4 *>(r);
       const float4 b = *reinterpret_cast<const float4 *>(r + 4);
-      prm[k][0] = a.x;
-      prm[k][1] = a.y;
-      prm[k][2] = a.z;
-      prm[k][3] = a.w;
-      prm[k][4] = b.x;
-      prm[k][5] = b.y;
-      prm[k][6] = b.z;
+      cp[k].p1a = a.x;
+      cp[k].p2a = a.y;
+      cp[k].pmf0 = a.z;
+      cp[k].pmf1 = a.w;
+      cp[k].ha = b.x;
+      cp[k].hb = b.y;
+      cp[k].hc = b.z;
+      // (6 - d/2)(d - pmf0) log2(e) = A d^2 + B d + C
+      cp[k].pb = LOG2E * fmaf(0.5f, a.z, 6.0f);
+      cp[k].pc = -6.0f * LOG2E * a.z;
+      // contacts (d <= pmf0) and the near electrostatic form (d < 1/el1)
+      // lie within this radius; the margin covers the rounding of d
+      const float rn = fmaxf(a.z, C.inv_el1) * 1.0001f;
+      cp[k].nr2 = rn * rn;
+      // far radius: (d/2 - 6)(d - pmf0) >= 25 ln 2 makes exp2 of the PMF
+      // exponent <= 2^-25, so 1 + e == 1; the H-bond exponent -h^2 <= -60
+      const float u = 0.5f * a.z - 6.0f;
+      float rf = (6.0f + 0.5f * a.z) + sqrtf(u * u + 2.0f * 25.0f * 0.69314718f);
+      if (b.z != 0.0f) rf = fmaxf(rf, (7.7459667f - b.y) / b.x);
+      rf *= 1.0001f;
+      cp[k].rf2 = rf * rf;
     }
     if (kind == KIND_PLAIN)
-      chunk_pairs<APL, KIND_PLAIN>(S, beg, end, K, x, y, z, prm, psprow, vdw, ele, pmf, psp, hdb,
+      chunk_pairs<APL, KIND_PLAIN>(S, beg, end, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
                                    hpc, cnt);
     else if (kind == KIND_HPC)
-      chunk_pairs<APL, KIND_HPC>(S, beg, end, K, x, y, z, prm, psprow, vdw, ele, pmf, psp, hdb,
+      chunk_pairs<APL, KIND_HPC>(S, beg, end, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
                                  hpc, cnt);
     else
-      chunk_pairs<APL, KIND_PSP>(S, beg, end, K, x, y, z, prm, psprow, vdw, ele, pmf, psp, hdb,
+      chunk_pairs<APL, KIND_PSP>(S, beg, end, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
                                  hpc, cnt);
   }
 
@@ -213,7 +295,7 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
       const float4 P = S.xyze[r.x];
       const float d2 = pair_d2(x[k], y[k], z[k], P.x, P.y, P.z);
       const float d = d2 * fast_rsq(d2);
-      t_tp += (d - __int_as_float(r.y) <= 0.0f) ? 1 : 0;
+      t_tp += (d <= __int_as_float(r.y)) ? 1 : 0;
     }
     // RMSD against the start pose
     const float dx = x[k] - (ox[k] + C.pocket[0]);
@@ -549,7 +631,7 @@ extern "C" __global__ void gd_ref_kernel(DevComplex *cxs, int cxi, int *overflow
       const float4 P = C.prt_xyze[p];
       const float d2 = pair_d2(x, y, z, P.x, P.y, P.z);
       const float d = d2 * fast_rsq(d2);
-      if (d - pmf0 <= 0.0f) {
+      if (d <= pmf0) {
         if (n < KMAX) C.ref[(size_t)l * KMAX + n] = make_int2(p, __float_as_int(pmf0));
         ++n;
       }
