@@ -58,8 +58,28 @@ def load_library():
     lib.gd_philox.argtypes = [vp, C.c_int, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     lib.gd_num_replicas.argtypes = [vp, C.c_int]
     lib.gd_num_replicas.restype = C.c_int
+    lib.gd_sur_num_params.argtypes = []
+    lib.gd_sur_num_params.restype = C.c_int
+    lib.gd_sur_build_grids.argtypes = [vp, C.c_float, C.c_float]
+    lib.gd_sur_features.argtypes = [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_int), P(C.c_float),
+                                    P(C.c_float)]
+    lib.gd_sur_set_params.argtypes = [vp, P(C.c_float)]
+    lib.gd_sur_get_params.argtypes = [vp, P(C.c_float)]
+    lib.gd_sur_dataset_clear.argtypes = [vp]
+    lib.gd_sur_dataset_reserve.argtypes = [vp, C.c_int64]
+    lib.gd_sur_dataset_add.argtypes = [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_int),
+                                       P(C.c_float), P(C.c_float)]
+    lib.gd_sur_dataset_size.argtypes = [vp]
+    lib.gd_sur_dataset_size.restype = C.c_int64
+    lib.gd_sur_loss_grad.argtypes = [vp, C.c_int, C.c_int, P(C.c_double), P(C.c_float)]
+    lib.gd_sur_predict.argtypes = [vp, C.c_int, C.c_int, P(C.c_float)]
+    lib.gd_sur_train.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                                 P(C.c_double), P(C.c_double)]
     for f in ("gd_create", "gd_add_complex", "gd_run", "gd_get_state", "gd_checkpoint",
-              "gd_restore", "gd_calc_energy", "gd_accept", "gd_exchange", "gd_philox"):
+              "gd_restore", "gd_calc_energy", "gd_accept", "gd_exchange", "gd_philox",
+              "gd_sur_build_grids", "gd_sur_features", "gd_sur_set_params", "gd_sur_get_params",
+              "gd_sur_dataset_clear", "gd_sur_dataset_reserve", "gd_sur_dataset_add",
+              "gd_sur_loss_grad", "gd_sur_predict", "gd_sur_train"):
         getattr(lib, f).restype = C.c_int
     _lib = lib
     return lib
@@ -197,6 +217,71 @@ class Engine:
         self._check(self.lib.gd_exchange(self._ctx, n_groups, n_tmp, mode_t, _f(etot), _f(mb),
                                          _f(u), _i(it), _i(ac)))
         return it, ac
+
+    # ---- learned surrogate -------------------------------------------------
+    @property
+    def sur_num_params(self):
+        return self.lib.gd_sur_num_params()
+
+    def sur_build_grids(self, spacing=0.5, half_width=16.0):
+        self._check(self.lib.gd_sur_build_grids(self._ctx, spacing, half_width))
+
+    def sur_features(self, cid, idx_lig, idx_prt, movematrix):
+        il = np.ascontiguousarray(idx_lig, np.int32)
+        ip = np.ascontiguousarray(idx_prt, np.int32)
+        mm = np.ascontiguousarray(movematrix, np.float32).reshape(-1, 6)
+        lna = self.complexes[cid].lna
+        out = np.zeros((len(mm), lna, 56), np.float32)
+        self._check(self.lib.gd_sur_features(self._ctx, cid, len(mm), _i(il), _i(ip), _f(mm),
+                                             _f(out)))
+        return out
+
+    def sur_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.sur_num_params
+        self._check(self.lib.gd_sur_set_params(self._ctx, _f(p)))
+
+    def sur_get_params(self):
+        p = np.zeros(self.sur_num_params, np.float32)
+        self._check(self.lib.gd_sur_get_params(self._ctx, _f(p)))
+        return p
+
+    def sur_dataset_clear(self):
+        self._check(self.lib.gd_sur_dataset_clear(self._ctx))
+
+    def sur_dataset_reserve(self, rows):
+        self._check(self.lib.gd_sur_dataset_reserve(self._ctx, int(rows)))
+
+    def sur_dataset_add(self, cid, idx_lig, idx_prt, movematrix, target):
+        il = np.ascontiguousarray(idx_lig, np.int32)
+        ip = np.ascontiguousarray(idx_prt, np.int32)
+        mm = np.ascontiguousarray(movematrix, np.float32).reshape(-1, 6)
+        t = np.ascontiguousarray(target, np.float32)
+        self._check(self.lib.gd_sur_dataset_add(self._ctx, cid, len(mm), _i(il), _i(ip), _f(mm),
+                                                _f(t)))
+
+    def sur_dataset_size(self):
+        return self.lib.gd_sur_dataset_size(self._ctx)
+
+    def sur_loss_grad(self, begin, end, grad=True):
+        loss = C.c_double(0.0)
+        g = np.zeros(self.sur_num_params, np.float32) if grad else None
+        self._check(self.lib.gd_sur_loss_grad(self._ctx, begin, end, C.byref(loss),
+                                              _f(g) if grad else None))
+        return loss.value, g
+
+    def sur_predict(self, begin, end):
+        out = np.zeros(end - begin, np.float32)
+        self._check(self.lib.gd_sur_predict(self._ctx, begin, end, _f(out)))
+        return out
+
+    def sur_train(self, begin, end, epochs, batch_poses, lr=1e-3):
+        hist = np.zeros(max(epochs, 1), np.float64)
+        secs = C.c_double(0.0)
+        self._check(self.lib.gd_sur_train(self._ctx, begin, end, epochs, batch_poses, lr,
+                                          hist.ctypes.data_as(C.POINTER(C.c_double)),
+                                          C.byref(secs)))
+        return hist[:epochs], secs.value
 
     def philox(self, ctr, key):
         ctr = np.ascontiguousarray(ctr, np.uint32).reshape(-1, 4)

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "gd_mc.hip"
+#include "gd_sur.hip"
 
 using namespace gd;
 
@@ -61,6 +62,30 @@ struct gd_ctx {
   int n_packs = 0, block = 0;
   size_t lds = 0;
   KParams lay{};  // LDS layout of the MC launch
+  // learned surrogate
+  struct Sur {
+    std::vector<float *> grid;  // per complex [n_prt][n^3][32]
+    std::vector<SurGrid> g;
+    float *p = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;  // FP32 master + Adam
+    unsigned short *W1T = nullptr, *W2T = nullptr, *W2R = nullptr;    // bf16 layouts
+    int64_t adam_t = 0;
+    bool have_params = false;
+    // dataset (rows padded, last 64 rows zero)
+    unsigned short *X = nullptr, *XT = nullptr;
+    int64_t rows = 0, cap = 0;
+    std::vector<int64_t> off{0};  // pose row offsets
+    std::vector<float> target;
+    std::vector<int> pose_cx;
+    int *pose_of_row = nullptr;
+    int64_t *d_off = nullptr;
+    float *d_target = nullptr;
+    bool dirty = true;
+    // work buffers
+    float *y = nullptr, *dS = nullptr, *sq = nullptr, *S = nullptr;
+    unsigned short *A1T = nullptr, *dH2T = nullptr, *dH1T = nullptr;
+    float *p1 = nullptr, *p2 = nullptr, *pb2 = nullptr, *pw3 = nullptr;
+    int64_t work_rows = 0;
+  } sur;
 };
 
 namespace {
@@ -182,6 +207,16 @@ void gd_destroy(gd_ctx *ctx) {
                 ctx->d_packs, ctx->d_slots};
   for (void *p : ps)
     if (p) (void)hipFree(p);
+  {
+    auto &S = ctx->sur;
+    for (float *g : S.grid)
+      if (g) (void)hipFree(g);
+    void *qs[] = {S.p, S.m, S.v, S.grad, S.W1T, S.W2T, S.W2R, S.X, S.XT, S.pose_of_row,
+                  S.d_off, S.d_target, S.y, S.dS, S.sq, S.S, S.A1T, S.dH2T, S.dH1T, S.p1,
+                  S.p2, S.pb2, S.pw3};
+    for (void *p : qs)
+      if (p) (void)hipFree(p);
+  }
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -388,6 +423,11 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
   TRY(dev_upload(ctx, h, (float4 **)&D.prt_xyze, xyze.data(), xyze.size()));
   TRY(dev_upload(ctx, h, (float **)&D.prt_hpp, hpp.data(), hpp.size()));
   TRY(dev_upload(ctx, h, (int **)&D.prt_psp, pspi.data(), pspi.size()));
+  {
+    std::vector<int> cls(pnp);
+    for (int i = 0; i < pnp; ++i) cls[i] = d->prt_t[perm[i]] % 4;
+    TRY(dev_upload(ctx, h, (int **)&D.prt_cls, cls.data(), cls.size()));
+  }
   TRY(dev_upload(ctx, h, (int4 **)&D.chunks, chunks.data(), chunks.size()));
   TRY(dev_upload(ctx, h, (float **)&D.params, par.data(), par.size()));
   TRY(dev_upload(ctx, h, (float **)&D.lig_orig, orig.data(), orig.size()));
@@ -987,6 +1027,403 @@ gd_status gd_philox(gd_ctx *ctx, int n, const uint32_t *ctr, const uint32_t *key
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   free_all(keep);
   if (e != hipSuccess) return fail(ctx, GD_ERR_HIP, "gd_philox: %s", hipGetErrorString(e));
+  return GD_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Learned surrogate: feature grids, dataset, BF16 MFMA training, prediction.
+// ===========================================================================
+namespace {
+
+gd_status sur_alloc_params(gd_ctx *ctx) {
+  auto &S = ctx->sur;
+  if (S.p) return GD_OK;
+  HIPCHK(hipMalloc(&S.p, SUR_NPARAM * sizeof(float)));
+  HIPCHK(hipMalloc(&S.m, SUR_NPARAM * sizeof(float)));
+  HIPCHK(hipMalloc(&S.v, SUR_NPARAM * sizeof(float)));
+  HIPCHK(hipMalloc(&S.grad, SUR_NPARAM * sizeof(float)));
+  HIPCHK(hipMalloc(&S.W1T, SUR_F * SUR_H * 2));
+  HIPCHK(hipMalloc(&S.W2T, SUR_H * SUR_H * 2));
+  HIPCHK(hipMalloc(&S.W2R, SUR_H * SUR_H * 2));
+  return GD_OK;
+}
+
+SurW sur_weights(gd_ctx *ctx) {
+  auto &S = ctx->sur;
+  SurW W;
+  W.W1T = S.W1T;
+  W.W2T = S.W2T;
+  W.W2R = S.W2R;
+  W.b2 = S.p + SP_B2;
+  W.w3 = S.p + SP_W3;
+  W.b3 = S.p + SP_B3;
+  return W;
+}
+
+// refresh the bf16 layouts from the FP32 master (Adam with lr = 0)
+gd_status sur_refresh(gd_ctx *ctx) {
+  auto &S = ctx->sur;
+  hipLaunchKernelGGL(gd_mlp_adam_kernel, dim3((SUR_NPARAM + 255) / 256), dim3(256), 0, ctx->stream,
+                     S.p, S.m, S.v, S.grad, 0.0f, 0.f, 0.f, 0.f, 1.f, 1.f, S.W1T, S.W2T, S.W2R);
+  HIPCHK(hipGetLastError());
+  return GD_OK;
+}
+
+gd_status sur_upload_index(gd_ctx *ctx) {
+  auto &S = ctx->sur;
+  if (!S.dirty) return GD_OK;
+  const int64_t np = (int64_t)S.target.size();
+  if (S.d_off) (void)hipFree(S.d_off);
+  if (S.d_target) (void)hipFree(S.d_target);
+  if (S.pose_of_row) (void)hipFree(S.pose_of_row);
+  S.d_off = nullptr;
+  S.d_target = nullptr;
+  S.pose_of_row = nullptr;
+  HIPCHK(hipMalloc(&S.d_off, (np + 1) * sizeof(int64_t)));
+  HIPCHK(hipMalloc(&S.d_target, std::max<int64_t>(np, 1) * sizeof(float)));
+  HIPCHK(hipMalloc(&S.pose_of_row, (S.cap + 64) * sizeof(int)));
+  // per-row outputs are indexed by absolute dataset row
+  if (S.y) (void)hipFree(S.y);
+  S.y = nullptr;
+  HIPCHK(hipMalloc(&S.y, (S.cap + 64) * sizeof(float)));
+  std::vector<int> por(S.cap + 64, -1);
+  for (int64_t p = 0; p < np; ++p)
+    for (int64_t r = S.off[p]; r < S.off[p + 1]; ++r) por[r] = (int)p;
+  HIPCHK(hipMemcpyAsync(S.d_off, S.off.data(), (np + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+  if (np) HIPCHK(hipMemcpyAsync(S.d_target, S.target.data(), np * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(S.pose_of_row, por.data(), por.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  S.dirty = false;
+  return GD_OK;
+}
+
+// work buffers for batches of up to `rows` rows (rounded up to 64)
+gd_status sur_work(gd_ctx *ctx, int64_t rows, int64_t n_pose_total) {
+  auto &S = ctx->sur;
+  rows = (rows + 63) / 64 * 64 + 64;
+  const int64_t R = 1024;
+  if (rows > S.work_rows) {
+    void *qs[] = {S.A1T, S.dH2T, S.dH1T, S.p1, S.p2, S.pb2, S.pw3};
+    for (void *p : qs)
+      if (p) (void)hipFree(p);
+    HIPCHK(hipMalloc(&S.A1T, (size_t)SUR_H * rows * 2));
+    HIPCHK(hipMalloc(&S.dH2T, (size_t)SUR_H * rows * 2));
+    HIPCHK(hipMalloc(&S.dH1T, (size_t)SUR_H * rows * 2));
+    const int64_t nch = (rows + R - 1) / R;
+    HIPCHK(hipMalloc(&S.p1, (size_t)nch * SUR_F * SUR_H * sizeof(float)));
+    HIPCHK(hipMalloc(&S.p2, (size_t)nch * SUR_H * SUR_H * sizeof(float)));
+    HIPCHK(hipMalloc(&S.pb2, (size_t)nch * SUR_H * sizeof(float)));
+    HIPCHK(hipMalloc(&S.pw3, (size_t)(rows / 16) * (SUR_H + 1) * sizeof(float)));
+    S.work_rows = rows;
+  }
+  if (S.dS) (void)hipFree(S.dS);
+  if (S.sq) (void)hipFree(S.sq);
+  if (S.S) (void)hipFree(S.S);
+  const int64_t np = std::max<int64_t>(n_pose_total, 1);
+  HIPCHK(hipMalloc(&S.dS, np * sizeof(float)));
+  HIPCHK(hipMalloc(&S.sq, np * sizeof(float)));
+  HIPCHK(hipMalloc(&S.S, np * sizeof(float)));
+  return GD_OK;
+}
+
+// forward (+ backward and gradient when grad) over poses [p0, p1) of the dataset
+gd_status sur_batch(gd_ctx *ctx, int64_t p0, int64_t p1, bool grad, double *loss) {
+  auto &S = ctx->sur;
+  const int64_t r0 = S.off[p0], r1 = S.off[p1];
+  const int64_t nr = (r1 - r0 + 63) / 64 * 64;  // padded rows read from the dataset
+  const int np = (int)(p1 - p0);
+  const SurW W = sur_weights(ctx);
+  const unsigned short *X = S.X + r0 * SUR_F;
+  hipLaunchKernelGGL(gd_mlp_fwd_kernel, dim3((unsigned)(nr / 64)), dim3(256), 0, ctx->stream, X,
+                     nr, W, S.y + r0);
+  hipLaunchKernelGGL(gd_mlp_pose_kernel, dim3((np + 255) / 256), dim3(256), 0, ctx->stream, S.y,
+                     S.d_off, (int)p0, (int)p1, S.d_target, S.dS, S.sq, S.S);
+  if (grad) {
+    hipLaunchKernelGGL(gd_mlp_bwd_kernel, dim3((unsigned)(nr / 64)), dim3(256), 0, ctx->stream, X,
+                       nr, S.pose_of_row + r0, (int)p0, (int)p1, S.dS, W, S.A1T, S.dH2T, S.dH1T,
+                       S.work_rows, S.pw3);
+    const int R = 1024;
+    const int nch = (int)((nr + R - 1) / R);
+    hipLaunchKernelGGL(gd_mlp_gemm_kernel, dim3(nch), dim3(256), 0, ctx->stream,
+                       (const unsigned short *)(S.XT + r0), S.cap + 64, SUR_F,
+                       (const unsigned short *)S.dH1T, S.work_rows, nr, R, S.p1, (float *)nullptr);
+    hipLaunchKernelGGL(gd_mlp_gemm_kernel, dim3(nch), dim3(256), 0, ctx->stream,
+                       (const unsigned short *)S.A1T, S.work_rows, SUR_H,
+                       (const unsigned short *)S.dH2T, S.work_rows, nr, R, S.p2, S.pb2);
+    hipLaunchKernelGGL(gd_mlp_gradsum_kernel, dim3((SUR_NPARAM + 255) / 256), dim3(256), 0,
+                       ctx->stream, S.p1, nch, S.p2, nch, S.pb2, S.pw3, nr / 16, S.grad);
+  }
+  HIPCHK(hipGetLastError());
+  if (loss) {
+    std::vector<float> sq(np);
+    HIPCHK(hipMemcpyAsync(sq.data(), S.sq + p0, np * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double s = 0.0;
+    for (float v : sq) s += v;
+    *loss = s / np;
+  }
+  return GD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gd_sur_num_params(void) { return SUR_NPARAM; }
+
+gd_status gd_sur_build_grids(gd_ctx *ctx, float spacing, float half_width) {
+  if (!ctx || !(spacing > 0.0f) || !(half_width > spacing)) return GD_ERR_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  TRY(sync_complexes(ctx));
+  auto &S = ctx->sur;
+  const int n = (int)std::ceil(2.0f * half_width / spacing) + 1;
+  const size_t node_bytes = 32 * sizeof(float);
+  for (float *g : S.grid)
+    if (g) (void)hipFree(g);
+  S.grid.assign(ctx->cx.size(), nullptr);
+  S.g.assign(ctx->cx.size(), SurGrid{});
+  for (size_t c = 0; c < ctx->cx.size(); ++c) {
+    const HostComplex &h = ctx->cx[c];
+    const size_t nodes = (size_t)n * n * n;
+    HIPCHK(hipMalloc(&S.grid[c], (size_t)h.d.n_prt * nodes * node_bytes));
+    SurGrid G;
+    G.grid = S.grid[c];
+    G.ox = h.d.pocket_center[0] - half_width;
+    G.oy = h.d.pocket_center[1] - half_width;
+    G.oz = h.d.pocket_center[2] - half_width;
+    G.h = spacing;
+    G.n = n;
+    S.g[c] = G;
+    for (int p = 0; p < h.d.n_prt; ++p) {
+      hipLaunchKernelGGL(gd_sur_grid_kernel, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0,
+                         ctx->stream, h.dev.prt_xyze + (size_t)p * h.d.pnp, h.dev.prt_cls, h.d.pnp,
+                         G.ox, G.oy, G.oz, G.h, n, S.grid[c] + (size_t)p * nodes * 32);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GD_OK;
+}
+
+gd_status gd_sur_features(gd_ctx *ctx, int cid, int n, const int *idx_lig, const int *idx_prt,
+                          const float *movematrix, float *out) {
+  if (!ctx || cid < 0 || cid >= (int)ctx->cx.size() || n < 0) return GD_ERR_INVALID;
+  if ((int)ctx->sur.g.size() <= cid) return fail(ctx, GD_ERR_STATE, "call gd_sur_build_grids first");
+  if (n == 0) return GD_OK;
+  const HostComplex &h = ctx->cx[cid];
+  for (int i = 0; i < n; ++i)
+    if (idx_lig[i] < 0 || idx_lig[i] >= h.d.n_lig || idx_prt[i] < 0 || idx_prt[i] >= h.d.n_prt)
+      return fail(ctx, GD_ERR_INVALID, "pose %d: conformer index out of range", i);
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<void *> keep;
+  int *dl, *dp;
+  float *dm, *dout;
+  gd_status s = GD_OK;
+  const size_t rows = (size_t)n * h.d.lna;
+  if ((s = tmp_dev(ctx, keep, &dl, idx_lig, (size_t)n)) || (s = tmp_dev(ctx, keep, &dp, idx_prt, (size_t)n)) ||
+      (s = tmp_dev(ctx, keep, &dm, movematrix, (size_t)n * 6)) ||
+      (s = tmp_dev<float>(ctx, keep, &dout, nullptr, rows * SUR_NF))) {
+    free_all(keep);
+    return s;
+  }
+  hipLaunchKernelGGL(gd_sur_feat_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0,
+                     ctx->stream, ctx->d_cx, cid, ctx->sur.g[cid], n, dl, dp, dm, (int64_t)0,
+                     (unsigned short *)nullptr, (unsigned short *)nullptr, (int64_t)0, dout);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, rows * SUR_NF * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  free_all(keep);
+  if (e != hipSuccess) return fail(ctx, GD_ERR_HIP, "gd_sur_features: %s", hipGetErrorString(e));
+  return GD_OK;
+}
+
+gd_status gd_sur_set_params(gd_ctx *ctx, const float *params) {
+  if (!ctx || !params) return GD_ERR_INVALID;
+  HIPCHK(hipSetDevice(ctx->device));
+  TRY(sur_alloc_params(ctx));
+  auto &S = ctx->sur;
+  HIPCHK(hipMemcpyAsync(S.p, params, SUR_NPARAM * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(S.m, 0, SUR_NPARAM * sizeof(float), ctx->stream));
+  HIPCHK(hipMemsetAsync(S.v, 0, SUR_NPARAM * sizeof(float), ctx->stream));
+  S.adam_t = 0;
+  TRY(sur_refresh(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  S.have_params = true;
+  return GD_OK;
+}
+
+gd_status gd_sur_get_params(gd_ctx *ctx, float *params) {
+  if (!ctx || !params) return GD_ERR_INVALID;
+  if (!ctx->sur.have_params) return fail(ctx, GD_ERR_STATE, "no surrogate parameters");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipMemcpyAsync(params, ctx->sur.p, SUR_NPARAM * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GD_OK;
+}
+
+gd_status gd_sur_dataset_clear(gd_ctx *ctx) {
+  if (!ctx) return GD_ERR_INVALID;
+  auto &S = ctx->sur;
+  S.rows = 0;
+  S.off.assign(1, 0);
+  S.target.clear();
+  S.pose_cx.clear();
+  S.dirty = true;
+  return GD_OK;
+}
+
+gd_status gd_sur_dataset_reserve(gd_ctx *ctx, int64_t rows) {
+  if (!ctx || rows < 0) return GD_ERR_INVALID;
+  auto &S = ctx->sur;
+  if (rows <= S.cap) return GD_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  unsigned short *X = nullptr, *XT = nullptr;
+  const int64_t cap = (rows + 63) / 64 * 64;
+  HIPCHK(hipMalloc(&X, (size_t)(cap + 64) * SUR_F * 2));
+  HIPCHK(hipMalloc(&XT, (size_t)SUR_F * (cap + 64) * 2));
+  HIPCHK(hipMemsetAsync(X, 0, (size_t)(cap + 64) * SUR_F * 2, ctx->stream));
+  HIPCHK(hipMemsetAsync(XT, 0, (size_t)SUR_F * (cap + 64) * 2, ctx->stream));
+  if (S.rows > 0) {
+    HIPCHK(hipMemcpyAsync(X, S.X, (size_t)S.rows * SUR_F * 2, hipMemcpyDeviceToDevice, ctx->stream));
+    for (int f = 0; f < SUR_F; ++f)
+      HIPCHK(hipMemcpyAsync(XT + (size_t)f * (cap + 64), S.XT + (size_t)f * (S.cap + 64),
+                            (size_t)S.rows * 2, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (S.X) (void)hipFree(S.X);
+  if (S.XT) (void)hipFree(S.XT);
+  S.X = X;
+  S.XT = XT;
+  S.cap = cap;
+  S.dirty = true;
+  return GD_OK;
+}
+
+gd_status gd_sur_dataset_add(gd_ctx *ctx, int cid, int n, const int *idx_lig, const int *idx_prt,
+                             const float *movematrix, const float *target) {
+  if (!ctx || cid < 0 || cid >= (int)ctx->cx.size() || n < 0) return GD_ERR_INVALID;
+  if ((int)ctx->sur.g.size() <= cid) return fail(ctx, GD_ERR_STATE, "call gd_sur_build_grids first");
+  if (n == 0) return GD_OK;
+  if (!idx_lig || !idx_prt || !movematrix || !target) return GD_ERR_INVALID;
+  const HostComplex &h = ctx->cx[cid];
+  for (int i = 0; i < n; ++i)
+    if (idx_lig[i] < 0 || idx_lig[i] >= h.d.n_lig || idx_prt[i] < 0 || idx_prt[i] >= h.d.n_prt)
+      return fail(ctx, GD_ERR_INVALID, "pose %d: conformer index out of range", i);
+  auto &S = ctx->sur;
+  const int64_t add = (int64_t)n * h.d.lna;
+  if (S.rows + add > S.cap) TRY(gd_sur_dataset_reserve(ctx, std::max<int64_t>(S.rows + add, 2 * S.cap)));
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<void *> keep;
+  int *dl, *dp;
+  float *dm;
+  gd_status s = GD_OK;
+  if ((s = tmp_dev(ctx, keep, &dl, idx_lig, (size_t)n)) || (s = tmp_dev(ctx, keep, &dp, idx_prt, (size_t)n)) ||
+      (s = tmp_dev(ctx, keep, &dm, movematrix, (size_t)n * 6))) {
+    free_all(keep);
+    return s;
+  }
+  hipLaunchKernelGGL(gd_sur_feat_kernel, dim3((unsigned)((add + 255) / 256)), dim3(256), 0,
+                     ctx->stream, ctx->d_cx, cid, S.g[cid], n, dl, dp, dm, S.rows, S.X, S.XT,
+                     S.cap + 64, (float *)nullptr);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  free_all(keep);
+  if (e != hipSuccess) return fail(ctx, GD_ERR_HIP, "gd_sur_dataset_add: %s", hipGetErrorString(e));
+  for (int i = 0; i < n; ++i) {
+    S.off.push_back(S.off.back() + h.d.lna);
+    S.target.push_back(target[i]);
+    S.pose_cx.push_back(cid);
+  }
+  S.rows += add;
+  S.dirty = true;
+  return GD_OK;
+}
+
+int64_t gd_sur_dataset_size(const gd_ctx *ctx) { return ctx ? (int64_t)ctx->sur.target.size() : -1; }
+
+gd_status gd_sur_loss_grad(gd_ctx *ctx, int pose_begin, int pose_end, double *loss, float *grad) {
+  if (!ctx) return GD_ERR_INVALID;
+  auto &S = ctx->sur;
+  if (!S.have_params) return fail(ctx, GD_ERR_STATE, "no surrogate parameters");
+  if (pose_begin < 0 || pose_end > (int)S.target.size() || pose_begin >= pose_end)
+    return fail(ctx, GD_ERR_INVALID, "pose range [%d,%d) outside the dataset", pose_begin, pose_end);
+  HIPCHK(hipSetDevice(ctx->device));
+  TRY(sur_upload_index(ctx));
+  TRY(sur_work(ctx, S.off[pose_end] - S.off[pose_begin], (int64_t)S.target.size()));
+  TRY(sur_batch(ctx, pose_begin, pose_end, grad != nullptr, loss));
+  if (grad) {
+    HIPCHK(hipMemcpyAsync(grad, S.grad, SUR_NPARAM * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  return GD_OK;
+}
+
+gd_status gd_sur_predict(gd_ctx *ctx, int pose_begin, int pose_end, float *out) {
+  if (!ctx || !out) return GD_ERR_INVALID;
+  auto &S = ctx->sur;
+  if (!S.have_params) return fail(ctx, GD_ERR_STATE, "no surrogate parameters");
+  if (pose_begin < 0 || pose_end > (int)S.target.size() || pose_begin >= pose_end)
+    return fail(ctx, GD_ERR_INVALID, "pose range outside the dataset");
+  HIPCHK(hipSetDevice(ctx->device));
+  TRY(sur_upload_index(ctx));
+  TRY(sur_work(ctx, S.off[pose_end] - S.off[pose_begin], (int64_t)S.target.size()));
+  // in chunks that bound the work buffers
+  const int step = 65536;
+  for (int p = pose_begin; p < pose_end; p += step)
+    TRY(sur_batch(ctx, p, std::min(pose_end, p + step), false, nullptr));
+  HIPCHK(hipMemcpyAsync(out, S.S + pose_begin, (size_t)(pose_end - pose_begin) * sizeof(float),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GD_OK;
+}
+
+gd_status gd_sur_train(gd_ctx *ctx, int pose_begin, int pose_end, int epochs, int batch_poses,
+                       float lr, double *epoch_loss, double *seconds) {
+  if (!ctx || epochs < 0 || batch_poses <= 0 || !(lr > 0.0f)) return GD_ERR_INVALID;
+  auto &S = ctx->sur;
+  if (!S.have_params) return fail(ctx, GD_ERR_STATE, "no surrogate parameters");
+  if (pose_begin < 0 || pose_end > (int)S.target.size() || pose_begin >= pose_end)
+    return fail(ctx, GD_ERR_INVALID, "pose range outside the dataset");
+  HIPCHK(hipSetDevice(ctx->device));
+  TRY(sur_upload_index(ctx));
+  int64_t maxrows = 0;
+  for (int p = pose_begin; p < pose_end; p += batch_poses) {
+    const int q = std::min(pose_end, p + batch_poses);
+    maxrows = std::max<int64_t>(maxrows, S.off[q] - S.off[p]);
+  }
+  TRY(sur_work(ctx, maxrows, (int64_t)S.target.size()));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int ep = 0; ep < epochs; ++ep) {
+    for (int p = pose_begin; p < pose_end; p += batch_poses) {
+      const int q = std::min(pose_end, p + batch_poses);
+      TRY(sur_batch(ctx, p, q, true, nullptr));
+      S.adam_t += 1;
+      const float c1 = 1.0f - std::pow(b1, (float)S.adam_t);
+      const float c2 = 1.0f - std::pow(b2, (float)S.adam_t);
+      hipLaunchKernelGGL(gd_mlp_adam_kernel, dim3((SUR_NPARAM + 255) / 256), dim3(256), 0,
+                         ctx->stream, S.p, S.m, S.v, S.grad, lr, b1, b2, eps, c1, c2, S.W1T, S.W2T,
+                         S.W2R);
+      HIPCHK(hipGetLastError());
+    }
+    if (epoch_loss) {
+      // loss over the whole range with the epoch's final parameters
+      double tot = 0.0;
+      int64_t cnt = 0;
+      for (int p = pose_begin; p < pose_end; p += batch_poses) {
+        const int q = std::min(pose_end, p + batch_poses);
+        double l = 0.0;
+        TRY(sur_batch(ctx, p, q, false, &l));
+        tot += l * (q - p);
+        cnt += q - p;
+      }
+      epoch_loss[ep] = tot / (double)cnt;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return GD_OK;
 }
 

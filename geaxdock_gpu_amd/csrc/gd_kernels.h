@@ -51,6 +51,7 @@ struct DevComplex {
   const float *psp;        // [NLT * n_res]
   int2 *ref;               // [lna * KMAX] sorted point index, pmf0 bits
   int *ref_cnt;            // [lna]
+  const int *prt_cls;      // [pnp] receptor class t mod 4 (sorted order), surrogate features
 };
 
 // One workgroup's work: replicas (or poses) of one complex that share one

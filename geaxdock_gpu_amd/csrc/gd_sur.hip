@@ -1,0 +1,457 @@
+// gd_sur.hip -- learned energy surrogate: receptor feature grids, BF16 MFMA
+// MLP forward/backward (FP32 accumulate), FP32 Adam.
+//
+// Per ligand atom: 56 features (24 one-hot ligand type, 32 radial counts of
+// receptor points by class t mod 4 at 8 radial centres 1..8 A, Gaussian width
+// 1 A, smooth cosine cutoff at 8 A), plus a constant 1 in column 56 that
+// carries the first-layer bias. MLP 56 -> 128 -> 128 -> 1 with SiLU; a pose's
+// value is the sum over its atoms. Rows are padded to 64 columns (bf16).
+//
+// MFMA: v_mfma_f32_16x16x16_bf16 on wave64. Fragment layout used throughout
+// (lane l, i = 0..3):
+//   A[16x16]: A[l % 16][4 (l / 16) + i]      (row-major run of 4)
+//   B[16x16]: B[4 (l / 16) + i][l % 16]      (column-major run of 4)
+//   C[16x16]: C[4 (l / 16) + i][l % 16]
+#include "gd_device.h"
+#include "gd_kernels.h"
+
+namespace gd {
+
+constexpr int SUR_F = 64;        // padded feature width (56 + bias column + zeros)
+constexpr int SUR_NF = 56;       // real features
+constexpr int SUR_BIAS_COL = 56; // constant-1 column
+constexpr int SUR_H = 128;
+constexpr int SUR_NCLS = 4, SUR_NRBF = 8;
+constexpr float SUR_CUT = 8.0f;
+// parameter vector (FP32): W1[64][128] (row 56 = b1), W2[128][128], b2[128], w3[128], b3
+constexpr int SP_W1 = 0;
+constexpr int SP_W2 = SP_W1 + SUR_F * SUR_H;
+constexpr int SP_B2 = SP_W2 + SUR_H * SUR_H;
+constexpr int SP_W3 = SP_B2 + SUR_H;
+constexpr int SP_B3 = SP_W3 + SUR_H;
+constexpr int SUR_NPARAM = SP_B3 + 1;
+
+typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4_t;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
+
+__host__ __device__ inline unsigned short f2bf(float f) {
+  unsigned int u;
+  memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf2f(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
+
+__device__ inline float silu(float x) { return x * fast_rcp(1.0f + fast_exp2(-x * 1.44269504f)); }
+__device__ inline float dsilu(float x) {
+  const float s = fast_rcp(1.0f + fast_exp2(-x * 1.44269504f));
+  return s * (1.0f + x * (1.0f - s));
+}
+
+__device__ inline f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ inline bf16x4_t ld4(const unsigned short *p) {
+  return *reinterpret_cast<const bf16x4_t *>(p);
+}
+
+// ---------------------------------------------------------------------------
+// Radial features of one position against a receptor conformer (sorted
+// points with per-point class).
+// ---------------------------------------------------------------------------
+__device__ inline void radial32(float x, float y, float z, const float4 *xyze, const int *cls,
+                                int pnp, float (&f)[32]) {
+#pragma unroll
+  for (int i = 0; i < 32; ++i) f[i] = 0.0f;
+  const float c2 = SUR_CUT * SUR_CUT;
+  for (int p = 0; p < pnp; ++p) {
+    const float4 P = xyze[p];
+    const float d2 = pair_d2(x, y, z, P.x, P.y, P.z);
+    if (d2 < c2) {
+      const float d = sqrtf(d2);
+      const float fc = 0.5f * (1.0f + cosf(3.14159265358979f * d / SUR_CUT));
+      const int c = cls[p];
+#pragma unroll
+      for (int k = 0; k < SUR_NRBF; ++k) {
+        const float t = d - (float)(k + 1);
+        const float g = expf(-0.5f * t * t) * fc;
+#pragma unroll
+        for (int cc = 0; cc < SUR_NCLS; ++cc)
+          if (cc == c) f[cc * SUR_NRBF + k] += g;
+      }
+    }
+  }
+}
+
+// Feature grid of one protein conformer: n^3 nodes at origin + h*(i,j,k),
+// 32 floats per node, node index (i*n + j)*n + k.
+extern "C" __global__ void gd_sur_grid_kernel(const float4 *xyze, const int *cls, int pnp,
+                                              float ox, float oy, float oz, float h, int n,
+                                              float *grid) {
+  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)n * n * n;
+  if (node >= total) return;
+  const int k = (int)(node % n);
+  const int j = (int)((node / n) % n);
+  const int i = (int)(node / ((int64_t)n * n));
+  float f[32];
+  radial32(ox + h * i, oy + h * j, oz + h * k, xyze, cls, pnp, f);
+  float4 *o = reinterpret_cast<float4 *>(grid + node * 32);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) o[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
+}
+
+struct SurGrid {
+  const float *grid;  // [n_prt][n^3][32]
+  float ox, oy, oz, h;
+  int n;
+};
+
+// Trilinear interpolation of the 32 radial features at (x, y, z), clamped.
+__device__ inline void grid_radial(const SurGrid &G, int prt, float x, float y, float z,
+                                   float (&f)[32]) {
+  const int n = G.n;
+  const float lim = (float)(n - 1) - 1e-4f;
+  const float ux = fminf(fmaxf((x - G.ox) / G.h, 0.0f), lim);
+  const float uy = fminf(fmaxf((y - G.oy) / G.h, 0.0f), lim);
+  const float uz = fminf(fmaxf((z - G.oz) / G.h, 0.0f), lim);
+  const int ix = (int)ux, iy = (int)uy, iz = (int)uz;
+  const float tx = ux - ix, ty = uy - iy, tz = uz - iz;
+  const float *base = G.grid + (size_t)prt * n * n * n * 32;
+#pragma unroll
+  for (int q = 0; q < 32; ++q) f[q] = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int dx = c >> 2, dy = (c >> 1) & 1, dz = c & 1;
+    const float w = (dx ? tx : 1.0f - tx) * (dy ? ty : 1.0f - ty) * (dz ? tz : 1.0f - tz);
+    const float4 *src = reinterpret_cast<const float4 *>(
+        base + (((size_t)(ix + dx) * n + (iy + dy)) * n + (iz + dz)) * 32);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float4 v = src[q];
+      f[4 * q] = fmaf(w, v.x, f[4 * q]);
+      f[4 * q + 1] = fmaf(w, v.y, f[4 * q + 1]);
+      f[4 * q + 2] = fmaf(w, v.z, f[4 * q + 2]);
+      f[4 * q + 3] = fmaf(w, v.w, f[4 * q + 3]);
+    }
+  }
+}
+
+// Feature rows of poses: one thread per ligand atom of each pose. Writes
+// bf16 rows X [row][64] and, when xt != null, the transpose XT [64][ld].
+extern "C" __global__ void gd_sur_feat_kernel(const DevComplex *cxs, int cxi, SurGrid G, int n,
+                                              const int *idx_lig, const int *idx_prt,
+                                              const float *mm, int64_t row0,
+                                              unsigned short *X, unsigned short *XT,
+                                              int64_t ld, float *xf) {
+  const DevComplex &C = cxs[cxi];
+  const int lna = C.lna;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * lna) return;
+  const int pose = (int)(t / lna), l = (int)(t % lna);
+  const float *m = mm + (size_t)pose * 6;
+  const Rot R = euler(m[3], m[4], m[5]);
+  const float *orig = C.lig_orig + (size_t)idx_lig[pose] * 3 * lna;
+  const float ox = orig[l], oy = orig[lna + l], oz = orig[2 * lna + l];
+  const float x = R.r00 * ox + R.r01 * oy + R.r02 * oz + m[0] + C.pocket[0];
+  const float y = R.r10 * ox + R.r11 * oy + R.r12 * oz + m[1] + C.pocket[1];
+  const float z = R.r20 * ox + R.r21 * oy + R.r22 * oz + m[2] + C.pocket[2];
+  float f[32];
+  grid_radial(G, idx_prt[pose], x, y, z, f);
+  const int lt = C.lig_t[l];
+  const int64_t row = row0 + t;
+  float v[SUR_F];
+#pragma unroll
+  for (int q = 0; q < SUR_F; ++q) v[q] = 0.0f;
+#pragma unroll
+  for (int q = 0; q < NLT; ++q) v[q] = (q == lt) ? 1.0f : 0.0f;
+#pragma unroll
+  for (int q = 0; q < 32; ++q) v[NLT + q] = f[q];
+  v[SUR_BIAS_COL] = 1.0f;
+  if (X)
+#pragma unroll
+    for (int q = 0; q < SUR_F; ++q) X[row * SUR_F + q] = f2bf(v[q]);
+  if (XT)
+#pragma unroll
+    for (int q = 0; q < SUR_F; ++q) XT[(int64_t)q * ld + row] = f2bf(v[q]);
+  if (xf)
+#pragma unroll
+    for (int q = 0; q < SUR_NF; ++q) xf[(size_t)t * SUR_NF + q] = v[q];
+}
+
+// bf16 weight copies in the layouts the fragments read:
+//   W1T[128][64] = W1^T, W2T[128][128] = W2^T, W2R[128][128] = W2.
+struct SurW {
+  const unsigned short *W1T, *W2T, *W2R;
+  const float *b2, *w3, *b3;
+};
+
+// Forward of one wave's 16 rows. Returns h1, h2 pre-activations in C
+// layout; writes bf16 A1 rows into the wave's LDS tile a1s[16][128].
+__device__ inline void mlp_fwd16(const unsigned short *X, int64_t r0, const SurW &W,
+                                 unsigned short *a1s, f32x4_t (&h1)[8], f32x4_t (&h2)[8]) {
+  const int l = threadIdx.x & 63;
+  const int lr = l & 15, lq = l >> 4;
+  bf16x4_t xa[4];
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) xa[ks] = ld4(X + (r0 + lr) * SUR_F + 16 * ks + 4 * lq);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    f32x4_t c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+      c = mfma16(xa[ks], ld4(W.W1T + (16 * t + lr) * SUR_F + 16 * ks + 4 * lq), c);
+    h1[t] = c;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a1s[(4 * lq + i) * SUR_H + 16 * t + lr] = f2bf(silu(c[i]));
+  }
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): LDS writes visible to the wave
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    f32x4_t c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks)
+      c = mfma16(ld4(a1s + lr * SUR_H + 16 * ks + 4 * lq),
+                 ld4(W.W2T + (16 * u + lr) * SUR_H + 16 * ks + 4 * lq), c);
+    const float bb = W.b2[16 * u + lr];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c[i] += bb;
+    h2[u] = c;
+  }
+}
+
+// y for the wave's 16 rows: lane with l%16 == 0 holds rows 4*(l/16)+i.
+__device__ inline void mlp_out16(const f32x4_t (&h2)[8], const SurW &W, float (&y)[4]) {
+  const int l = threadIdx.x & 63;
+  const int lr = l & 15;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) y[i] = 0.0f;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const float w = W.w3[16 * u + lr];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) y[i] = fmaf(silu(h2[u][i]), w, y[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float v = y[i];
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 8);
+    y[i] = v + W.b3[0];
+  }
+}
+
+// Per-row outputs of a batch: 4 waves x 16 rows per workgroup.
+extern "C" __global__ void __launch_bounds__(256) gd_mlp_fwd_kernel(const unsigned short *X,
+                                                                    int64_t n_rows, SurW W,
+                                                                    float *y) {
+  __shared__ unsigned short a1s[4][16 * SUR_H];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t r0 = ((int64_t)blockIdx.x * 4 + wv) * 16;
+  if (r0 >= n_rows) return;  // whole wave leaves; no block barrier below
+  f32x4_t h1[8], h2[8];
+  mlp_fwd16(X, r0, W, a1s[wv], h1, h2);
+  float yy[4];
+  mlp_out16(h2, W, yy);
+  if ((l & 15) == 0)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) y[r0 + 4 * (l >> 4) + i] = yy[i];
+}
+
+// Pose sums over poses [p0, p1): S = sum of row outputs, residual against the
+// target, dS = 2 (S - target) / (p1 - p0), squared residuals for the loss.
+// Row offsets are absolute; y is indexed by absolute row.
+extern "C" __global__ void gd_mlp_pose_kernel(const float *y, const int64_t *off, int p0, int p1,
+                                              const float *target, float *dS, float *sq,
+                                              float *S_out) {
+  const int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= p1) return;
+  float s = 0.0f;
+  for (int64_t r = off[p]; r < off[p + 1]; ++r) s += y[r];
+  const float res = s - target[p];
+  dS[p] = 2.0f * res / (float)(p1 - p0);
+  sq[p] = res * res;
+  if (S_out) S_out[p] = s;
+}
+
+// Backward of one batch: recomputes the forward, then per wave
+//   dH2 = dy w3^T * silu'(H2);  dA1 = dH2 W2^T;  dH1 = dA1 * silu'(H1)
+// and writes A1T[128][ld], dH2T[128][ld], dH1T[128][ld] (bf16, column per
+// row) for the weight-gradient GEMMs, plus per-wave partials of grad w3, b3.
+extern "C" __global__ void __launch_bounds__(256) gd_mlp_bwd_kernel(
+    const unsigned short *X, int64_t n_rows, const int *pose_of_row, int p0, int p1,
+    const float *dS, SurW W, unsigned short *A1T, unsigned short *dH2T, unsigned short *dH1T,
+    int64_t ld, float *gw3_part) {
+  __shared__ unsigned short tile[4][16 * SUR_H];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int lr = l & 15, lq = l >> 4;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + wv;
+  const int64_t r0 = wave * 16;
+  if (r0 >= n_rows) return;
+  unsigned short *ts = tile[wv];
+  f32x4_t h1[8], h2[8];
+  mlp_fwd16(X, r0, W, ts, h1, h2);
+  // A1^T to global (rows r0 + 4 lq + i of column 16 t + lr)
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    unsigned short v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = f2bf(silu(h1[t][i]));
+    *reinterpret_cast<uint2 *>(A1T + (int64_t)(16 * t + lr) * ld + r0 + 4 * lq) =
+        make_uint2(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16));
+  }
+  float dy[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int p = pose_of_row[r0 + 4 * lq + i];
+    dy[i] = (p >= p0 && p < p1) ? dS[p] : 0.0f;
+  }
+  // grad w3 (column 16 u + lr) and b3
+  float gw[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    float s = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s = fmaf(silu(h2[u][i]), dy[i], s);
+    s += __shfl_xor(s, 16);
+    s += __shfl_xor(s, 32);
+    gw[u] = s;
+  }
+  float gb = dy[0] + dy[1] + dy[2] + dy[3];
+  gb += __shfl_xor(gb, 16);
+  gb += __shfl_xor(gb, 32);
+  if (lq == 0) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) gw3_part[wave * (SUR_H + 1) + 16 * u + lr] = gw[u];
+    if (lr == 0) gw3_part[wave * (SUR_H + 1) + SUR_H] = gb;
+  }
+  // dH2 into the wave tile (row-major, for dA1) and to global (transposed)
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const float w = W.w3[16 * u + lr];
+    unsigned short v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[i] = f2bf(dy[i] * w * dsilu(h2[u][i]));
+      ts[(4 * lq + i) * SUR_H + 16 * u + lr] = v[i];
+    }
+    *reinterpret_cast<uint2 *>(dH2T + (int64_t)(16 * u + lr) * ld + r0 + 4 * lq) =
+        make_uint2(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16));
+  }
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  __builtin_amdgcn_wave_barrier();
+  // dA1 = dH2 W2^T: B[j][i] = W2[i][j]
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    f32x4_t c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks)
+      c = mfma16(ld4(ts + lr * SUR_H + 16 * ks + 4 * lq),
+                 ld4(W.W2R + (16 * t + lr) * SUR_H + 16 * ks + 4 * lq), c);
+    unsigned short v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = f2bf(c[i] * dsilu(h1[t][i]));
+    *reinterpret_cast<uint2 *>(dH1T + (int64_t)(16 * t + lr) * ld + r0 + 4 * lq) =
+        make_uint2(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16));
+  }
+}
+
+// Weight gradient partials G[M][128] = P^T Q over rows [c*R, (c+1)*R):
+// P^T given as PT[M][ld] (M = 64 or 128), Q^T as QT[128][ld]. One workgroup
+// (4 waves) per chunk; wave w owns output rows [16*mt, ...) for
+// mt = w, w + 4, .... Column sums of Q (bias gradient) when qsum != null.
+extern "C" __global__ void __launch_bounds__(256) gd_mlp_gemm_kernel(
+    const unsigned short *PT, int64_t ldp, int M, const unsigned short *QT, int64_t ldq,
+    int64_t n_rows, int R, float *part, float *qsum) {
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int lr = l & 15, lq = l >> 4;
+  const int64_t b = (int64_t)blockIdx.x * R;
+  const int64_t e = b + R < n_rows ? b + R : n_rows;
+  const int mtiles = M / 16;
+  float *out = part + (int64_t)blockIdx.x * M * SUR_H;
+  for (int mt = wv; mt < mtiles; mt += 4) {
+    f32x4_t c[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) c[n] = {0.f, 0.f, 0.f, 0.f};
+    float cs[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) cs[n] = 0.0f;
+    for (int64_t k0 = b; k0 < e; k0 += 16) {
+      const bf16x4_t a = ld4(PT + (int64_t)(16 * mt + lr) * ldp + k0 + 4 * lq);
+#pragma unroll
+      for (int n = 0; n < 8; ++n) {
+        const bf16x4_t q = ld4(QT + (int64_t)(16 * n + lr) * ldq + k0 + 4 * lq);
+        c[n] = mfma16(a, q, c[n]);
+        if (qsum && mt == 0)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) cs[n] += bf2f((unsigned short)q[i]);
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) out[(16 * mt + 4 * lq + i) * SUR_H + 16 * n + lr] = c[n][i];
+    if (qsum && mt == 0) {
+#pragma unroll
+      for (int n = 0; n < 8; ++n) {
+        float s = cs[n];
+        s += __shfl_xor(s, 16);
+        s += __shfl_xor(s, 32);
+        if (lq == 0) qsum[(int64_t)blockIdx.x * SUR_H + 16 * n + lr] = s;
+      }
+    }
+  }
+}
+
+// Sums gradient partials into g[SUR_NPARAM] (fixed order: deterministic).
+extern "C" __global__ void gd_mlp_gradsum_kernel(const float *p1, int n1, const float *p2, int n2,
+                                                 const float *pb2, const float *pw3, int64_t nw,
+                                                 float *g) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= SUR_NPARAM) return;
+  float s = 0.0f;
+  if (q < SP_W2) {
+    for (int c = 0; c < n1; ++c) s += p1[(int64_t)c * SUR_F * SUR_H + q];
+  } else if (q < SP_B2) {
+    for (int c = 0; c < n2; ++c) s += p2[(int64_t)c * SUR_H * SUR_H + (q - SP_W2)];
+  } else if (q < SP_W3) {
+    for (int c = 0; c < n2; ++c) s += pb2[(int64_t)c * SUR_H + (q - SP_B2)];
+  } else {
+    const int j = q - SP_W3;  // w3[j] or b3 (j == 128)
+    for (int64_t w = 0; w < nw; ++w) s += pw3[w * (SUR_H + 1) + j];
+  }
+  g[q] = s;
+}
+
+// Adam on the FP32 master parameters; refreshes the bf16 layouts.
+extern "C" __global__ void gd_mlp_adam_kernel(float *p, float *m, float *v, const float *g,
+                                              float lr, float b1, float b2, float eps, float c1,
+                                              float c2, unsigned short *W1T, unsigned short *W2T,
+                                              unsigned short *W2R) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= SUR_NPARAM) return;
+  float x = p[q];
+  if (lr > 0.0f) {
+    const float gq = g[q];
+    const float mq = b1 * m[q] + (1.0f - b1) * gq;
+    const float vq = b2 * v[q] + (1.0f - b2) * gq * gq;
+    m[q] = mq;
+    v[q] = vq;
+    x -= lr * (mq / c1) / (sqrtf(vq / c2) + eps);
+    p[q] = x;
+  }
+  if (q < SP_W2) {
+    const int f = q / SUR_H, i = q % SUR_H;
+    W1T[i * SUR_F + f] = f2bf(x);
+  } else if (q < SP_B2) {
+    const int i = (q - SP_W2) / SUR_H, j = (q - SP_W2) % SUR_H;
+    W2T[j * SUR_H + i] = f2bf(x);
+    W2R[i * SUR_H + j] = f2bf(x);
+  }
+}
+
+}  // namespace gd

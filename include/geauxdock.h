@@ -242,6 +242,35 @@ gd_status gd_philox(gd_ctx *ctx, int n, const uint32_t *ctr, const uint32_t *key
 /* Number of replicas of a complex (n_lig*n_prt*n_tmp). */
 int gd_num_replicas(const gd_ctx *ctx, int complex_id);
 
+/* ---- learned energy surrogate -------------------------------------------
+ * No counterpart in the reference. A small MLP (56 -> 128 -> 128 -> 1, SiLU,
+ * summed over ligand atoms) on per-atom features: one-hot ligand type (24) and
+ * 32 radial counts of receptor points by class (t mod 4) at 1..8 A with a
+ * smooth 8 A cutoff, read from per-protein-conformer grids. Trained on the GPU
+ * with BF16 MFMA forward/backward kernels (FP32 accumulate) and FP32 Adam.
+ * Parameter vector (FP32, gd_sur_num_params() floats): W1[64][128] (row 56 is
+ * the first-layer bias, rows 57..63 unused), W2[128][128], b2[128], w3[128], b3. */
+int gd_sur_num_params(void);
+/* Feature grids around each complex's pocket (spacing, half width in A). */
+gd_status gd_sur_build_grids(gd_ctx *ctx, float spacing, float half_width);
+/* Features of n poses: out[(pose*lna + atom)*56 + f]. */
+gd_status gd_sur_features(gd_ctx *ctx, int complex_id, int n, const int *idx_lig,
+                          const int *idx_prt, const float *movematrix, float *out);
+gd_status gd_sur_set_params(gd_ctx *ctx, const float *params);
+gd_status gd_sur_get_params(gd_ctx *ctx, float *params);
+/* Training set held on the device: poses of a complex with their targets. */
+gd_status gd_sur_dataset_clear(gd_ctx *ctx);
+gd_status gd_sur_dataset_reserve(gd_ctx *ctx, int64_t rows);
+gd_status gd_sur_dataset_add(gd_ctx *ctx, int complex_id, int n, const int *idx_lig,
+                             const int *idx_prt, const float *movematrix, const float *target);
+int64_t gd_sur_dataset_size(const gd_ctx *ctx);
+/* Mean squared error over poses [pose_begin, pose_end) and its gradient. */
+gd_status gd_sur_loss_grad(gd_ctx *ctx, int pose_begin, int pose_end, double *loss, float *grad);
+gd_status gd_sur_predict(gd_ctx *ctx, int pose_begin, int pose_end, float *out);
+/* Adam over contiguous batches of batch_poses poses; epoch_loss[epochs]. */
+gd_status gd_sur_train(gd_ctx *ctx, int pose_begin, int pose_end, int epochs, int batch_poses,
+                       float lr, double *epoch_loss, double *seconds);
+
 #ifdef __cplusplus
 }
 #endif
