@@ -58,6 +58,9 @@ def load_library():
     lib.gd_philox.argtypes = [vp, C.c_int, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     lib.gd_num_replicas.argtypes = [vp, C.c_int]
     lib.gd_num_replicas.restype = C.c_int
+    lib.gd_run_da.argtypes = [vp, C.c_int64, C.c_int, C.c_int, C.c_float, abi.RECORD_SINK, vp,
+                              P(abi.RunStats)]
+    lib.gd_run_da.restype = C.c_int
     lib.gd_sur_num_params.argtypes = []
     lib.gd_sur_num_params.restype = C.c_int
     lib.gd_sur_build_grids.argtypes = [vp, C.c_float, C.c_float]
@@ -163,6 +166,20 @@ class Engine:
             cb = abi.RECORD_SINK(_cb)
         self._check(self.lib.gd_run(self._ctx, int(n_steps), int(steps_per_exchange), cb, None,
                                     C.byref(st)))
+        return {f: getattr(st, f) for f, _ in abi.RunStats._fields_}
+
+    def run_da(self, n_steps, steps_per_exchange, k_sub, sur_scale, sink=None):
+        """Delayed-acceptance run: k_sub surrogate steps per exact evaluation."""
+        st = abi.RunStats()
+        if sink is None:
+            cb = abi.RECORD_SINK()
+        else:
+            def _cb(user, cid, recs, n):
+                buf = C.cast(recs, C.POINTER(C.c_char * (n * abi.RECORD_DTYPE.itemsize))).contents
+                sink(cid, np.frombuffer(bytes(buf), dtype=abi.RECORD_DTYPE).copy())
+            cb = abi.RECORD_SINK(_cb)
+        self._check(self.lib.gd_run_da(self._ctx, int(n_steps), int(steps_per_exchange),
+                                       int(k_sub), float(sur_scale), cb, None, C.byref(st)))
         return {f: getattr(st, f) for f, _ in abi.RunStats._fields_}
 
     def num_replicas(self, cid):

@@ -15,8 +15,8 @@
 #include <string>
 #include <vector>
 
-#include "gd_mc.hip"
 #include "gd_sur.hip"
+#include "gd_mc.hip"
 
 using namespace gd;
 
@@ -706,14 +706,35 @@ gd_status drain_records(gd_ctx *ctx, gd_record_sink sink, void *user) {
   return GD_OK;
 }
 
-gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float *ms) {
+gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float *ms,
+                    int da_k = 0, float sur_scale = 1.0f) {
   KParams P = base_params(ctx);
   P.mode = mode;
   P.step0 = step0;
   P.n_steps = n_steps;
   P.nc = nc;
+  size_t lds = ctx->lds;
+  if (da_k > 0 && mode == MODE_MC) {
+    auto &S = ctx->sur;
+    P.da_k = da_k;
+    P.sur_scale = sur_scale;
+    P.sW1T = S.W1T;
+    P.sW2T = S.W2T;
+    P.sb2 = S.p + SP_B2;
+    P.sw3 = S.p + SP_W3;
+    P.sb3 = S.p + SP_B3;
+    P.off_da = (int)((lds + 15) & ~(size_t)15);
+    lds = P.off_da + (size_t)(ctx->block / 64) * 6208;
+    if (lds > ctx->max_lds)
+      return fail(ctx, GD_ERR_CAPACITY, "delayed acceptance needs %zu bytes of LDS", lds);
+    (void)hipFuncSetAttribute((const void *)gd_mc_da_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  }
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  hipLaunchKernelGGL(gd_mc_kernel, dim3(ctx->n_packs), dim3(ctx->block), ctx->lds, ctx->stream, P);
+  if (P.da_k > 0)
+    hipLaunchKernelGGL(gd_mc_da_kernel, dim3(ctx->n_packs), dim3(ctx->block), lds, ctx->stream, P);
+  else
+    hipLaunchKernelGGL(gd_mc_kernel, dim3(ctx->n_packs), dim3(ctx->block), lds, ctx->stream, P);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(hipEventSynchronize(ctx->ev1));
@@ -725,8 +746,30 @@ gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float
 
 extern "C" {
 
+static gd_status run_impl(gd_ctx *ctx, int64_t n_steps, int steps_per_exchange,
+                          gd_record_sink sink, void *user, gd_run_stats *stats, int da_k,
+                          float sur_scale);
+
 gd_status gd_run(gd_ctx *ctx, int64_t n_steps, int steps_per_exchange, gd_record_sink sink,
                  void *user, gd_run_stats *stats) {
+  return run_impl(ctx, n_steps, steps_per_exchange, sink, user, stats, 0, 1.0f);
+}
+
+gd_status gd_run_da(gd_ctx *ctx, int64_t n_steps, int steps_per_exchange, int k_sub,
+                    float sur_scale, gd_record_sink sink, void *user, gd_run_stats *stats) {
+  if (!ctx) return GD_ERR_INVALID;
+  if (k_sub < 1) return fail(ctx, GD_ERR_INVALID, "k_sub must be >= 1");
+  if (ctx->sur.g.size() != ctx->cx.size()) return fail(ctx, GD_ERR_STATE, "call gd_sur_build_grids first");
+  if (!ctx->sur.have_params) return fail(ctx, GD_ERR_STATE, "no surrogate parameters");
+  if (n_steps % k_sub != 0 || (steps_per_exchange > 0 && steps_per_exchange % k_sub != 0) ||
+      ctx->step % k_sub != 0)
+    return fail(ctx, GD_ERR_INVALID, "step counts must be multiples of k_sub");
+  return run_impl(ctx, n_steps, steps_per_exchange, sink, user, stats, k_sub, sur_scale);
+}
+
+static gd_status run_impl(gd_ctx *ctx, int64_t n_steps, int steps_per_exchange,
+                          gd_record_sink sink, void *user, gd_run_stats *stats, int da_k,
+                          float sur_scale) {
   if (!ctx) return GD_ERR_INVALID;
   if (n_steps < 0 || steps_per_exchange < 0) return fail(ctx, GD_ERR_INVALID, "negative step count");
   if (ctx->step + n_steps > 2000000000LL) return fail(ctx, GD_ERR_CAPACITY, "step counter overflow");
@@ -745,9 +788,10 @@ gd_status gd_run(gd_ctx *ctx, int64_t n_steps, int steps_per_exchange, gd_record
   int64_t left = n_steps;
   double t_ms = 0.0;
   while (left > 0) {
-    const int win = (int)std::min<int64_t>(left, ctx->opts.steps_per_dump);
+    int win = (int)std::min<int64_t>(left, ctx->opts.steps_per_dump);
+    if (da_k > 0) win = std::max(da_k, win / da_k * da_k);
     float ms = 0.f;
-    TRY(launch_mc(ctx, MODE_MC, (int)ctx->step, win, steps_per_exchange, &ms));
+    TRY(launch_mc(ctx, MODE_MC, (int)ctx->step, win, steps_per_exchange, &ms, da_k, sur_scale));
     t_ms += ms;
     st.kernel_launches += 1;
     ctx->step += win;
@@ -1196,6 +1240,13 @@ gd_status gd_sur_build_grids(gd_ctx *ctx, float spacing, float half_width) {
     G.h = spacing;
     G.n = n;
     S.g[c] = G;
+    ctx->cx[c].dev.sgrid = G.grid;
+    ctx->cx[c].dev.sox = G.ox;
+    ctx->cx[c].dev.soy = G.oy;
+    ctx->cx[c].dev.soz = G.oz;
+    ctx->cx[c].dev.sh = G.h;
+    ctx->cx[c].dev.sn = G.n;
+    ctx->cx_dirty = true;
     for (int p = 0; p < h.d.n_prt; ++p) {
       hipLaunchKernelGGL(gd_sur_grid_kernel, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0,
                          ctx->stream, h.dev.prt_xyze + (size_t)p * h.d.pnp, h.dev.prt_cls, h.d.pnp,
@@ -1204,6 +1255,7 @@ gd_status gd_sur_build_grids(gd_ctx *ctx, float spacing, float half_width) {
     }
   }
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  TRY(sync_complexes(ctx));
   return GD_OK;
 }
 

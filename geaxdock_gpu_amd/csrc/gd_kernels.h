@@ -52,6 +52,10 @@ struct DevComplex {
   int2 *ref;               // [lna * KMAX] sorted point index, pmf0 bits
   int *ref_cnt;            // [lna]
   const int *prt_cls;      // [pnp] receptor class t mod 4 (sorted order), surrogate features
+  // surrogate feature grid [n_prt][sn^3][32] at origin (sox, soy, soz), spacing sh
+  const float *sgrid;
+  float sox, soy, soz, sh;
+  int sn;
 };
 
 // One workgroup's work: replicas (or poses) of one complex that share one
@@ -86,6 +90,12 @@ struct KParams {
   uint32_t k0, k1;
   // LDS layout (bytes)
   int off_hpp, off_psp, off_par, off_kde, off_kr, off_ch, off_xe, off_xi, off_xa, off_xo;
+  int off_da;  // per-wave surrogate scratch (MODE_MC with da_k > 0)
+  // delayed acceptance: k surrogate sub-steps per exact evaluation (0 = plain)
+  int da_k;
+  float sur_scale;
+  const unsigned short *sW1T, *sW2T;
+  const float *sb2, *sw3, *sb3;
 };
 
 }  // namespace gd

@@ -364,6 +364,119 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
   E.cms = dv != 0.0f ? (tp0 * tn0 - fp0 * fn0) / dv : 0.0f;
 }
 
+// Weighted KDE, MCS and pocket-distance terms (e[6..8] after normalization)
+// of one pose: the cheap part of the energy, evaluated exactly inside the
+// surrogate (src/kernel_cuda_l2_calcenergy.cu:224-326).
+template <int APL>
+__device__ __forceinline__ float cheap_terms(const DevComplex &C, const Lds &S, int lpr,
+                                             const float (&x)[APL], const float (&y)[APL],
+                                             const float (&z)[APL], const int (&lt)[APL],
+                                             const int (&li)[APL], const bool (&val)[APL],
+                                             const float mm[6]) {
+  float t_kde = 0.0f;
+  const float kde2l = C.kde2l;
+#pragma unroll
+  for (int k = 0; k < APL; ++k) {
+    if (!val[k]) continue;
+    const int2 kr = S.kr[lt[k]];
+    float ks = 0.0f;
+    for (int j = kr.x; j < kr.y; ++j) {
+      const float4 Q = S.kde[j];
+      ks += fast_exp2(kde2l * pair_d2(x[k], y[k], z[k], Q.x, Q.y, Q.z));
+    }
+    if (kr.y > kr.x) t_kde += ks / (float)(kr.y - kr.x);
+  }
+  float elhm = 0.0f;
+  const int pos = C.pos;
+  for (int m = 0; m < pos; ++m) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < APL; ++k) {
+      if (!val[k]) continue;
+      const float4 M = C.mcs[(size_t)m * C.lna + li[k]];
+      if (M.w != 0.0f) s += pair_d2(x[k], y[k], z[k], M.x, M.y, M.z);
+    }
+    s = seg_sum(s, lpr);
+    const float n = C.mcs_cnt[m];
+    if (n > 0.0f) elhm += C.mcs_tcc[m] * sqrtf(s / n);
+  }
+  t_kde = seg_sum(t_kde, lpr);
+  const float lna = (float)C.lna;
+  const float e6 = (t_kde / C.kde3) / lna;
+  const float e7 = pos > 0 ? logf(elhm / (float)pos) : 0.0f;
+  const float cx = C.pocket[0] + mm[0], cy = C.pocket[1] + mm[1], cz = C.pocket[2] + mm[2];
+  const float dx = cx - C.pocket[0], dy = cy - C.pocket[1], dz = cz - C.pocket[2];
+  const float e8 = sqrtf(dx * dx + dy * dy + dz * dz);
+  return (C.a_para[6] * e6 + C.b_para[6]) * C.w[6] + (C.a_para[7] * e7 + C.b_para[7]) * C.w[7] +
+         (C.a_para[8] * e8 + C.b_para[8]) * C.w[8];
+}
+
+// Surrogate total of the pose held by each replica of the wave: the MLP's
+// receptor-pair estimate (scaled) plus the exact cheap terms. Rows are the
+// wave's atoms, 16 per MFMA tile: tile (k, sub) holds atom k of lanes
+// 16 sub .. 16 sub + 15. `da` is the wave's scratch: X tile [16][64] bf16,
+// A1 tile [16][128] bf16, y[16].
+template <int APL>
+__device__ __forceinline__ float sur_total(const KParams &P, const DevComplex &C, const Lds &S,
+                                           unsigned char *da, int lpr, int prt,
+                                           const float (&x)[APL], const float (&y)[APL],
+                                           const float (&z)[APL], const int (&lt)[APL],
+                                           const int (&li)[APL], const bool (&val)[APL],
+                                           const float mm[6]) {
+  const int l = threadIdx.x & 63;
+  unsigned short *Xs = reinterpret_cast<unsigned short *>(da);
+  unsigned short *A1s = Xs + 16 * SUR_F;
+  float *ys = reinterpret_cast<float *>(A1s + 16 * SUR_H);
+  SurW W;
+  W.W1T = P.sW1T;
+  W.W2T = P.sW2T;
+  W.W2R = nullptr;
+  W.b2 = P.sb2;
+  W.w3 = P.sw3;
+  W.b3 = P.sb3;
+  SurGrid G;
+  G.grid = C.sgrid;
+  G.ox = C.sox;
+  G.oy = C.soy;
+  G.oz = C.soz;
+  G.h = C.sh;
+  G.n = C.sn;
+  float ysum = 0.0f;
+#pragma unroll
+  for (int k = 0; k < APL; ++k) {
+    float f[32];
+    grid_radial(G, prt, x[k], y[k], z[k], f);
+    for (int sub = 0; sub < 4; ++sub) {
+      if ((l >> 4) == sub) {
+        unsigned short *row = Xs + (l & 15) * SUR_F;
+#pragma unroll
+        for (int q = 0; q < NLT; ++q) row[q] = (q == lt[k]) ? (unsigned short)0x3F80 : (unsigned short)0;
+#pragma unroll
+        for (int q = 0; q < 32; ++q) row[NLT + q] = f2bf(f[q]);
+        row[SUR_BIAS_COL] = 0x3F80;  // 1.0
+#pragma unroll
+        for (int q = SUR_BIAS_COL + 1; q < SUR_F; ++q) row[q] = 0;
+      }
+      __builtin_amdgcn_s_waitcnt(0xc07f);
+      __builtin_amdgcn_wave_barrier();
+      f32x4_t h1[8], h2[8];
+      mlp_fwd16(Xs, 0, W, A1s, h1, h2);
+      float yy[4];
+      mlp_out16(h2, W, yy);
+      if ((l & 15) == 0)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ys[4 * (l >> 4) + i] = yy[i];
+      __builtin_amdgcn_s_waitcnt(0xc07f);
+      __builtin_amdgcn_wave_barrier();
+      if ((l >> 4) == sub && val[k]) ysum += ys[l & 15];
+      __builtin_amdgcn_s_waitcnt(0xc07f);
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  ysum = seg_sum(ysum, lpr);
+  return P.sur_scale * ysum + cheap_terms<APL>(C, S, lpr, x, y, z, lt, li, val, mm);
+}
+
 // ExchangeReplicas_d for one ladder (src/kernel_cuda_l1_exchangereplicas.cu:33-78):
 // temperature i1 = 2 pair + mode_t and i1 + 1 trade slots when
 // exp((mb1 - mb2)(E2 - E1)) > U.
@@ -393,7 +506,7 @@ __device__ __forceinline__ void exchange_ladder(int n_tmp, int mode_t, const flo
   }
 }
 
-template <int APL>
+template <int APL, bool DA>
 __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const DevComplex &C, const Lds &S,
                          int lpr) {
   const int tid = threadIdx.x;
@@ -489,8 +602,39 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
 #pragma unroll
   for (int i = 0; i < 6; ++i) scale[i] = C.move_scale[i];
 
-  for (int s = P.step0; s < P.step0 + P.n_steps; ++s) {
+  auto record = [&](const gd_energy &E, int step) {
+    if (P.rec && active) {
+      if (a == 0 && rc < P.rec_cap) {
+        gd_record &r = P.rec[(size_t)sv * P.rec_cap + rc];
+        r.replica.idx_rep = flat;
+        r.replica.idx_prt = pk.prt;
+        r.replica.idx_tmp = S.xi[sl];
+        r.replica.idx_lig = ilig;
+        r.energy = E;
+        for (int i = 0; i < 6; ++i) r.movematrix[i] = mo[i];
+        r.step = step;
+      }
+      rc += 1;
+    }
+  };
+  // delayed acceptance (surrogate transition): K surrogate Metropolis
+  // sub-steps, then one exact test of the composite move with
+  // min(1, exp(-beta [(E(y) - E(x)) - (S(y) - S(x))]))
+  const int K = DA ? P.da_k : 1;
+  unsigned char *da = nullptr;
+  float s_old = 0.0f;
+  if constexpr (DA) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_da[];
+    da = lds_da + P.off_da + (threadIdx.x >> 6) * 6208;
     if (wave_active) {
+      place(mo);
+      s_old = sur_total<APL>(P, C, S, da, lpr, pk.prt, x, y, z, lt, li, val, mo);
+    }
+  }
+
+  for (int s = P.step0; s < P.step0 + P.n_steps; s += K) {
+    if (wave_active) {
+    if constexpr (!DA) {
       float u[8], dm[6], mn[6];
       mc_uniforms(P.k0, P.k1, cid, flat, s, u);
       move_offsets(u, scale, C.move_kind, dm);
@@ -504,27 +648,55 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
         for (int i = 0; i < 6; ++i) mo[i] = mn[i];
         eo = E;
         acs += 1;
-        if (P.rec && active) {
-          if (a == 0 && rc < P.rec_cap) {
-            gd_record &r = P.rec[(size_t)sv * P.rec_cap + rc];
-            r.replica.idx_rep = flat;
-            r.replica.idx_prt = pk.prt;
-            r.replica.idx_tmp = S.xi[sl];
-            r.replica.idx_lig = ilig;
-            r.energy = E;
-            for (int i = 0; i < 6; ++i) r.movematrix[i] = mo[i];
-            r.step = s;
-          }
-          rc += 1;
+        record(E, s);
+      }
+    } else {
+      float yv[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) yv[i] = mo[i];
+      float s_y = s_old, u7 = 1.0f;
+      bool moved = false;
+      for (int j = 0; j < K; ++j) {
+        float u[8], dm[6], mn[6];
+        mc_uniforms(P.k0, P.k1, cid, flat, s + j, u);
+        move_offsets(u, scale, C.move_kind, dm);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) mn[i] = yv[i] + dm[i];
+        place(mn);
+        const float s_n = sur_total<APL>(P, C, S, da, lpr, pk.prt, x, y, z, lt, li, val, mn);
+        if (metropolis(s_n, s_y, mb, u[6])) {
+#pragma unroll
+          for (int i = 0; i < 6; ++i) yv[i] = mn[i];
+          s_y = s_n;
+          moved = true;
+        }
+        u7 = u[7];
+      }
+      if (__ballot(moved) != 0) {
+        place(yv);
+        gd_energy E;
+        pose_energy<APL>(C, S, lpr, x, y, z, ox, oy, oz, lt, q, li, val, yv, E);
+        const float d1 = E.e[GD_MAXWEI - 1] - eo.e[GD_MAXWEI - 1];
+        const float d2 = s_y - s_old;
+        const float delta = d1 - d2;
+        const float prod = delta * mb;
+        if (moved && u7 < decision_expf(prod)) {
+#pragma unroll
+          for (int i = 0; i < 6; ++i) mo[i] = yv[i];
+          eo = E;
+          s_old = s_y;
+          acs += 1;
+          record(E, s + K - 1);
         }
       }
     }
-    if (nc > 0 && ((s + 1) % nc) == 0) {
+    }
+    if (nc > 0 && ((s + K) % nc) == 0) {
       if (active && a == 0) S.xe[slot] = eo.e[GD_MAXWEI - 1];
       __syncthreads();
       if (tid < pk.n_groups) {
         const int n_tmp = C.n_tmp;
-        const int seg = (s + 1) / nc - 1;
+        const int seg = (s + K) / nc - 1;
         const int mode_t = (seg % 2 == 0) ? 1 : 0;
         const int f0 = P.rep_flat[P.slots[pk.slot_off + tid * n_tmp]];
         const int group = (f0 / (n_tmp * n_lig)) * n_lig + (f0 % n_lig);
@@ -550,7 +722,8 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   }
 }
 
-extern "C" __global__ void __launch_bounds__(512) gd_mc_kernel(KParams P) {
+template <bool DA>
+__device__ __forceinline__ void mc_kernel_body(const KParams &P) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   const Pack pk = P.packs[blockIdx.x];
   const DevComplex &C = P.cxs[pk.cx];
@@ -600,11 +773,21 @@ extern "C" __global__ void __launch_bounds__(512) gd_mc_kernel(KParams P) {
 
   const int lpr = C.lpr;
   switch (C.apl) {
-    case 1: run_pack<1>(P, pk, C, S, lpr); break;
-    case 2: run_pack<2>(P, pk, C, S, lpr); break;
-    case 3: run_pack<3>(P, pk, C, S, lpr); break;
-    default: run_pack<4>(P, pk, C, S, lpr); break;
+    case 1: run_pack<1, DA>(P, pk, C, S, lpr); break;
+    case 2: run_pack<2, DA>(P, pk, C, S, lpr); break;
+    case 3: run_pack<3, DA>(P, pk, C, S, lpr); break;
+    default: run_pack<4, DA>(P, pk, C, S, lpr); break;
   }
+}
+
+// Plain MC (and start-pose / pose evaluation) kernel.
+extern "C" __global__ void __launch_bounds__(512) gd_mc_kernel(KParams P) {
+  mc_kernel_body<false>(P);
+}
+
+// Delayed-acceptance MC kernel: surrogate sub-steps screened by the MLP.
+extern "C" __global__ void __launch_bounds__(512) gd_mc_da_kernel(KParams P) {
+  mc_kernel_body<true>(P);
 }
 
 // Reference contact map of a complex (InitRefMatrix_d,

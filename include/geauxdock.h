@@ -267,6 +267,15 @@ int64_t gd_sur_dataset_size(const gd_ctx *ctx);
 /* Mean squared error over poses [pose_begin, pose_end) and its gradient. */
 gd_status gd_sur_loss_grad(gd_ctx *ctx, int pose_begin, int pose_end, double *loss, float *grad);
 gd_status gd_sur_predict(gd_ctx *ctx, int pose_begin, int pose_end, float *out);
+/* Delayed-acceptance MC (surrogate transition, no counterpart in the
+ * reference): each group of k_sub steps runs k_sub Metropolis steps on the
+ * surrogate total S = sur_scale * MLP + the exact KDE, MCS and pocket-distance
+ * terms, then accepts the composite move with min(1, exp(-beta [(E(y) - E(x)) -
+ * (S(y) - S(x))])) from one exact evaluation, so the chain keeps the exact
+ * Boltzmann distribution. n_steps and steps_per_exchange are multiples of
+ * k_sub; replica-steps count surrogate steps. */
+gd_status gd_run_da(gd_ctx *ctx, int64_t n_steps, int steps_per_exchange, int k_sub,
+                    float sur_scale, gd_record_sink sink, void *user, gd_run_stats *stats);
 /* Adam over contiguous batches of batch_poses poses; epoch_loss[epochs]. */
 gd_status gd_sur_train(gd_ctx *ctx, int pose_begin, int pose_end, int epochs, int batch_poses,
                        float lr, double *epoch_loss, double *seconds);

@@ -589,6 +589,35 @@ static void put_record(gdo_mc_io *io, int flat, const gd_replica *rp, const gd_e
 }
 
 /* MonteCarlo_Init_d + MonteCarlo_d + ExchangeReplicas_d on the CPU. */
+/* Replica exchange after segment seg for every (prt, lig) ladder. */
+static void exchange_all(const gd_complex_desc *cx, uint64_t seed, int cid, int seg,
+                         gdo_mc_io *io) {
+  const int n_lig = cx->n_lig, n_prt = cx->n_prt, n_tmp = cx->n_tmp;
+  const int mode_t = (seg % 2 == 0) ? 1 : 0;
+  float mb[GD_MAX_NTMP];
+  for (int t = 0; t < n_tmp; ++t) mb[t] = cx->temp[t].minus_beta;
+  for (int p = 0; p < n_prt; ++p) {
+    for (int l = 0; l < n_lig; ++l) {
+      const int g = p * n_lig + l;
+      float et[GD_MAX_NTMP], u[GD_MAX_NTMP];
+      int32_t it[GD_MAX_NTMP], ac[GD_MAX_NTMP];
+      for (int t = 0; t < n_tmp; ++t) {
+        const int f = n_tmp * n_lig * p + n_lig * t + l;
+        et[t] = io->e_old[f].e[9];
+        it[t] = io->idx_tmp[f];
+        ac[t] = io->acs_x[f];
+      }
+      for (int k = 0; k < n_tmp / 2; ++k) u[k] = gdo_exchange_uniform(seed, cid, g, seg, k);
+      gdo_exchange_group(n_tmp, mode_t, et, mb, u, it, ac);
+      for (int t = 0; t < n_tmp; ++t) {
+        const int f = n_tmp * n_lig * p + n_lig * t + l;
+        io->idx_tmp[f] = it[t];
+        io->acs_x[f] = ac[t];
+      }
+    }
+  }
+}
+
 int gdo_mc_run(const gd_complex_desc *cx, uint64_t seed, int cid, int init, int step0,
                int n_steps, int nc, gdo_mc_io *io, int n_threads) {
   const int n_lig = cx->n_lig, n_prt = cx->n_prt, n_tmp = cx->n_tmp;
@@ -646,32 +675,94 @@ int gdo_mc_run(const gd_complex_desc *cx, uint64_t seed, int cid, int init, int 
       }
     }
     s = blk_end;
-    if (nc > 0 && s % nc == 0) {
-      const int seg = s / nc - 1;
-      const int mode_t = (seg % 2 == 0) ? 1 : 0;
-      float mb[GD_MAX_NTMP];
-      for (int t = 0; t < n_tmp; ++t) mb[t] = cx->temp[t].minus_beta;
-      for (int p = 0; p < n_prt; ++p) {
-        for (int l = 0; l < n_lig; ++l) {
-          const int g = p * n_lig + l;
-          float et[GD_MAX_NTMP], u[GD_MAX_NTMP];
-          int32_t it[GD_MAX_NTMP], ac[GD_MAX_NTMP];
-          for (int t = 0; t < n_tmp; ++t) {
-            const int f = n_tmp * n_lig * p + n_lig * t + l;
-            et[t] = io->e_old[f].e[9];
-            it[t] = io->idx_tmp[f];
-            ac[t] = io->acs_x[f];
+    if (nc > 0 && s % nc == 0) exchange_all(cx, seed, cid, s / nc - 1, io);
+  }
+  free(ref);
+  return 0;
+}
+
+/* Weighted sum of the terms that need no receptor pairs (KDE, MCS, pocket
+ * distance): the exact part of the delayed-acceptance surrogate. */
+static float cheap_total(const gd_complex_desc *cx, const gd_energy *e) {
+  const gd_enepara *ep = cx->enepara;
+  float t = 0.0f;
+  for (int i = 6; i < 9; ++i) t += ep->w[i] * e->e[i];
+  return t;
+}
+
+/* Delayed-acceptance MC with the surrogate reduced to its exact cheap terms
+ * (surrogate scale 0): K Metropolis sub-steps on S, then the composite move
+ * is accepted when U7 < exp(-beta ((E(y) - E(x)) - (S(y) - S(x)))). The
+ * uniforms of sub-step j are those of step s + j; U7 is the last sub-step's
+ * eighth uniform; an accepted composite move is recorded at step s + K - 1. */
+int gdo_mc_run_da(const gd_complex_desc *cx, uint64_t seed, int cid, int step0, int n_steps,
+                  int nc, int k_sub, gdo_mc_io *io, int n_threads) {
+  const int n_lig = cx->n_lig, n_prt = cx->n_prt, n_tmp = cx->n_tmp;
+  const int n_rep = n_lig * n_prt * n_tmp;
+  if (cx->lna > GD_MAX_LNA || n_tmp > GD_MAX_NTMP) return 2;
+  if (k_sub < 1 || n_steps % k_sub || step0 % k_sub || (nc > 0 && nc % k_sub)) return 1;
+  uint8_t *ref = (uint8_t *)malloc((size_t)cx->lna * cx->pnp);
+  if (!ref) return 5;
+  gdo_refmap_f32(cx, ref);
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+  (void)n_threads;
+#endif
+  int s = step0;
+  const int s_end = step0 + n_steps;
+  while (s < s_end) {
+    int blk_end = s_end;
+    if (nc > 0) {
+      const int next_x = (s / nc + 1) * nc;
+      if (next_x < blk_end) blk_end = next_x;
+    }
+    const int sb = s;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int flat = 0; flat < n_rep; ++flat) {
+      const int p = flat / (n_tmp * n_lig), l = flat % n_lig;
+      float *mm = io->mm + 6 * (size_t)flat;
+      gd_energy ex;
+      gdo_energy_f32(cx, ref, p, l, mm, &ex);
+      float s_old = cheap_total(cx, &ex);
+      for (int st = sb; st < blk_end; st += k_sub) {
+        const float mb = cx->temp[io->idx_tmp[flat]].minus_beta;
+        float yv[6], s_y = s_old, u7 = 1.0f;
+        int moved = 0;
+        for (int i = 0; i < 6; ++i) yv[i] = mm[i];
+        for (int j = 0; j < k_sub; ++j) {
+          float u[8], dm[6], mn[6];
+          gdo_mc_uniforms(seed, cid, flat, st + j, u);
+          gdo_move_offsets(u, cx->move_scale, cx->move_kind, dm);
+          for (int i = 0; i < 6; ++i) mn[i] = yv[i] + dm[i];
+          gd_energy en;
+          gdo_energy_f32(cx, ref, p, l, mn, &en);
+          const float s_n = cheap_total(cx, &en);
+          if (gdo_accept(s_n, s_y, mb, u[6])) {
+            for (int i = 0; i < 6; ++i) yv[i] = mn[i];
+            s_y = s_n;
+            moved = 1;
           }
-          for (int k = 0; k < n_tmp / 2; ++k) u[k] = gdo_exchange_uniform(seed, cid, g, seg, k);
-          gdo_exchange_group(n_tmp, mode_t, et, mb, u, it, ac);
-          for (int t = 0; t < n_tmp; ++t) {
-            const int f = n_tmp * n_lig * p + n_lig * t + l;
-            io->idx_tmp[f] = it[t];
-            io->acs_x[f] = ac[t];
-          }
+          u7 = u[7];
+        }
+        if (!moved) continue;
+        gd_energy E;
+        gdo_energy_f32(cx, ref, p, l, yv, &E);
+        const float d1 = E.e[9] - io->e_old[flat].e[9];
+        const float d2 = s_y - s_old;
+        const float delta = d1 - d2;
+        if (u7 < gdo_expf(delta * mb)) {
+          for (int i = 0; i < 6; ++i) mm[i] = yv[i];
+          io->e_old[flat] = E;
+          s_old = s_y;
+          io->acs_mc[flat] += 1;
+          const gd_replica rp = {flat, p, io->idx_tmp[flat], l};
+          put_record(io, flat, &rp, &E, mm, st + k_sub - 1);
         }
       }
     }
+    s = blk_end;
+    if (nc > 0 && s % nc == 0) exchange_all(cx, seed, cid, s / nc - 1, io);
   }
   free(ref);
   return 0;

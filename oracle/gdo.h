@@ -86,6 +86,13 @@ typedef struct gdo_mc_io {
 int gdo_mc_run(const gd_complex_desc *cx, uint64_t seed, int cid, int init, int step0,
                int n_steps, int nc, gdo_mc_io *io, int n_threads);
 
+/* Delayed-acceptance MC whose surrogate is the exact cheap-term total
+ * (KDE + MCS + pocket distance, weighted): k_sub surrogate Metropolis
+ * sub-steps then one exact composite test. Start poses must already be
+ * evaluated (e_old). step0, n_steps and nc are multiples of k_sub. */
+int gdo_mc_run_da(const gd_complex_desc *cx, uint64_t seed, int cid, int step0, int n_steps,
+                  int nc, int k_sub, gdo_mc_io *io, int n_threads);
+
 #ifdef __cplusplus
 }
 #endif

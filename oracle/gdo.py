@@ -50,6 +50,9 @@ def load():
     lib.gdo_mc_run.argtypes = [C.POINTER(abi.ComplexDesc), C.c_uint64, C.c_int, C.c_int,
                                C.c_int, C.c_int, C.c_int, C.POINTER(McIO), C.c_int]
     lib.gdo_mc_run.restype = C.c_int
+    lib.gdo_mc_run_da.argtypes = [C.POINTER(abi.ComplexDesc), C.c_uint64, C.c_int, C.c_int,
+                                  C.c_int, C.c_int, C.c_int, C.POINTER(McIO), C.c_int]
+    lib.gdo_mc_run_da.restype = C.c_int
     return lib
 
 
@@ -165,3 +168,13 @@ def mc_run(desc, state, seed, cid=0, init=False, step0=0, n_steps=0, nc=0, threa
                           threads)
     if rc != 0:
         raise RuntimeError(f"gdo_mc_run failed: {rc}")
+
+
+def mc_run_da(desc, state, seed, cid=0, step0=0, n_steps=0, nc=0, k_sub=1, threads=0):
+    """Delayed-acceptance CPU path with the surrogate reduced to its exact
+    cheap terms (surrogate scale 0); state must hold evaluated start poses."""
+    io = state.io()
+    rc = lib().gdo_mc_run_da(C.byref(desc), seed, cid, step0, n_steps, nc, k_sub, C.byref(io),
+                             threads)
+    if rc != 0:
+        raise RuntimeError(f"gdo_mc_run_da failed: {rc}")

@@ -1,0 +1,143 @@
+"""Delayed-acceptance MC (gd_run_da) on the GPU.
+
+* Surrogate scale 0 reduces the surrogate to the exact cheap terms; the chain
+  then has a CPU restatement (oracle gdo_mc_run_da) and the decision
+  sequences must agree step for step, as for the plain path.
+* With a real (random or trained) MLP every recorded state and the final
+  state carry the exact energy of their pose (FP32, 1e-4 of the pair
+  magnitude), a run is a function of the seed alone (record windows of any
+  length give bit-identical states), and an exact rejection never moves a
+  replica.
+* The composite test keeps the exact Boltzmann distribution: with exchanges
+  off, the per-temperature mean energy of a DA chain matches that of the
+  plain chain within sampling error.
+"""
+import numpy as np
+import pytest
+
+from geaxdock_gpu_amd import abi, synth
+from geaxdock_gpu_amd import surrogate as sg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gd():
+    import geaxdock_gpu_amd as g
+    return g
+
+
+def _engine(gd, cx, seed, params_seed=3, **kw):
+    eng = gd.Engine(seed=seed, **kw)
+    cid = eng.add_complex(cx)
+    eng.sur_build_grids(0.8, 12.0)
+    eng.sur_set_params(sg.pack_params(sg.init_params(params_seed)))
+    return eng, cid
+
+
+def test_da_scale0_tracks_cpu_path(gd):
+    from oracle import gdo
+    cx = synth.config0(seed=0)
+    n_steps, nc, k = 400, 20, 4
+    recs = []
+    eng, cid = _engine(gd, cx, 11)
+    st = eng.run_da(n_steps, nc, k, 0.0, sink=lambda c, r: recs.append(r))
+    state = eng.state(cid)
+    eng.close()
+    recs = np.concatenate(recs)
+    d = cx.desc()
+    cpu = gdo.McState(cx, rec_cap=n_steps + 1)
+    gdo.mc_run(d, cpu, seed=11, init=True, step0=0, n_steps=0, nc=nc)
+    gdo.mc_run_da(d, cpu, seed=11, step0=0, n_steps=n_steps, nc=nc, k_sub=k)
+    same = 0
+    for r in range(cx.n_rep):
+        g = recs[recs["replica"][:, 0] == r]
+        c = cpu.records(r)
+        if len(g) == len(c) and np.array_equal(g["step"], c["step"]):
+            same += 1
+            np.testing.assert_allclose(g["energy"]["e"][:, 9], c["energy"]["e"][:, 9], rtol=2e-4,
+                                       atol=2e-4)
+            np.testing.assert_allclose(g["movematrix"], c["movematrix"], rtol=1e-4, atol=1e-4)
+    print("\nDA(scale 0) replicas with identical decisions:", same, "of", cx.n_rep,
+          "acs", state["acs_mc"], "x", state["acs_x"])
+    assert same >= cx.n_rep - 1, same
+    assert st["accepted"] == int(state["acs_mc"].sum())
+    # composite moves are recorded at the last sub-step of a group
+    assert np.all((recs["step"][recs["step"] > 0] + 1) % k == 0)
+
+
+def test_da_states_carry_exact_energy(gd):
+    cx = synth.config0(seed=1)
+    recs = []
+    eng, cid = _engine(gd, cx, 5)
+    eng.run_da(240, 20, 4, 3.0, sink=lambda c, r: recs.append(r))
+    state = eng.state(cid)
+    recs = np.concatenate(recs)
+    ex = eng.calc_energy(cid, recs["replica"][:, 3], recs["replica"][:, 1], recs["movematrix"])
+    np.testing.assert_allclose(recs["energy"]["e"], ex["e"], rtol=1e-5, atol=1e-5)
+    ex = eng.calc_energy(cid, state["replica"][:, 3], state["replica"][:, 1],
+                         state["movematrix"])
+    np.testing.assert_allclose(state["energy"]["e"], ex["e"], rtol=1e-5, atol=1e-5)
+    # every replica's last record is its current state
+    for r in range(cx.n_rep):
+        last = recs[recs["replica"][:, 0] == r][-1]
+        np.testing.assert_array_equal(last["movematrix"], state["movematrix"][r])
+    eng.close()
+
+
+def test_da_deterministic_and_window_independent(gd):
+    cx = synth.config0(seed=2)
+    out = []
+    for dump in (40, 2000):
+        eng, cid = _engine(gd, cx, 9, steps_per_dump=dump)
+        eng.run_da(200, 20, 4, 2.0)
+        out.append(eng.state(cid))
+        eng.close()
+    for key in out[0]:
+        assert np.array_equal(out[0][key], out[1][key]), key
+
+
+def test_da_rejects_bad_arguments(gd):
+    cx = synth.config0(seed=2)
+    with gd.Engine(seed=1) as eng:
+        eng.add_complex(cx)
+        with pytest.raises(gd.GeauxDockError):
+            eng.run_da(40, 20, 4, 1.0)  # no grids yet
+        eng.sur_build_grids(0.8, 12.0)
+        with pytest.raises(gd.GeauxDockError):
+            eng.run_da(40, 20, 4, 1.0)  # no parameters yet
+        eng.sur_set_params(sg.pack_params(sg.init_params(0)))
+        with pytest.raises(gd.GeauxDockError):
+            eng.run_da(42, 20, 4, 1.0)  # not a multiple of k
+        with pytest.raises(gd.GeauxDockError):
+            eng.run_da(40, 10, 4, 1.0)
+        eng.run_da(40, 20, 4, 1.0)
+
+
+def test_da_keeps_boltzmann_mean_energy(gd):
+    """Plain and DA chains (no exchange) at fixed temperatures: after burn-in,
+    per-temperature mean total energies agree within 5 standard errors of
+    the mean paired difference over replicas (independent chains)."""
+    cx = synth.make_complex(seed=4, lna=24, pnp=900, n_lig=16, n_prt=1, n_tmp=4,
+                            t_floor=1.0, t_ceiling=4.0)
+    k = 4
+    means = []
+    for da in (False, True):
+        eng, cid = _engine(gd, cx, 21, record=False)
+        run = (lambda n: eng.run_da(n, 0, k, 1.0)) if da else (lambda n: eng.run(n, 0))
+        run(3000)
+        acc = []
+        for _ in range(30):
+            run(100)
+            s = eng.state(cid)
+            acc.append(s["energy"]["e"][:, 9].astype(np.float64))
+        eng.close()
+        e = np.stack(acc)  # [sample, replica]
+        it = s["replica"][:, 2]
+        means.append([e[:, it == t].mean(0) for t in range(cx.n_tmp)])
+    for t in range(cx.n_tmp):
+        # per-replica time means, paired by replica (same ligand conformer)
+        dlt = means[0][t] - means[1][t]
+        se = dlt.std(ddof=1) / np.sqrt(len(dlt))
+        print(f"T{t}: plain {means[0][t].mean():.4f} DA {means[1][t].mean():.4f} se {se:.4f}")
+        assert abs(dlt.mean()) <= 5 * se + 1e-3, (t, dlt.mean(), se)
