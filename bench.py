@@ -111,6 +111,12 @@ def cpu_baseline(cxs, args):
                       f"{dt:.1f} s on {threads} threads; scaled by pair count to the workload"}
 
 
+def kernel_names(eng):
+    """Kernel instances the MC launch uses (one per atoms-per-lane class)."""
+    apls = sorted({eng.shape(c)[1] for c in range(len(eng.complexes))})
+    return ", ".join(f"gd::gd_mc_kernel<{a}, false>" for a in apls)
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -159,7 +165,7 @@ def main():
         "config": dict(desc, replicas=n_rep, steps_per_exchange=args.nc),
         "roofline": {"bound": "mfma", "unit": "TFLOP/s", "achieved": achieved / 1e12,
                      "peak": PEAK_FP32 / 1e12, "frac": achieved / PEAK_FP32, "traffic": None,
-                     "kernel": "gd_mc_kernel", "kernel_ms_avg": st["kernel_ms_avg"],
+                     "kernel": kernel_names(eng), "kernel_ms_avg": st["kernel_ms_avg"],
                      "flops_per_launch": flops_launch,
                      "note": "FP32 vector (VALU) bound; 36 flops per ligand-receptor pair"},
         "acceptance_ratio": state_accept,

@@ -58,6 +58,8 @@ def load_library():
     lib.gd_philox.argtypes = [vp, C.c_int, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     lib.gd_num_replicas.argtypes = [vp, C.c_int]
     lib.gd_num_replicas.restype = C.c_int
+    lib.gd_complex_shape.argtypes = [vp, C.c_int, P(C.c_int), P(C.c_int)]
+    lib.gd_complex_shape.restype = C.c_int
     lib.gd_run_da.argtypes = [vp, C.c_int64, C.c_int, C.c_int, C.c_float, abi.RECORD_SINK, vp,
                               P(abi.RunStats)]
     lib.gd_run_da.restype = C.c_int
@@ -181,6 +183,12 @@ class Engine:
         self._check(self.lib.gd_run_da(self._ctx, int(n_steps), int(steps_per_exchange),
                                        int(k_sub), float(sur_scale), cb, None, C.byref(st)))
         return {f: getattr(st, f) for f, _ in abi.RunStats._fields_}
+
+    def shape(self, cid):
+        """(lanes per replica, ligand atoms per lane) chosen for a complex."""
+        lpr, apl = C.c_int(0), C.c_int(0)
+        self._check(self.lib.gd_complex_shape(self._ctx, cid, C.byref(lpr), C.byref(apl)))
+        return lpr.value, apl.value
 
     def num_replicas(self, cid):
         return self.lib.gd_num_replicas(self._ctx, cid)

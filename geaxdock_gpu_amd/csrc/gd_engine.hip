@@ -39,6 +39,9 @@ struct gd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // side streams for concurrent launches of the per-APL kernel classes
+  hipStream_t aux[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
   std::string err;
   int n_cu = 0;
   size_t max_lds = 65536;
@@ -60,6 +63,7 @@ struct gd_ctx {
   Pack *d_packs = nullptr;
   int *d_slots = nullptr;
   int n_packs = 0, block = 0;
+  int cls[5] = {0, 0, 0, 0, 0};  // packs of APL class a are [cls[a-1], cls[a])
   size_t lds = 0;
   KParams lay{};  // LDS layout of the MC launch
   // learned surrogate
@@ -190,6 +194,11 @@ gd_status gd_create(gd_ctx **out, const gd_opts *opts) {
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&ctx->ev0));
   HIPCHK(hipEventCreate(&ctx->ev1));
+  HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+  for (int i = 0; i < 3; ++i) {
+    HIPCHK(hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ctx->ev_join[i], hipEventDisableTiming));
+  }
   HIPCHK(hipMalloc(&ctx->d_overflow, sizeof(int)));
   return GD_OK;
 }
@@ -219,8 +228,20 @@ void gd_destroy(gd_ctx *ctx) {
   }
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  for (int i = 0; i < 3; ++i) {
+    if (ctx->ev_join[i]) (void)hipEventDestroy(ctx->ev_join[i]);
+    if (ctx->aux[i]) (void)hipStreamDestroy(ctx->aux[i]);
+  }
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+}
+
+gd_status gd_complex_shape(const gd_ctx *ctx, int cid, int *lpr, int *apl) {
+  if (!ctx || cid < 0 || cid >= (int)ctx->cx.size() || !lpr || !apl) return GD_ERR_INVALID;
+  *lpr = ctx->cx[cid].lpr;
+  *apl = ctx->cx[cid].apl;
+  return GD_OK;
 }
 
 int gd_num_replicas(const gd_ctx *ctx, int cid) {
@@ -539,21 +560,32 @@ struct PackSet {
   std::vector<Pack> packs;
   std::vector<int> slots;
   int block = 64, pnp_max = 0, pnk_max = 0, nch_max = 0, slots_max = 1;
+  int cls[5] = {0, 0, 0, 0, 0};  // packs of APL class a are [cls[a-1], cls[a])
 };
 
 void pack_finish(const gd_ctx *ctx, PackSet &ps) {
-  // longest packs first so the hardware dispatcher fills the CUs in LPT order
+  // grouped by APL class (one kernel instance each), longest packs first
+  // within a class so the hardware dispatcher fills the CUs in LPT order
   std::vector<int> order(ps.packs.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   auto cost = [&](const Pack &p) {
     const HostComplex &h = ctx->cx[p.cx];
     return (double)h.d.pnp * (double)((p.n_slots * h.lpr + 63) / 64) * h.apl;
   };
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return cost(ps.packs[a]) > cost(ps.packs[b]); });
+  auto apl = [&](const Pack &p) { return ctx->cx[p.cx].apl; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const Pack &pa = ps.packs[a], &pb = ps.packs[b];
+    if (apl(pa) != apl(pb)) return apl(pa) < apl(pb);
+    return cost(pa) > cost(pb);
+  });
   std::vector<Pack> sorted;
   for (int i : order) sorted.push_back(ps.packs[i]);
   ps.packs.swap(sorted);
+  for (int a = 1; a <= 4; ++a) {
+    int e = ps.cls[a - 1];
+    while (e < (int)ps.packs.size() && apl(ps.packs[e]) == a) ++e;
+    ps.cls[a] = e;
+  }
   for (const Pack &p : ps.packs) {
     const HostComplex &h = ctx->cx[p.cx];
     ps.block = std::max(ps.block, ((p.n_slots * h.lpr + 63) / 64) * 64);
@@ -636,6 +668,7 @@ gd_status alloc_state(gd_ctx *ctx) {
   HIPCHK(hipMemcpyAsync(ctx->d_slots, ps.slots.data(), ps.slots.size() * sizeof(int),
                         hipMemcpyHostToDevice, ctx->stream));
   ctx->n_packs = (int)ps.packs.size();
+  for (int a = 0; a < 5; ++a) ctx->cls[a] = ps.cls[a];
   ctx->block = ps.block;
   layout(ctx->lay, &ctx->lds, ps.pnp_max, ps.pnk_max, ps.nch_max, ps.slots_max);
   if (ctx->lds > ctx->max_lds)
@@ -643,8 +676,6 @@ gd_status alloc_state(gd_ctx *ctx) {
                 ctx->max_lds);
   if (ctx->block > 512)
     return fail(ctx, GD_ERR_CAPACITY, "a temperature ladder needs %d lanes (> 512)", ctx->block);
-  (void)hipFuncSetAttribute((const void *)gd_mc_kernel,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->lds);
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->state_alloc = true;
   if (ctx->opts.verbose)
@@ -670,6 +701,43 @@ KParams base_params(gd_ctx *ctx) {
   P.k0 = (uint32_t)ctx->opts.seed;
   P.k1 = (uint32_t)(ctx->opts.seed >> 32);
   return P;
+}
+
+const void *mc_kernel_fn(int apl, bool da) {
+  switch (apl) {
+    case 1: return da ? (const void *)gd_mc_kernel<1, true> : (const void *)gd_mc_kernel<1, false>;
+    case 2: return da ? (const void *)gd_mc_kernel<2, true> : (const void *)gd_mc_kernel<2, false>;
+    case 3: return da ? (const void *)gd_mc_kernel<3, true> : (const void *)gd_mc_kernel<3, false>;
+    default: return da ? (const void *)gd_mc_kernel<4, true> : (const void *)gd_mc_kernel<4, false>;
+  }
+}
+
+// Launch every APL class of a pack set: one kernel instance per class, the
+// classes concurrently on side streams forked from and joined to `stream`.
+hipError_t launch_classes(gd_ctx *ctx, const int cls[5], const KParams &P0, bool da, int block,
+                          size_t lds, hipStream_t stream) {
+  int used[4], n_used = 0;
+  for (int a = 1; a <= 4; ++a)
+    if (cls[a] > cls[a - 1]) used[n_used++] = a;
+  hipError_t e = hipSuccess;
+  if (n_used > 1) {
+    e = hipEventRecord(ctx->ev_fork, stream);
+    for (int i = 1; i < n_used && e == hipSuccess; ++i) e = hipStreamWaitEvent(ctx->aux[i - 1], ctx->ev_fork, 0);
+  }
+  for (int i = 0; i < n_used && e == hipSuccess; ++i) {
+    const int a = used[i];
+    const void *f = mc_kernel_fn(a, da);
+    e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) break;
+    KParams P = P0;
+    P.pack_base = cls[a - 1];
+    void *args[] = {&P};
+    hipStream_t s = i == 0 ? stream : ctx->aux[i - 1];
+    e = hipLaunchKernel(f, dim3(cls[a] - cls[a - 1]), dim3(block), args, lds, s);
+    if (e == hipSuccess && i > 0) e = hipEventRecord(ctx->ev_join[i - 1], s);
+  }
+  for (int i = 1; i < n_used && e == hipSuccess; ++i) e = hipStreamWaitEvent(stream, ctx->ev_join[i - 1], 0);
+  return e;
 }
 
 gd_status drain_records(gd_ctx *ctx, gd_record_sink sink, void *user) {
@@ -727,14 +795,9 @@ gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float
     lds = P.off_da + (size_t)(ctx->block / 64) * 6208;
     if (lds > ctx->max_lds)
       return fail(ctx, GD_ERR_CAPACITY, "delayed acceptance needs %zu bytes of LDS", lds);
-    (void)hipFuncSetAttribute((const void *)gd_mc_da_kernel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   }
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  if (P.da_k > 0)
-    hipLaunchKernelGGL(gd_mc_da_kernel, dim3(ctx->n_packs), dim3(ctx->block), lds, ctx->stream, P);
-  else
-    hipLaunchKernelGGL(gd_mc_kernel, dim3(ctx->n_packs), dim3(ctx->block), lds, ctx->stream, P);
+  HIPCHK(launch_classes(ctx, ctx->cls, P, P.da_k > 0, ctx->block, lds, ctx->stream));
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(hipEventSynchronize(ctx->ev1));
@@ -957,9 +1020,7 @@ gd_status gd_calc_energy(gd_ctx *ctx, int cid, int n, const int *idx_lig, const 
   P.ev_out = d_out;
   P.mode = MODE_EVAL;
   if (ps.block > 512) return fail(ctx, GD_ERR_CAPACITY, "pose pack of %d lanes (> 512)", ps.block);
-  (void)hipFuncSetAttribute((const void *)gd_mc_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(gd_mc_kernel, dim3(ps.packs.size()), dim3(ps.block), lds, ctx->stream, P);
-  hipError_t le = hipGetLastError();
+  hipError_t le = launch_classes(ctx, ps.cls, P, false, ps.block, lds, ctx->stream);
   if (le == hipSuccess)
     le = hipMemcpyAsync(out, d_out, (size_t)n * sizeof(gd_energy), hipMemcpyDeviceToHost, ctx->stream);
   if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
@@ -968,8 +1029,6 @@ gd_status gd_calc_energy(gd_ctx *ctx, int cid, int n, const int *idx_lig, const 
   (void)hipFree(d_lig);
   (void)hipFree(d_mm);
   (void)hipFree(d_out);
-  if (ctx->state_alloc)
-    (void)hipFuncSetAttribute((const void *)gd_mc_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->lds);
   if (le != hipSuccess) return fail(ctx, GD_ERR_HIP, "energy evaluation: %s", hipGetErrorString(le));
   return GD_OK;
 }

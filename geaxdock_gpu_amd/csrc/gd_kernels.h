@@ -69,7 +69,8 @@ enum { MODE_MC = 0, MODE_INIT = 1, MODE_EVAL = 2 };
 
 struct KParams {
   const DevComplex *cxs;
-  const Pack *packs;
+  const Pack *packs;  // this launch's workgroups use packs[pack_base + blockIdx.x]
+  int pack_base;
   const int *slots;
   // replica state (MODE_MC / MODE_INIT), indexed by global replica id
   float *mm;

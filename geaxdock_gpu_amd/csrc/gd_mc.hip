@@ -722,10 +722,10 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   }
 }
 
-template <bool DA>
+template <int APL, bool DA>
 __device__ __forceinline__ void mc_kernel_body(const KParams &P) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-  const Pack pk = P.packs[blockIdx.x];
+  const Pack pk = P.packs[P.pack_base + blockIdx.x];
   const DevComplex &C = P.cxs[pk.cx];
   Lds S;
   S.xyze = reinterpret_cast<const float4 *>(lds);
@@ -771,23 +771,16 @@ __device__ __forceinline__ void mc_kernel_body(const KParams &P) {
   }
   __syncthreads();
 
-  const int lpr = C.lpr;
-  switch (C.apl) {
-    case 1: run_pack<1, DA>(P, pk, C, S, lpr); break;
-    case 2: run_pack<2, DA>(P, pk, C, S, lpr); break;
-    case 3: run_pack<3, DA>(P, pk, C, S, lpr); break;
-    default: run_pack<4, DA>(P, pk, C, S, lpr); break;
-  }
+  run_pack<APL, DA>(P, pk, C, S, C.lpr);
 }
 
-// Plain MC (and start-pose / pose evaluation) kernel.
-extern "C" __global__ void __launch_bounds__(512) gd_mc_kernel(KParams P) {
-  mc_kernel_body<false>(P);
-}
-
-// Delayed-acceptance MC kernel: surrogate sub-steps screened by the MLP.
-extern "C" __global__ void __launch_bounds__(512) gd_mc_da_kernel(KParams P) {
-  mc_kernel_body<true>(P);
+// MC kernel for packs whose complexes place APL ligand atoms on each lane:
+// plain MC, start-pose evaluation and pose evaluation (DA = false), or
+// delayed acceptance with surrogate sub-steps (DA = true). One instance per
+// APL keeps each variant's register allocation to what it needs.
+template <int APL, bool DA>
+__global__ void __launch_bounds__(512) gd_mc_kernel(KParams P) {
+  mc_kernel_body<APL, DA>(P);
 }
 
 // Reference contact map of a complex (InitRefMatrix_d,

@@ -241,6 +241,11 @@ gd_status gd_philox(gd_ctx *ctx, int n, const uint32_t *ctr, const uint32_t *key
 
 /* Number of replicas of a complex (n_lig*n_prt*n_tmp). */
 int gd_num_replicas(const gd_ctx *ctx, int complex_id);
+/* Lane shape chosen for a complex: lanes per replica and ligand atoms per
+ * lane (the MC kernel instance that runs it). No reference counterpart: the
+ * reference fixes its launch shape at compile time (src/Makefile:77-105). */
+gd_status gd_complex_shape(const gd_ctx *ctx, int complex_id, int *lanes_per_replica,
+                           int *atoms_per_lane);
 
 /* ---- learned energy surrogate -------------------------------------------
  * No counterpart in the reference. A small MLP (56 -> 128 -> 128 -> 1, SiLU,
