@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lpr", type=int, default=0)
     ap.add_argument("--apl", type=int, default=0)
+    ap.add_argument("--split", type=int, default=0, help="receptor split (0 = auto, 1, 2)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -132,7 +133,7 @@ def main():
 
     cxs, desc = workload(args)
     eng = gd.Engine(seed=0, record=True, steps_per_dump=2000, lanes_per_replica=args.lpr,
-                    atoms_per_lane=args.apl)
+                    atoms_per_lane=args.apl, receptor_split=args.split)
     for cx in cxs:
         eng.add_complex(cx)
     n_rep = sum(c.n_rep for c in cxs)

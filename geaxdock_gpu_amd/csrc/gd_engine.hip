@@ -62,7 +62,7 @@ struct gd_ctx {
   // packs of the MC launch
   Pack *d_packs = nullptr;
   int *d_slots = nullptr;
-  int n_packs = 0, block = 0;
+  int n_packs = 0, block = 0, n_split = 1;
   int cls[5] = {0, 0, 0, 0, 0};  // packs of APL class a are [cls[a-1], cls[a])
   size_t lds = 0;
   KParams lay{};  // LDS layout of the MC launch
@@ -530,7 +530,8 @@ gd_status ensure_ref(gd_ctx *ctx) {
 }
 
 // LDS layout for a launch covering packs with these maxima.
-void layout(KParams &P, size_t *bytes, int pnp_max, int pnk_max, int nch_max, int slots_max) {
+void layout(KParams &P, size_t *bytes, int pnp_max, int pnk_max, int nch_max, int slots_max,
+            int n_split, int half) {
   auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
   size_t o = al((size_t)pnp_max * 16);
   P.off_hpp = (int)o;
@@ -553,6 +554,13 @@ void layout(KParams &P, size_t *bytes, int pnp_max, int pnk_max, int nch_max, in
   o = al(o + (size_t)slots_max * 4);
   P.off_xo = (int)o;
   o = al(o + (size_t)slots_max * 4);
+  P.n_split = n_split;
+  P.half = half;
+  // every lane of a group maps to a slot below half / 8 (lpr >= 8), active or not
+  P.xp_slots = half / 8;
+  P.xp_floats = n_split > 1 ? n_split * half * 4 + n_split * P.xp_slots * 8 : 0;
+  P.off_xp = (int)o;
+  o = al(o + (size_t)2 * P.xp_floats * 4);
   *bytes = o;
 }
 
@@ -598,15 +606,15 @@ void pack_finish(const gd_ctx *ctx, PackSet &ps) {
 }
 
 // MC packs: whole ladders of one (complex, protein conformer), about 256
-// lanes per workgroup and at most 512.
-PackSet build_mc_packs(const gd_ctx *ctx) {
+// lanes per workgroup and at most max_lanes.
+PackSet build_mc_packs(const gd_ctx *ctx, int max_lanes) {
   PackSet ps;
   for (int c = 0; c < (int)ctx->cx.size(); ++c) {
     const HostComplex &h = ctx->cx[c];
     const int n_tmp = h.d.n_tmp, n_lig = h.d.n_lig;
     const int glanes = n_tmp * h.lpr;
     int G = std::max(1, 256 / glanes);
-    while (G > 1 && G * glanes > 512) --G;
+    while (G > 1 && G * glanes > max_lanes) --G;
     for (int p = 0; p < h.d.n_prt; ++p)
       for (int l0 = 0; l0 < n_lig; l0 += G) {
         const int ng = std::min(G, n_lig - l0);
@@ -660,7 +668,22 @@ gd_status alloc_state(gd_ctx *ctx) {
     ctx->rec_cap = ctx->opts.steps_per_dump + 1;
     HIPCHK(hipMalloc(&ctx->d_rec, (size_t)n * ctx->rec_cap * sizeof(gd_record)));
   }
-  PackSet ps = build_mc_packs(ctx);
+  // receptor split: two lane groups per ladder pack when one group per pack
+  // would leave fewer than two waves per SIMD
+  int ns = ctx->opts.receptor_split;
+  if (ns < 0 || ns > 2) return fail(ctx, GD_ERR_INVALID, "receptor_split must be 0, 1 or 2");
+  bool fits2 = true;
+  for (auto &h : ctx->cx) fits2 = fits2 && h.d.n_tmp * h.lpr <= 256;
+  if (ns == 2 && !fits2)
+    return fail(ctx, GD_ERR_CAPACITY, "receptor split needs ladders of at most 256 lanes");
+  if (ns == 0) {
+    PackSet p1 = build_mc_packs(ctx, 512);
+    int64_t waves = 0;
+    for (const Pack &pk : p1.packs) waves += (pk.n_slots * ctx->cx[pk.cx].lpr + 63) / 64;
+    ns = (fits2 && waves < (int64_t)8 * ctx->n_cu) ? 2 : 1;
+  }
+  ctx->n_split = ns;
+  PackSet ps = build_mc_packs(ctx, ns == 2 ? 256 : 512);
   HIPCHK(hipMalloc(&ctx->d_packs, ps.packs.size() * sizeof(Pack)));
   HIPCHK(hipMalloc(&ctx->d_slots, ps.slots.size() * sizeof(int)));
   HIPCHK(hipMemcpyAsync(ctx->d_packs, ps.packs.data(), ps.packs.size() * sizeof(Pack),
@@ -669,8 +692,8 @@ gd_status alloc_state(gd_ctx *ctx) {
                         hipMemcpyHostToDevice, ctx->stream));
   ctx->n_packs = (int)ps.packs.size();
   for (int a = 0; a < 5; ++a) ctx->cls[a] = ps.cls[a];
-  ctx->block = ps.block;
-  layout(ctx->lay, &ctx->lds, ps.pnp_max, ps.pnk_max, ps.nch_max, ps.slots_max);
+  ctx->block = ps.block * ns;
+  layout(ctx->lay, &ctx->lds, ps.pnp_max, ps.pnk_max, ps.nch_max, ps.slots_max, ns, ps.block);
   if (ctx->lds > ctx->max_lds)
     return fail(ctx, GD_ERR_CAPACITY, "launch needs %zu bytes of LDS, device has %zu", ctx->lds,
                 ctx->max_lds);
@@ -679,8 +702,8 @@ gd_status alloc_state(gd_ctx *ctx) {
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->state_alloc = true;
   if (ctx->opts.verbose)
-    fprintf(stderr, "[geauxdock] %d replicas, %d packs, block %d, LDS %zu B\n", n, ctx->n_packs,
-            ctx->block, ctx->lds);
+    fprintf(stderr, "[geauxdock] %d replicas, %d packs, block %d (split %d), LDS %zu B\n", n,
+            ctx->n_packs, ctx->block, ctx->n_split, ctx->lds);
   return GD_OK;
 }
 
@@ -997,7 +1020,11 @@ gd_status gd_calc_energy(gd_ctx *ctx, int cid, int n, const int *idx_lig, const 
   pack_finish(ctx, ps);
   KParams P{};
   size_t lds = 0;
-  layout(P, &lds, ps.pnp_max, ps.pnk_max, ps.nch_max, ps.slots_max);
+  // pose packs use the MC launch's receptor split (or the option's) so the
+  // same reduction order is exercised
+  int ns = ctx->state_alloc ? ctx->n_split : (ctx->opts.receptor_split == 2 ? 2 : 1);
+  if (ps.block > 256) ns = 1;
+  layout(P, &lds, ps.pnp_max, ps.pnk_max, ps.nch_max, ps.slots_max, ns, ps.block);
   if (lds > ctx->max_lds) return fail(ctx, GD_ERR_CAPACITY, "LDS %zu > %zu", lds, ctx->max_lds);
   Pack *d_packs = nullptr;
   int *d_slots = nullptr, *d_lig = nullptr;
@@ -1019,8 +1046,8 @@ gd_status gd_calc_energy(gd_ctx *ctx, int cid, int n, const int *idx_lig, const 
   P.ev_lig = d_lig;
   P.ev_out = d_out;
   P.mode = MODE_EVAL;
-  if (ps.block > 512) return fail(ctx, GD_ERR_CAPACITY, "pose pack of %d lanes (> 512)", ps.block);
-  hipError_t le = launch_classes(ctx, ps.cls, P, false, ps.block, lds, ctx->stream);
+  if (ps.block * ns > 512) return fail(ctx, GD_ERR_CAPACITY, "pose pack of %d lanes (> 512)", ps.block);
+  hipError_t le = launch_classes(ctx, ps.cls, P, false, ps.block * ns, lds, ctx->stream);
   if (le == hipSuccess)
     le = hipMemcpyAsync(out, d_out, (size_t)n * sizeof(gd_energy), hipMemcpyDeviceToHost, ctx->stream);
   if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);

@@ -91,6 +91,10 @@ struct KParams {
   uint32_t k0, k1;
   // LDS layout (bytes)
   int off_hpp, off_psp, off_par, off_kde, off_kr, off_ch, off_xe, off_xi, off_xa, off_xo;
+  // receptor split: n_split (1 or 2) groups of `half` lanes each evaluate the
+  // same replicas over their share of every receptor chunk and exchange
+  // partial sums through LDS at off_xp (two parity buffers of xp_floats)
+  int n_split, half, off_xp, xp_floats, xp_slots;
   int off_da;  // per-wave surrogate scratch (MODE_MC with da_k > 0)
   // delayed acceptance: k surrogate sub-steps per exact evaluation (0 = plain)
   int da_k;

@@ -29,6 +29,17 @@ struct Lds {
   int *xi;
   int *xa;
   int *xo;
+  float *xp;
+};
+
+// Receptor split of one workgroup: lanes [h*half, (h+1)*half) form group h
+// of n_split; every group holds the same replicas and covers its share of
+// each receptor chunk. `par` selects the LDS parity buffer of the next
+// exchange (each exchange flips it, so one barrier per exchange suffices).
+struct Split {
+  int h, ns, half, ltid, slot, slots;
+  int xp_floats;
+  int par;
 };
 
 __device__ __forceinline__ float seg_sum(float v, int lpr) {
@@ -191,8 +202,15 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, cons
 // Energy, RMSD and MCC of one pose held by `lpr` lanes (CalcEnergy_d +
 // CombineEnergy_d + CalcRmsd_d + CalcMcc_d). Every lane of the replica ends
 // with the same bit pattern in E.
+//
+// With a receptor split (sp.ns == 2) every thread of the workgroup must call
+// this together: `live` (wave-uniform) says whether the wave has a pose to
+// evaluate; the groups exchange their partial sums through LDS behind one
+// workgroup barrier. Two-operand float sums are order-free, so both groups
+// end with identical bits.
 template <int APL>
-__device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, int lpr, const float (&x)[APL],
+__device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, int lpr, Split &sp,
+                            bool live, const float (&x)[APL],
                             const float (&y)[APL], const float (&z)[APL],
                             const float (&ox)[APL], const float (&oy)[APL],
                             const float (&oz)[APL], const int (&lt)[APL],
@@ -217,10 +235,12 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
     psprow[k] = C.psp + (size_t)lt[k] * C.n_res;
   }
 
-  for (int ci = 0; ci < n_chunks; ++ci) {
+  for (int ci = 0; live && ci < n_chunks; ++ci) {
     const int4 ch = S.ch[ci];
-    const int beg = __builtin_amdgcn_readfirstlane(ch.x);
-    const int end = beg + __builtin_amdgcn_readfirstlane(ch.y);
+    const int cb = __builtin_amdgcn_readfirstlane(ch.x);
+    const int cl = __builtin_amdgcn_readfirstlane(ch.y);
+    const int beg = cb + (cl * sp.h) / sp.ns;
+    const int end = cb + (cl * (sp.h + 1)) / sp.ns;
     const int type = __builtin_amdgcn_readfirstlane(ch.z);
     const int kind = __builtin_amdgcn_readfirstlane(ch.w);
     ChunkPar cp[APL];
@@ -262,11 +282,10 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
                                  hpc, cnt);
   }
 
-  // per-atom terms
+  // pair sums that are linear in the receptor points
   float t_vdw = 0.f, t_ele = 0.f, t_pmf = 0.f, t_psp = 0.f, t_hdb = 0.f, t_hpc = 0.f;
   float t_kde = 0.f, t_rms = 0.f;
   int t_cnt = 0, t_tp = 0;
-  const float kde2l = C.kde2l;
 #pragma unroll
   for (int k = 0; k < APL; ++k) {
     if (!val[k]) continue;
@@ -275,10 +294,58 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
     t_pmf += pmf[k];
     t_psp += psp[k];
     t_hdb += hdb[k];
+    t_cnt += cnt[k];
+  }
+  t_vdw = seg_sum(t_vdw, lpr);
+  t_ele = seg_sum(t_ele, lpr);
+  t_pmf = seg_sum(t_pmf, lpr);
+  t_psp = seg_sum(t_psp, lpr);
+  t_hdb = seg_sum(t_hdb, lpr);
+  t_cnt = seg_sum_i(t_cnt, lpr);
+  if (sp.ns > 1) {
+    // exchange: per-atom HPC sums (the restraint is nonlinear in them) and
+    // the replica's linear sums
+    float *buf = S.xp + sp.par * sp.xp_floats;
+    float *hb = buf;                                // [n_split*half][4]
+    float *rb = buf + (size_t)sp.ns * sp.half * 4;  // [n_split][slots][8]
+    if (live) {
+#pragma unroll
+      for (int k = 0; k < APL; ++k) hb[threadIdx.x * 4 + k] = hpc[k];
+      if ((sp.ltid - sp.slot * lpr) == 0) {
+        float *r = rb + ((size_t)sp.h * sp.slots + sp.slot) * 8;
+        r[0] = t_vdw;
+        r[1] = t_ele;
+        r[2] = t_pmf;
+        r[3] = t_psp;
+        r[4] = t_hdb;
+        r[5] = __int_as_float(t_cnt);
+      }
+    }
+    __syncthreads();
+    if (live) {
+      const int other = threadIdx.x + (sp.h == 0 ? sp.half : -sp.half);
+#pragma unroll
+      for (int k = 0; k < APL; ++k) hpc[k] += hb[other * 4 + k];
+      const float *r = rb + ((size_t)(1 - sp.h) * sp.slots + sp.slot) * 8;
+      t_vdw += r[0];
+      t_ele += r[1];
+      t_pmf += r[2];
+      t_psp += r[3];
+      t_hdb += r[4];
+      t_cnt += __float_as_int(r[5]);
+    }
+    sp.par ^= 1;
+  }
+  if (!live) return;
+
+  // per-atom terms
+  const float kde2l = C.kde2l;
+#pragma unroll
+  for (int k = 0; k < APL; ++k) {
+    if (!val[k]) continue;
     // hydrophobic restraint, src/kernel_cuda_l2_calcenergy.cu:200-205
     const float h2 = (hpc[k] - C.hpl0[lt[k]]) / C.hpl1[lt[k]];
     t_hpc += 0.5f * h2 * h2 - C.hpl2[lt[k]];
-    t_cnt += cnt[k];
     // KDE: mean over same-type points, src/kernel_cuda_l2_calcenergy.cu:224-253
     const int2 kr = S.kr[lt[k]];
     float ks = 0.0f;
@@ -320,15 +387,9 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
     if (n > 0.0f) elhm += C.mcs_tcc[m] * sqrtf(s / n);
   }
 
-  t_vdw = seg_sum(t_vdw, lpr);
-  t_ele = seg_sum(t_ele, lpr);
-  t_pmf = seg_sum(t_pmf, lpr);
-  t_psp = seg_sum(t_psp, lpr);
-  t_hdb = seg_sum(t_hdb, lpr);
   t_hpc = seg_sum(t_hpc, lpr);
   t_kde = seg_sum(t_kde, lpr);
   t_rms = seg_sum(t_rms, lpr);
-  t_cnt = seg_sum_i(t_cnt, lpr);
   t_tp = seg_sum_i(t_tp, lpr);
 
   // src/kernel_cuda_l2_calcenergy.cu:334-359, src/kernel_cuda_l3_combineenergy.cu:28-37
@@ -510,8 +571,18 @@ template <int APL, bool DA>
 __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const DevComplex &C, const Lds &S,
                          int lpr) {
   const int tid = threadIdx.x;
-  const int slot = tid / lpr;
-  const int a = tid - slot * lpr;
+  Split sp;
+  sp.ns = P.n_split;
+  sp.half = P.half;
+  sp.h = tid >= sp.half ? 1 : 0;
+  sp.ltid = tid - sp.h * sp.half;
+  sp.slots = P.xp_slots;
+  sp.xp_floats = P.xp_floats;
+  sp.par = 0;
+  const int slot = sp.ltid / lpr;
+  sp.slot = slot;
+  const int a = sp.ltid - slot * lpr;
+  const bool lead = a == 0 && sp.h == 0;  // writes the replica's results
   const int n_slots = pk.n_slots;
   const bool active = slot < n_slots;
   const int sl = active ? slot : 0;
@@ -566,8 +637,8 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   if (P.mode != MODE_MC) {
     place(mo);
     gd_energy E;
-    pose_energy<APL>(C, S, lpr, x, y, z, ox, oy, oz, lt, q, li, val, mo, E);
-    if (active && a == 0) {
+    pose_energy<APL>(C, S, lpr, sp, true, x, y, z, ox, oy, oz, lt, q, li, val, mo, E);
+    if (active && lead) {
       if (P.mode == MODE_EVAL) {
         P.ev_out[sv] = E;
       } else {
@@ -604,7 +675,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
 
   auto record = [&](const gd_energy &E, int step) {
     if (P.rec && active) {
-      if (a == 0 && rc < P.rec_cap) {
+      if (lead && rc < P.rec_cap) {
         gd_record &r = P.rec[(size_t)sv * P.rec_cap + rc];
         r.replica.idx_rep = flat;
         r.replica.idx_prt = pk.prt;
@@ -633,17 +704,19 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   }
 
   for (int s = P.step0; s < P.step0 + P.n_steps; s += K) {
-    if (wave_active) {
     if constexpr (!DA) {
-      float u[8], dm[6], mn[6];
-      mc_uniforms(P.k0, P.k1, cid, flat, s, u);
-      move_offsets(u, scale, C.move_kind, dm);
+      float u[8], mn[6];
+      if (wave_active) {
+        float dm[6];
+        mc_uniforms(P.k0, P.k1, cid, flat, s, u);
+        move_offsets(u, scale, C.move_kind, dm);
 #pragma unroll
-      for (int i = 0; i < 6; ++i) mn[i] = mo[i] + dm[i];
-      place(mn);
+        for (int i = 0; i < 6; ++i) mn[i] = mo[i] + dm[i];
+        place(mn);
+      }
       gd_energy E;
-      pose_energy<APL>(C, S, lpr, x, y, z, ox, oy, oz, lt, q, li, val, mn, E);
-      if (metropolis(E.e[GD_MAXWEI - 1], eo.e[GD_MAXWEI - 1], mb, u[6])) {
+      pose_energy<APL>(C, S, lpr, sp, wave_active, x, y, z, ox, oy, oz, lt, q, li, val, mn, E);
+      if (wave_active && metropolis(E.e[GD_MAXWEI - 1], eo.e[GD_MAXWEI - 1], mb, u[6])) {
 #pragma unroll
         for (int i = 0; i < 6; ++i) mo[i] = mn[i];
         eo = E;
@@ -656,7 +729,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
       for (int i = 0; i < 6; ++i) yv[i] = mo[i];
       float s_y = s_old, u7 = 1.0f;
       bool moved = false;
-      for (int j = 0; j < K; ++j) {
+      for (int j = 0; wave_active && j < K; ++j) {
         float u[8], dm[6], mn[6];
         mc_uniforms(P.k0, P.k1, cid, flat, s + j, u);
         move_offsets(u, scale, C.move_kind, dm);
@@ -672,10 +745,11 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
         }
         u7 = u[7];
       }
-      if (__ballot(moved) != 0) {
-        place(yv);
-        gd_energy E;
-        pose_energy<APL>(C, S, lpr, x, y, z, ox, oy, oz, lt, q, li, val, yv, E);
+      const bool eval = wave_active && __ballot(moved) != 0;
+      if (eval) place(yv);
+      gd_energy E;
+      pose_energy<APL>(C, S, lpr, sp, eval, x, y, z, ox, oy, oz, lt, q, li, val, yv, E);
+      if (eval) {
         const float d1 = E.e[GD_MAXWEI - 1] - eo.e[GD_MAXWEI - 1];
         const float d2 = s_y - s_old;
         const float delta = d1 - d2;
@@ -690,9 +764,8 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
         }
       }
     }
-    }
     if (nc > 0 && ((s + K) % nc) == 0) {
-      if (active && a == 0) S.xe[slot] = eo.e[GD_MAXWEI - 1];
+      if (active && lead) S.xe[slot] = eo.e[GD_MAXWEI - 1];
       __syncthreads();
       if (tid < pk.n_groups) {
         const int n_tmp = C.n_tmp;
@@ -711,7 +784,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
     }
   }
 
-  if (active && a == 0) {
+  if (active && lead) {
 #pragma unroll
     for (int i = 0; i < 6; ++i) P.mm[(size_t)sv * 6 + i] = mo[i];
     P.e_old[sv] = eo;
@@ -739,6 +812,7 @@ __device__ __forceinline__ void mc_kernel_body(const KParams &P) {
   S.xi = reinterpret_cast<int *>(lds + P.off_xi);
   S.xa = reinterpret_cast<int *>(lds + P.off_xa);
   S.xo = reinterpret_cast<int *>(lds + P.off_xo);
+  S.xp = reinterpret_cast<float *>(lds + P.off_xp);
 
   // stage the pack's receptor conformer and tables
   const int pnp = C.pnp;

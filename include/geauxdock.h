@@ -162,6 +162,8 @@ typedef struct gd_opts {
   int lanes_per_replica; /* 0 = auto; else 8, 16, 32 or 64 */
   int atoms_per_lane;    /* 0 = auto; else 1..4 */
   int verbose;
+  int receptor_split;    /* 0 = auto; 1 = one lane group per replica; 2 = two lane
+                            groups per replica, each over half of every receptor chunk */
 } gd_opts;
 
 typedef struct gd_run_stats {

@@ -58,17 +58,19 @@ def check_energy_parity(oracle, eng, cid, cx, mm, idx_lig, idx_prt, report):
     return out
 
 
-@pytest.mark.parametrize("lpr,apl", [(8, 4), (16, 2), (16, 3), (32, 1), (64, 1)])
-def test_energy_terms_vs_fp64(oracle, eng_mod, lpr, apl):
+@pytest.mark.parametrize("lpr,apl,split", [(8, 4, 1), (8, 4, 2), (16, 2, 1), (16, 3, 1),
+                                           (16, 3, 2), (32, 1, 2), (64, 1, 1)])
+def test_energy_terms_vs_fp64(oracle, eng_mod, lpr, apl, split):
     cx = synth.config0(seed=3, identity_norm=True, void_radius=7.0)
     rng = np.random.default_rng(11)
     mm = random_poses(rng, 48)
-    with eng_mod.Engine(seed=1, lanes_per_replica=lpr, atoms_per_lane=apl) as eng:
+    with eng_mod.Engine(seed=1, lanes_per_replica=lpr, atoms_per_lane=apl,
+                        receptor_split=split) as eng:
         cid = eng.add_complex(cx)
         rep = {}
         check_energy_parity(oracle, eng, cid, cx, mm, np.zeros(48, np.int32),
                             np.zeros(48, np.int32), rep)
-    print("\nlpr", lpr, "apl", apl, "worst (rel, cond):",
+    print("\nlpr", lpr, "apl", apl, "split", split, "worst (rel, cond):",
           {k: (f"{v[0]:.2e}", f"{v[1]:.2e}") for k, v in rep.items()})
 
 
@@ -174,13 +176,14 @@ def _run_gpu(eng_mod, cx, seed, n_steps, nc, **kw):
     return recs, state, st
 
 
-def test_mc_run_tracks_cpu_path(oracle, eng_mod):
+@pytest.mark.parametrize("split", [1, 2])
+def test_mc_run_tracks_cpu_path(oracle, eng_mod, split):
     """Same seed, same random stream: the GPU engine and the FP32 CPU path make
     the same accept / exchange decisions, step for step, unless an energy falls
-    within rounding of a threshold."""
+    within rounding of a threshold. Both receptor-split layouts."""
     cx = synth.config0(seed=0)
     n_steps, nc = 400, 20
-    recs, state, st = _run_gpu(eng_mod, cx, 7, n_steps, nc)
+    recs, state, st = _run_gpu(eng_mod, cx, 7, n_steps, nc, receptor_split=split)
     d = cx.desc()
     cpu = oracle.McState(cx, rec_cap=n_steps + 1)
     oracle.mc_run(d, cpu, seed=7, init=True, step0=0, n_steps=0, nc=nc)
