@@ -45,6 +45,17 @@ def parse():
     ap.add_argument("--lpr", type=int, default=0)
     ap.add_argument("--apl", type=int, default=0)
     ap.add_argument("--split", type=int, default=0, help="receptor split (0 = auto, 1, 2)")
+    ap.add_argument("--mode", default="plain", choices=["plain", "da"],
+                    help="plain engine only, or plain vs surrogate delayed acceptance")
+    ap.add_argument("--k-sub", type=int, default=4, help="surrogate steps per exact evaluation")
+    ap.add_argument("--log-steps", type=int, default=1000, help="plain steps logged for training")
+    ap.add_argument("--log-poses", type=int, default=400000)
+    ap.add_argument("--train-poses", type=int, default=200000)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--batch-poses", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--grid-spacing", type=float, default=0.5)
+    ap.add_argument("--grid-half", type=float, default=14.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -118,6 +129,95 @@ def kernel_names(eng):
     return ", ".join(f"gd::gd_mc_kernel<{a}, false>" for a in apls)
 
 
+def da_compare(args, cxs, desc):
+    """Plain vs delayed acceptance on the same complexes, seed and start state.
+
+    1. plain warmup, then log accepted states of a plain run (reservoir);
+    2. label training poses (logged states and states moved by 1-4
+       proposals) with the exact energies, train the MLP on the GPU, report
+       the held-out RMSE;
+    3. from one checkpoint, time `steps` plain steps and `steps` DA steps
+       (k_sub surrogate steps per exact evaluation) without records, and
+       compare replica-steps/s and the best total energy reached.
+    """
+    import numpy as np
+    import geaxdock_gpu_amd as gd
+    from geaxdock_gpu_amd import surrogate as sg
+
+    k = args.k_sub
+    if args.steps % k or args.warmup % k or args.log_steps % k or args.nc % k:
+        raise SystemExit("steps, warmup, log-steps and nc must be multiples of k-sub")
+    n_rep = sum(c.n_rep for c in cxs)
+    eng = gd.Engine(seed=0, record=True, steps_per_dump=500, receptor_split=args.split)
+    for cx in cxs:
+        eng.add_complex(cx)
+    eng.run(args.warmup, args.nc)
+    log = sg.PoseLog(args.log_poses, seed=0)
+    t0 = time.perf_counter()
+    eng.run(args.log_steps, args.nc, sink=log)
+    t_log = time.perf_counter() - t0
+    recs, rcid = log.records()
+    t0 = time.perf_counter()
+    eng.sur_build_grids(args.grid_spacing, args.grid_half)
+    ts = sg.build_training_set(eng, recs, rcid, args.train_poses, seed=1)
+    m, sigma = sg.load_training_set(eng, ts)
+    t_data = time.perf_counter() - t0
+    fit = sg.fit(eng, ts, sigma, n_train=int(0.9 * len(ts["t"])), epochs=args.epochs,
+                 batch_poses=args.batch_poses, lr=args.lr)
+    params = eng.sur_get_params()
+    blob = eng.checkpoint()
+    eng.close()
+
+    run = gd.Engine(seed=0, record=False, steps_per_dump=2000, receptor_split=args.split)
+    for cx in cxs:
+        run.add_complex(cx)
+    run.restore(blob)
+    run.sur_build_grids(args.grid_spacing, args.grid_half)
+    run.sur_set_params(params)
+
+    def best_energy():
+        return float(min(run.best(c)[0].min() for c in range(len(cxs))))
+
+    out = {}
+    for mode in ("plain", "da"):
+        run.restore(blob)
+        run.reset_best()
+        t0 = time.perf_counter()
+        if mode == "plain":
+            st = run.run(args.steps, args.nc)
+        else:
+            st = run.run_da(args.steps, args.nc, k, sigma)
+        dt = time.perf_counter() - t0
+        out[mode] = {"replica_steps_per_s": n_rep * args.steps / dt,
+                     "kernel_replica_steps_per_s": n_rep * args.steps / st["t_compute"],
+                     "best_energy": best_energy(),
+                     "accepted_per_exact_eval": st["accepted"] / (st["replica_steps"] / (k if mode == "da" else 1)),
+                     "accepted_per_step": st["accepted"] / st["replica_steps"],
+                     "kernel_ms_avg": st["kernel_ms_avg"], "seconds": dt}
+    run.close()
+    speed = out["da"]["replica_steps_per_s"] / out["plain"]["replica_steps_per_s"]
+    d_best = out["da"]["best_energy"] - out["plain"]["best_energy"]
+    return {
+        "metric": "MC replica-steps/s, plain vs surrogate-DA, at matched best-pose energy",
+        "value": out["da"]["replica_steps_per_s"],
+        "unit": "replica-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": out["da"]["seconds"] / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded generator, geaxdock_gpu_amd/synth.py)",
+        "config": dict(desc, replicas=n_rep, steps_per_exchange=args.nc, k_sub=k),
+        "plain": out["plain"], "da": out["da"],
+        "da_speedup": speed, "best_energy_delta": d_best,
+        "target_met": bool(speed >= 3.0 and abs(d_best) <= 0.1),
+        "surrogate": {"log_seconds": t_log, "label_seconds": t_data, "sigma": sigma,
+                      "train_poses": fit["n_train"], "test_poses": fit["n_test"],
+                      "epochs": args.epochs, "train_seconds": fit["seconds"],
+                      "train_loss": fit["train_loss"], "test_rmse_std": fit["test_rmse_std"],
+                      "test_rmse_energy": fit["test_rmse_energy"],
+                      "test_energy_std": fit["test_energy_std"]},
+    }
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -132,6 +232,9 @@ def main():
     import geaxdock_gpu_amd as gd
 
     cxs, desc = workload(args)
+    if args.mode == "da":
+        print(json.dumps(da_compare(args, cxs, desc)))
+        return
     eng = gd.Engine(seed=0, record=True, steps_per_dump=2000, lanes_per_replica=args.lpr,
                     atoms_per_lane=args.apl, receptor_split=args.split)
     for cx in cxs:

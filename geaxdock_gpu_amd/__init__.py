@@ -58,6 +58,10 @@ def load_library():
     lib.gd_philox.argtypes = [vp, C.c_int, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     lib.gd_num_replicas.argtypes = [vp, C.c_int]
     lib.gd_num_replicas.restype = C.c_int
+    lib.gd_get_best.argtypes = [vp, C.c_int, P(C.c_float), P(C.c_float)]
+    lib.gd_get_best.restype = C.c_int
+    lib.gd_reset_best.argtypes = [vp]
+    lib.gd_reset_best.restype = C.c_int
     lib.gd_complex_shape.argtypes = [vp, C.c_int, P(C.c_int), P(C.c_int)]
     lib.gd_complex_shape.restype = C.c_int
     lib.gd_run_da.argtypes = [vp, C.c_int64, C.c_int, C.c_int, C.c_float, abi.RECORD_SINK, vp,
@@ -184,6 +188,17 @@ class Engine:
         self._check(self.lib.gd_run_da(self._ctx, int(n_steps), int(steps_per_exchange),
                                        int(k_sub), float(sur_scale), cb, None, C.byref(st)))
         return {f: getattr(st, f) for f, _ in abi.RunStats._fields_}
+
+    def best(self, cid):
+        """(best total energy [n_rep], its movematrix [n_rep, 6]) per replica."""
+        n = self.num_replicas(cid)
+        e = np.zeros(n, np.float32)
+        mm = np.zeros((n, 6), np.float32)
+        self._check(self.lib.gd_get_best(self._ctx, cid, _f(e), _f(mm)))
+        return e, mm
+
+    def reset_best(self):
+        self._check(self.lib.gd_reset_best(self._ctx))
 
     def shape(self, cid):
         """(lanes per replica, ligand atoms per lane) chosen for a complex."""

@@ -55,6 +55,7 @@ struct gd_ctx {
   int64_t step = 0;
   float *d_mm = nullptr;
   gd_energy *d_eold = nullptr;
+  float *d_best_e = nullptr, *d_best_mm = nullptr;
   int *d_idx_tmp = nullptr, *d_acs_mc = nullptr, *d_acs_x = nullptr, *d_rep_flat = nullptr;
   gd_record *d_rec = nullptr;
   int *d_rec_cnt = nullptr;
@@ -213,7 +214,7 @@ void gd_destroy(gd_ctx *ctx) {
     for (void *p : h.bufs) (void)hipFree(p);
   void *ps[] = {ctx->d_cx, ctx->d_overflow, ctx->d_mm, ctx->d_eold, ctx->d_idx_tmp,
                 ctx->d_acs_mc, ctx->d_acs_x, ctx->d_rep_flat, ctx->d_rec, ctx->d_rec_cnt,
-                ctx->d_packs, ctx->d_slots};
+                ctx->d_packs, ctx->d_slots, ctx->d_best_e, ctx->d_best_mm};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   {
@@ -646,6 +647,8 @@ gd_status alloc_state(gd_ctx *ctx) {
   ctx->n_rep = n;
   HIPCHK(hipMalloc(&ctx->d_mm, (size_t)n * 6 * sizeof(float)));
   HIPCHK(hipMalloc(&ctx->d_eold, (size_t)n * sizeof(gd_energy)));
+  HIPCHK(hipMalloc(&ctx->d_best_e, (size_t)n * sizeof(float)));
+  HIPCHK(hipMalloc(&ctx->d_best_mm, (size_t)n * 6 * sizeof(float)));
   HIPCHK(hipMalloc(&ctx->d_idx_tmp, (size_t)n * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_acs_mc, (size_t)n * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_acs_x, (size_t)n * sizeof(int)));
@@ -714,6 +717,8 @@ KParams base_params(gd_ctx *ctx) {
   P.slots = ctx->d_slots;
   P.mm = ctx->d_mm;
   P.e_old = ctx->d_eold;
+  P.best_e = ctx->d_best_e;
+  P.best_mm = ctx->d_best_mm;
   P.idx_tmp = ctx->d_idx_tmp;
   P.acs_mc = ctx->d_acs_mc;
   P.acs_x = ctx->d_acs_x;
@@ -805,6 +810,7 @@ gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float
   P.n_steps = n_steps;
   P.nc = nc;
   size_t lds = ctx->lds;
+  int block = ctx->block;
   if (da_k > 0 && mode == MODE_MC) {
     auto &S = ctx->sur;
     P.da_k = da_k;
@@ -815,12 +821,19 @@ gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float
     P.sw3 = S.p + SP_W3;
     P.sb3 = S.p + SP_B3;
     P.off_da = (int)((lds + 15) & ~(size_t)15);
-    lds = P.off_da + (size_t)(ctx->block / 64) * 6208;
+    lds = P.off_da + (size_t)(block / 64) * 6208;
+    if (lds > ctx->max_lds && P.n_split > 1) {
+      // the per-wave surrogate scratch of both lane groups does not fit:
+      // run this launch with one group per pack (same packs and state)
+      P.n_split = 1;
+      block = P.half;
+      lds = P.off_da + (size_t)(block / 64) * 6208;
+    }
     if (lds > ctx->max_lds)
       return fail(ctx, GD_ERR_CAPACITY, "delayed acceptance needs %zu bytes of LDS", lds);
   }
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  HIPCHK(launch_classes(ctx, ctx->cls, P, P.da_k > 0, ctx->block, lds, ctx->stream));
+  HIPCHK(launch_classes(ctx, ctx->cls, P, P.da_k > 0, block, lds, ctx->stream));
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(hipEventSynchronize(ctx->ev1));
@@ -929,13 +942,39 @@ gd_status gd_get_state(gd_ctx *ctx, int cid, float *movematrix, gd_energy *energ
   return GD_OK;
 }
 
-static const uint32_t kCkptMagic = 0x47444b31u;  // "GDK1"
+gd_status gd_get_best(gd_ctx *ctx, int cid, float *best_e, float *best_mm) {
+  if (!ctx || cid < 0 || cid >= (int)ctx->cx.size()) return GD_ERR_INVALID;
+  if (!ctx->initialized) return fail(ctx, GD_ERR_STATE, "no state before gd_run");
+  HIPCHK(hipSetDevice(ctx->device));
+  const HostComplex &h = ctx->cx[cid];
+  const size_t b = h.rep_base, n = h.n_rep;
+  if (best_e)
+    HIPCHK(hipMemcpyAsync(best_e, ctx->d_best_e + b, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  if (best_mm)
+    HIPCHK(hipMemcpyAsync(best_mm, ctx->d_best_mm + b * 6, n * 6 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GD_OK;
+}
+
+gd_status gd_reset_best(gd_ctx *ctx) {
+  if (!ctx) return GD_ERR_INVALID;
+  if (!ctx->initialized) return fail(ctx, GD_ERR_STATE, "no state before gd_run");
+  HIPCHK(hipSetDevice(ctx->device));
+  const int n = ctx->n_rep;
+  hipLaunchKernelGGL(gd_reset_best_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
+                     ctx->d_eold, ctx->d_mm, ctx->d_best_e, ctx->d_best_mm);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GD_OK;
+}
+
+static const uint32_t kCkptMagic = 0x47444b32u;  // "GDK2"
 
 gd_status gd_checkpoint(gd_ctx *ctx, void *buf, size_t *size) {
   if (!ctx || !size) return GD_ERR_INVALID;
   if (!ctx->state_alloc) return fail(ctx, GD_ERR_STATE, "no state before gd_run");
   const size_t n = ctx->n_rep;
-  const size_t need = 32 + n * (6 * sizeof(float) + sizeof(gd_energy) + 3 * sizeof(int));
+  const size_t need = 32 + n * (13 * sizeof(float) + sizeof(gd_energy) + 3 * sizeof(int));
   if (!buf || *size < need) {
     *size = need;
     return buf ? fail(ctx, GD_ERR_INVALID, "checkpoint buffer too small") : GD_OK;
@@ -956,6 +995,10 @@ gd_status gd_checkpoint(gd_ctx *ctx, void *buf, size_t *size) {
   HIPCHK(hipMemcpyAsync(o, ctx->d_acs_mc, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   o += n * sizeof(int);
   HIPCHK(hipMemcpyAsync(o, ctx->d_acs_x, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  o += n * sizeof(int);
+  HIPCHK(hipMemcpyAsync(o, ctx->d_best_e, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  o += n * sizeof(float);
+  HIPCHK(hipMemcpyAsync(o, ctx->d_best_mm, n * 6 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   *size = need;
   return GD_OK;
@@ -966,7 +1009,7 @@ gd_status gd_restore(gd_ctx *ctx, const void *buf, size_t size) {
   HIPCHK(hipSetDevice(ctx->device));
   TRY(alloc_state(ctx));
   const size_t n = ctx->n_rep;
-  const size_t need = 32 + n * (6 * sizeof(float) + sizeof(gd_energy) + 3 * sizeof(int));
+  const size_t need = 32 + n * (13 * sizeof(float) + sizeof(gd_energy) + 3 * sizeof(int));
   uint32_t hdr[8];
   if (size < need) return fail(ctx, GD_ERR_INVALID, "checkpoint too small");
   memcpy(hdr, buf, 32);
@@ -982,6 +1025,10 @@ gd_status gd_restore(gd_ctx *ctx, const void *buf, size_t size) {
   HIPCHK(hipMemcpyAsync(ctx->d_acs_mc, o, n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   o += n * sizeof(int);
   HIPCHK(hipMemcpyAsync(ctx->d_acs_x, o, n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  o += n * sizeof(int);
+  HIPCHK(hipMemcpyAsync(ctx->d_best_e, o, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  o += n * sizeof(float);
+  HIPCHK(hipMemcpyAsync(ctx->d_best_mm, o, n * 6 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemsetAsync(ctx->d_rec_cnt, 0, n * sizeof(int), ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->initialized = hdr[2] != 0;

@@ -75,6 +75,8 @@ struct KParams {
   // replica state (MODE_MC / MODE_INIT), indexed by global replica id
   float *mm;
   gd_energy *e_old;
+  float *best_e;   // lowest total energy each replica has held, and its pose
+  float *best_mm;
   int *idx_tmp;
   int *acs_mc;
   int *acs_x;

@@ -96,7 +96,7 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const PairConst &
                                            float (&vdw)[APL], float (&ele)[APL],
                                            float (&pmf)[APL], float (&psp)[APL],
                                            float (&hdb)[APL], float (&hpc)[APL],
-                                           int (&cnt)[APL]) {
+                                           int (&cnt)[APL], float (&vdwf)[APL]) {
   const float pa = -0.5f * LOG2E;
   int nfar = 0;
   float4 P[NP];
@@ -152,7 +152,7 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const PairConst &
 #pragma unroll
     for (int j = 0; j < NP; ++j)
 #pragma unroll
-      for (int k = 0; k < APL; ++k) vdw[k] = fmaf(num[j][k], fast_rcp(den[j][k]), vdw[k]);
+      for (int k = 0; k < APL; ++k) vdwf[k] = fmaf(num[j][k], fast_rcp(den[j][k]), vdwf[k]);
     nfar = NP;
   }
   if (__ballot(near) != 0) {
@@ -188,15 +188,23 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, cons
   // points per group: enough independent pairs per lane to cover latency
   constexpr int NP = APL >= 3 ? 1 : (APL == 2 ? 2 : 4);
   int p = beg, nfar = 0;
+  // far-field vdW has its own accumulators so that no accumulator is
+  // updated on both sides of the far/mid branch (keeps them in place)
+  float vdwf[APL];
+#pragma unroll
+  for (int k = 0; k < APL; ++k) vdwf[k] = 0.0f;
   for (; p + NP <= end; p += NP)
     nfar += pair_group<APL, KIND, NP>(S, p, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc,
-                                      cnt);
+                                      cnt, vdwf);
   for (; p < end; ++p)
     nfar += pair_group<APL, KIND, 1>(S, p, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc,
-                                     cnt);
+                                     cnt, vdwf);
   const float nf = (float)nfar;
 #pragma unroll
-  for (int k = 0; k < APL; ++k) pmf[k] = fmaf(cp[k].pmf1, nf, pmf[k]);
+  for (int k = 0; k < APL; ++k) {
+    pmf[k] = fmaf(cp[k].pmf1, nf, pmf[k]);
+    vdw[k] += vdwf[k];
+  }
 }
 
 // Energy, RMSD and MCC of one pose held by `lpr` lanes (CalcEnergy_d +
@@ -473,7 +481,8 @@ __device__ __forceinline__ float cheap_terms(const DevComplex &C, const Lds &S, 
 }
 
 // Surrogate total of the pose held by each replica of the wave: the MLP's
-// receptor-pair estimate (scaled) plus the exact cheap terms. Rows are the
+// receptor-pair estimate, scale * sum_atoms y / lna, plus the exact cheap
+// terms. Rows are the
 // wave's atoms, 16 per MFMA tile: tile (k, sub) holds atom k of lanes
 // 16 sub .. 16 sub + 15. `da` is the wave's scratch: X tile [16][64] bf16,
 // A1 tile [16][128] bf16, y[16].
@@ -535,7 +544,9 @@ __device__ __forceinline__ float sur_total(const KParams &P, const DevComplex &C
     }
   }
   ysum = seg_sum(ysum, lpr);
-  return P.sur_scale * ysum + cheap_terms<APL>(C, S, lpr, x, y, z, lt, li, val, mm);
+  // the network sums per-atom pair energies; the terms are per-atom means
+  return P.sur_scale * (ysum / (float)C.lna) +
+         cheap_terms<APL>(C, S, lpr, x, y, z, lt, li, val, mm);
 }
 
 // ExchangeReplicas_d for one ladder (src/kernel_cuda_l1_exchangereplicas.cu:33-78):
@@ -574,7 +585,8 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   Split sp;
   sp.ns = P.n_split;
   sp.half = P.half;
-  sp.h = tid >= sp.half ? 1 : 0;
+  // half is a multiple of 64, so the group index is wave-uniform
+  sp.h = __builtin_amdgcn_readfirstlane(tid >= sp.half ? 1 : 0);
   sp.ltid = tid - sp.h * sp.half;
   sp.slots = P.xp_slots;
   sp.xp_floats = P.xp_floats;
@@ -643,6 +655,9 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
         P.ev_out[sv] = E;
       } else {
         P.e_old[sv] = E;
+        P.best_e[sv] = E.e[GD_MAXWEI - 1];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) P.best_mm[(size_t)sv * 6 + i] = mo[i];
         if (P.rec) {
           const int rc = P.rec_cnt[sv];
           if (rc < P.rec_cap) {
@@ -668,6 +683,15 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   float mb = C.minus_beta[S.xi[sl]];
   int acs = 0;
   int rc = P.rec ? P.rec_cnt[sv] : 0;
+  float best = P.best_e[sv];
+  auto track_best = [&]() {
+    if (eo.e[GD_MAXWEI - 1] < best) {
+      best = eo.e[GD_MAXWEI - 1];
+      if (active && lead)
+#pragma unroll
+        for (int i = 0; i < 6; ++i) P.best_mm[(size_t)sv * 6 + i] = mo[i];
+    }
+  };
   const bool wave_active = __ballot(active) != 0;
   float scale[6];
 #pragma unroll
@@ -722,6 +746,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
         eo = E;
         acs += 1;
         record(E, s);
+        track_best();
       }
     } else {
       float yv[6];
@@ -761,6 +786,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
           s_old = s_y;
           acs += 1;
           record(E, s + K - 1);
+          track_best();
         }
       }
     }
@@ -788,6 +814,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
 #pragma unroll
     for (int i = 0; i < 6; ++i) P.mm[(size_t)sv * 6 + i] = mo[i];
     P.e_old[sv] = eo;
+    P.best_e[sv] = best;
     P.idx_tmp[sv] = S.xi[slot];
     P.acs_mc[sv] += acs;
     P.acs_x[sv] = S.xa[slot];
@@ -855,6 +882,15 @@ __device__ __forceinline__ void mc_kernel_body(const KParams &P) {
 template <int APL, bool DA>
 __global__ void __launch_bounds__(512) gd_mc_kernel(KParams P) {
   mc_kernel_body<APL, DA>(P);
+}
+
+// Best-so-far := current state, for every replica.
+extern "C" __global__ void gd_reset_best_kernel(int n, const gd_energy *e_old, const float *mm,
+                                                float *best_e, float *best_mm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  best_e[i] = e_old[i].e[GD_MAXWEI - 1];
+  for (int k = 0; k < 6; ++k) best_mm[(size_t)i * 6 + k] = mm[(size_t)i * 6 + k];
 }
 
 // Reference contact map of a complex (InitRefMatrix_d,

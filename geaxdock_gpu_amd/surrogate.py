@@ -7,7 +7,7 @@ read from per-conformer grids by trilinear interpolation). It learns the
 weighted sum of the six receptor-pair terms (vdW, ele, PMF, PSP, HDB, HPC) in
 standardized units; the delayed-acceptance kernel adds the exact KDE, MCS
 and pocket-distance terms back, so the surrogate total is
-    S = scale * sum_atoms MLP(features) + cheap terms.
+    S = scale * sum_atoms MLP(features) / lna + cheap terms.
 
 Parameter vector layout (gd_sur_set_params): W1 as a [64][128] matrix whose
 rows 0..55 are the input weights, row 56 the first-layer bias and rows 57..63
@@ -70,58 +70,141 @@ def unpack_params(v):
     return out
 
 
-def pair_energy(energy, enepara):
-    """Weighted receptor-pair part of the total: sum_{i<6} w_i e_i."""
+def pair_target(energy, enepara, lna):
+    """Receptor-pair part of a pose's total as a sum over its ligand atoms:
+    T = lna * sum_{i<6} w_i (e_i - b_i) = sum_{i<6} w_i a_i raw_i, where the
+    normalized terms are e_i = a_i raw_i / lna + b_i."""
     w = np.asarray(enepara.w, np.float64)[PAIR_TERMS]
-    return np.asarray(energy["e"], np.float64)[:, PAIR_TERMS] @ w
+    b = np.asarray(enepara.b_para, np.float64)[PAIR_TERMS]
+    e = np.asarray(energy["e"], np.float64)[:, PAIR_TERMS]
+    return np.asarray(lna, np.float64) * ((e - b) @ w)
 
 
-def sample_poses(eng, cid, n, rng, t_sigma=2.0, from_state=True, jitter=(0.5, 0.3)):
-    """Training poses: the replicas' current poses jittered by a normal
-    offset (translation, rotation), plus uniformly random orientations around
-    the pocket centre. Returns (idx_lig, idx_prt, movematrix)."""
-    cx = eng.complexes[cid]
-    n_state = n // 2 if from_state else 0
-    mm = np.zeros((n, 6), np.float32)
-    il = rng.integers(0, cx.n_lig, n).astype(np.int32)
-    ip = rng.integers(0, cx.n_prt, n).astype(np.int32)
-    if n_state:
-        st = eng.state(cid)
-        pick = rng.integers(0, len(st["movematrix"]), n_state)
-        mm[:n_state] = st["movematrix"][pick]
-        mm[:n_state, :3] += rng.normal(0, jitter[0], (n_state, 3))
-        mm[:n_state, 3:] += rng.normal(0, jitter[1], (n_state, 3))
-        il[:n_state] = st["replica"][pick, 3]
-        ip[:n_state] = st["replica"][pick, 1]
-    r = n - n_state
-    mm[n_state:, :3] = rng.normal(0, t_sigma, (r, 3))
-    mm[n_state:, 3:] = rng.uniform(-np.pi, np.pi, (r, 3))
-    return il, ip, mm
+def perturb(mm, move_scale, move_kind, n_moves, rng):
+    """Apply n_moves random MC moves of the engine's proposal kind."""
+    mm = np.array(mm, np.float32, copy=True)
+    sc = np.asarray(move_scale, np.float32)
+    for _ in range(int(n_moves)):
+        if move_kind == 1:
+            mm += (rng.normal(0.0, 1.0, mm.shape) * sc).astype(np.float32)
+        else:
+            mm += ((2.0 * rng.random(mm.shape) - 1.0) * sc).astype(np.float32)
+    return mm
 
 
-def build_dataset(eng, cid, n, seed=0, **kw):
-    """Label n poses with the exact engine energy and append them to the
-    engine's training set. Returns (mean, std) of the pair-energy target,
-    which is stored standardized."""
+class PoseLog:
+    """Bounded reservoir of accepted states from an engine's record sink."""
+
+    def __init__(self, capacity, seed=0):
+        self.cap = int(capacity)
+        self.rng = np.random.default_rng(seed)
+        self.seen = 0
+        self.buf = None
+
+    def __call__(self, cid, recs):
+        n = len(recs)
+        if n == 0:
+            return
+        take = recs
+        if self.buf is None:
+            self.buf = np.zeros(self.cap, recs.dtype)
+            self.cid = np.zeros(self.cap, np.int32)
+            self.fill = 0
+        # reservoir sampling, vectorized per window
+        idx = self.seen + np.arange(n)
+        room = max(0, self.cap - self.fill)
+        if room:
+            k = min(room, n)
+            self.buf[self.fill:self.fill + k] = take[:k]
+            self.cid[self.fill:self.fill + k] = cid
+            self.fill += k
+            take, idx = take[k:], idx[k:]
+        if len(take):
+            j = (self.rng.random(len(take)) * (idx + 1)).astype(np.int64)
+            hit = j < self.cap
+            self.buf[j[hit]] = take[hit]
+            self.cid[j[hit]] = cid
+        self.seen += n
+
+    def records(self):
+        if self.buf is None:
+            return None, None
+        return self.buf[:self.fill], self.cid[:self.fill]
+
+
+def build_training_set(eng, recs, rec_cid, n_poses, perturbed=0.5, max_moves=4, seed=0):
+    """Poses for training: logged accepted states, and the same states moved
+    by 1..max_moves proposals (the surrogate ranks proposals). Labels are the
+    exact engine energies. Returns per-pose arrays (shuffled)."""
     rng = np.random.default_rng(seed)
-    il, ip, mm = sample_poses(eng, cid, n, rng, **kw)
-    e = eng.calc_energy(cid, il, ip, mm)
-    y = pair_energy(e, eng.complexes[cid].enepara_struct)
-    ok = np.isfinite(y)
-    # clip the steep repulsive wall: the Metropolis test rejects such poses
-    # whatever the exact value, so they only need to rank high
-    cap = np.percentile(y[ok], 95) if ok.any() else 0.0
-    y = np.where(ok, np.minimum(y, cap), cap)
-    mu, sd = float(y.mean()), float(y.std() + 1e-12)
-    eng.sur_dataset_add(cid, il, ip, mm, ((y - mu) / sd).astype(np.float32))
-    return mu, sd
+    pick = rng.integers(0, len(recs), n_poses)
+    r = recs[pick]
+    cid = rec_cid[pick].astype(np.int32)
+    il = r["replica"][:, 3].astype(np.int32)
+    ip = r["replica"][:, 1].astype(np.int32)
+    mm = r["movematrix"].astype(np.float32)
+    energy = r["energy"].copy()
+    n_pert = int(n_poses * perturbed)
+    for c in np.unique(cid[:n_pert]):
+        sel = np.nonzero(cid[:n_pert] == c)[0]
+        cx = eng.complexes[int(c)]
+        moves = rng.integers(1, max_moves + 1, len(sel))
+        for m in np.unique(moves):
+            s2 = sel[moves == m]
+            mm[s2] = perturb(mm[s2], cx.move_scale, cx.move_kind, m, rng)
+        energy[sel] = eng.calc_energy(int(c), il[sel], ip[sel], mm[sel])
+    lna = np.array([eng.complexes[int(c)].lna for c in cid], np.float64)
+    T = np.zeros(n_poses)
+    for c in np.unique(cid):
+        sel = cid == c
+        T[sel] = pair_target(energy[sel], eng.complexes[int(c)].enepara_struct, lna[sel])
+    ok = np.isfinite(T)
+    order = rng.permutation(np.nonzero(ok)[0])
+    return dict(cid=cid[order], idx_lig=il[order], idx_prt=ip[order], movematrix=mm[order],
+                target=T[order], lna=lna[order], energy=energy[order])
 
 
-def fit(eng, n_train, n_test, epochs=40, batch_poses=256, lr=2e-3, seed=0):
-    """Train on rows [0, n_train), report RMSE (standardized units) on the
-    held-out rows [n_train, n_train + n_test)."""
+def load_training_set(eng, ts, clip_pct=99.5):
+    """Standardize T per atom, t = (T - lna m) / sigma, clip the repulsive tail,
+    and append the poses to the engine's dataset in order. Returns (m, sigma):
+    the engine's surrogate scale is sigma."""
+    T, lna = ts["target"], ts["lna"]
+    m = float(np.mean(T / lna))
+    t = T - lna * m
+    sigma = float(np.std(t) + 1e-12)
+    t = t / sigma
+    cap = np.percentile(t, clip_pct)
+    t = np.minimum(t, cap).astype(np.float32)
+    ts["t"] = t
+    eng.sur_dataset_reserve(int(eng.sur_dataset_size() + lna.sum() + 64 * len(lna)))
+    # consecutive runs of one complex go in one call, keeping the pose order
+    cid = ts["cid"]
+    b = 0
+    n = len(cid)
+    while b < n:
+        e = b
+        while e < n and cid[e] == cid[b]:
+            e += 1
+        eng.sur_dataset_add(int(cid[b]), ts["idx_lig"][b:e], ts["idx_prt"][b:e],
+                            ts["movematrix"][b:e], t[b:e])
+        b = e
+    return m, sigma
+
+
+def fit(eng, ts, sigma, n_train, epochs=8, batch_poses=4096, lr=1e-3, seed=0):
+    """Train on poses [0, n_train) of the engine's dataset and evaluate the
+    held-out rest: RMSE in standardized units and of the pair energy per the
+    engine's normalization (sum_i w_i e_i units)."""
     eng.sur_set_params(pack_params(init_params(seed)))
+    n = len(ts["t"])
     hist, secs = eng.sur_train(0, n_train, epochs, batch_poses, lr)
-    lte, _ = eng.sur_loss_grad(n_train, n_train + n_test, grad=False)
-    return dict(train_loss=float(hist[-1]), test_rmse=float(np.sqrt(lte)), seconds=secs,
-                history=hist)
+    pred = eng.sur_predict(n_train, n)
+    t = ts["t"][n_train:]
+    lna = ts["lna"][n_train:]
+    err = (pred.astype(np.float64) - t) * sigma / lna
+    keep = t < t.max()  # clipped poses excluded from the energy RMSE
+    return dict(train_loss=float(hist[-1]), history=[float(h) for h in hist], seconds=secs,
+                test_rmse_std=float(np.sqrt(np.mean((pred - t) ** 2))),
+                test_rmse_energy=float(np.sqrt(np.mean(err[keep] ** 2))),
+                test_energy_std=float(np.std((ts["target"][n_train:] / lna)[keep])),
+                n_train=int(n_train), n_test=int(n - n_train))

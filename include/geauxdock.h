@@ -243,6 +243,15 @@ gd_status gd_philox(gd_ctx *ctx, int n, const uint32_t *ctr, const uint32_t *key
 
 /* Number of replicas of a complex (n_lig*n_prt*n_tmp). */
 int gd_num_replicas(const gd_ctx *ctx, int complex_id);
+/* Lowest total energy each replica of a complex has held since the start
+ * poses (or the last gd_reset_best), and that pose: best_e[n_rep],
+ * best_movematrix[n_rep * 6] (either may be NULL). No reference counterpart:
+ * the reference keeps every accepted state (src/kernel_cuda_l3_util.cu:73-108)
+ * and selects poses from those records afterwards (src/kgs.C); the engine
+ * also keeps the running minimum on the device so runs need not record. */
+gd_status gd_get_best(gd_ctx *ctx, int complex_id, float *best_e, float *best_movematrix);
+/* Best-so-far := each replica's current state. */
+gd_status gd_reset_best(gd_ctx *ctx);
 /* Lane shape chosen for a complex: lanes per replica and ligand atoms per
  * lane (the MC kernel instance that runs it). No reference counterpart: the
  * reference fixes its launch shape at compile time (src/Makefile:77-105). */
@@ -276,8 +285,9 @@ gd_status gd_sur_loss_grad(gd_ctx *ctx, int pose_begin, int pose_end, double *lo
 gd_status gd_sur_predict(gd_ctx *ctx, int pose_begin, int pose_end, float *out);
 /* Delayed-acceptance MC (surrogate transition, no counterpart in the
  * reference): each group of k_sub steps runs k_sub Metropolis steps on the
- * surrogate total S = sur_scale * MLP + the exact KDE, MCS and pocket-distance
- * terms, then accepts the composite move with min(1, exp(-beta [(E(y) - E(x)) -
+ * surrogate total S = sur_scale * (sum over ligand atoms of the MLP) / lna
+ * + the exact KDE, MCS and pocket-distance terms, then accepts the composite
+ * move with min(1, exp(-beta [(E(y) - E(x)) -
  * (S(y) - S(x))])) from one exact evaluation, so the chain keeps the exact
  * Boltzmann distribution. n_steps and steps_per_exchange are multiples of
  * k_sub; replica-steps count surrogate steps. */
