@@ -51,7 +51,9 @@ def parse():
     ap.add_argument("--log-steps", type=int, default=1000, help="plain steps logged for training")
     ap.add_argument("--log-poses", type=int, default=400000)
     ap.add_argument("--train-poses", type=int, default=200000)
-    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--sur-mode", default="tab", choices=["tab", "direct"],
+                    help="surrogate from tabulated network outputs or per-atom network")
     ap.add_argument("--batch-poses", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--grid-spacing", type=float, default=0.5)
@@ -174,6 +176,10 @@ def da_compare(args, cxs, desc):
     run.restore(blob)
     run.sur_build_grids(args.grid_spacing, args.grid_half)
     run.sur_set_params(params)
+    t0 = time.perf_counter()
+    if args.sur_mode == "tab":
+        run.sur_tabulate()
+    t_tab = time.perf_counter() - t0
 
     def best_energy():
         return float(min(run.best(c)[0].min() for c in range(len(cxs))))
@@ -205,11 +211,13 @@ def da_compare(args, cxs, desc):
         "ms_per_step": out["da"]["seconds"] / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded generator, geaxdock_gpu_amd/synth.py)",
-        "config": dict(desc, replicas=n_rep, steps_per_exchange=args.nc, k_sub=k),
+        "config": dict(desc, replicas=n_rep, steps_per_exchange=args.nc, k_sub=k,
+                       surrogate=args.sur_mode),
         "plain": out["plain"], "da": out["da"],
         "da_speedup": speed, "best_energy_delta": d_best,
         "target_met": bool(speed >= 3.0 and abs(d_best) <= 0.1),
         "surrogate": {"log_seconds": t_log, "label_seconds": t_data, "sigma": sigma,
+                      "tabulate_seconds": t_tab,
                       "train_poses": fit["n_train"], "test_poses": fit["n_test"],
                       "epochs": args.epochs, "train_seconds": fit["seconds"],
                       "train_loss": fit["train_loss"], "test_rmse_std": fit["test_rmse_std"],

@@ -67,6 +67,11 @@ def load_library():
     lib.gd_run_da.argtypes = [vp, C.c_int64, C.c_int, C.c_int, C.c_float, abi.RECORD_SINK, vp,
                               P(abi.RunStats)]
     lib.gd_run_da.restype = C.c_int
+    lib.gd_sur_tabulate.argtypes = [vp]
+    lib.gd_sur_tabulate.restype = C.c_int
+    lib.gd_sur_surrogate.argtypes = [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_int), P(C.c_float),
+                                     C.c_float, P(C.c_float)]
+    lib.gd_sur_surrogate.restype = C.c_int
     lib.gd_sur_num_params.argtypes = []
     lib.gd_sur_num_params.restype = C.c_int
     lib.gd_sur_build_grids.argtypes = [vp, C.c_float, C.c_float]
@@ -275,6 +280,20 @@ class Engine:
         out = np.zeros((len(mm), lna, 56), np.float32)
         self._check(self.lib.gd_sur_features(self._ctx, cid, len(mm), _i(il), _i(ip), _f(mm),
                                              _f(out)))
+        return out
+
+    def sur_tabulate(self):
+        """Tabulate the network per atom type on the surrogate grids."""
+        self._check(self.lib.gd_sur_tabulate(self._ctx))
+
+    def sur_surrogate(self, cid, idx_lig, idx_prt, movematrix, sur_scale):
+        """Surrogate totals of poses as the delayed-acceptance kernel sees them."""
+        il = np.ascontiguousarray(idx_lig, np.int32)
+        ip = np.ascontiguousarray(idx_prt, np.int32)
+        mm = np.ascontiguousarray(movematrix, np.float32).reshape(-1, 6)
+        out = np.zeros(len(mm), np.float32)
+        self._check(self.lib.gd_sur_surrogate(self._ctx, cid, len(mm), _i(il), _i(ip), _f(mm),
+                                              float(sur_scale), _f(out)))
         return out
 
     def sur_set_params(self, p):

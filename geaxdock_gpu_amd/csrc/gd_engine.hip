@@ -27,6 +27,7 @@ struct HostComplex {
   int n_rep = 0, rep_base = 0;
   int lpr = 0, apl = 0, n_chunks = 0;
   bool ref_built = false;
+  std::vector<int> lig_t;  // ligand atom types (host copy)
   DevComplex dev{};
   std::vector<void *> bufs;  // device allocations owned by this complex
   size_t lds_bytes = 0;
@@ -71,6 +72,8 @@ struct gd_ctx {
   struct Sur {
     std::vector<float *> grid;  // per complex [n_prt][n^3][32]
     std::vector<SurGrid> g;
+    std::vector<float *> tab;   // per complex [n_prt][NLT][n^3] network outputs
+    bool tab_valid = false;     // tab holds the current parameters' outputs
     float *p = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;  // FP32 master + Adam
     unsigned short *W1T = nullptr, *W2T = nullptr, *W2R = nullptr;    // bf16 layouts
     int64_t adam_t = 0;
@@ -221,6 +224,8 @@ void gd_destroy(gd_ctx *ctx) {
     auto &S = ctx->sur;
     for (float *g : S.grid)
       if (g) (void)hipFree(g);
+    for (float *g : S.tab)
+      if (g) (void)hipFree(g);
     void *qs[] = {S.p, S.m, S.v, S.grad, S.W1T, S.W2T, S.W2R, S.X, S.XT, S.pose_of_row,
                   S.d_off, S.d_target, S.y, S.dS, S.sq, S.S, S.A1T, S.dH2T, S.dH1T, S.p1,
                   S.p2, S.pb2, S.pw3};
@@ -298,6 +303,7 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
   const int cid = (int)ctx->cx.size() - 1;
   h.d = *d;
   h.n_rep = d->n_lig * d->n_prt * d->n_tmp;
+  h.lig_t.assign(d->lig_t, d->lig_t + d->lna);
   choose_shape(lna, d->n_tmp, ctx->opts.lanes_per_replica, ctx->opts.atoms_per_lane, &h.lpr,
                &h.apl);
   if (h.lpr * h.apl < lna || h.apl < 1 || h.apl > 4)
@@ -820,14 +826,16 @@ gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float
     P.sb2 = S.p + SP_B2;
     P.sw3 = S.p + SP_W3;
     P.sb3 = S.p + SP_B3;
+    P.sur_tab = S.tab_valid ? 1 : 0;
     P.off_da = (int)((lds + 15) & ~(size_t)15);
-    lds = P.off_da + (size_t)(block / 64) * 6208;
+    // the direct network evaluation needs per-wave LDS scratch
+    if (!P.sur_tab) lds = P.off_da + (size_t)(block / 64) * 6208;
     if (lds > ctx->max_lds && P.n_split > 1) {
       // the per-wave surrogate scratch of both lane groups does not fit:
       // run this launch with one group per pack (same packs and state)
       P.n_split = 1;
       block = P.half;
-      lds = P.off_da + (size_t)(block / 64) * 6208;
+      lds = P.off_da + (size_t)(block / 64) * 6208;  // only the direct mode gets here
     }
     if (lds > ctx->max_lds)
       return fail(ctx, GD_ERR_CAPACITY, "delayed acceptance needs %zu bytes of LDS", lds);
@@ -1036,11 +1044,31 @@ gd_status gd_restore(gd_ctx *ctx, const void *buf, size_t size) {
   return GD_OK;
 }
 
+static gd_status eval_poses(gd_ctx *ctx, int cid, int n, const int *idx_lig, const int *idx_prt,
+                            const float *movematrix, gd_energy *out, float *out_sur,
+                            float sur_scale);
+
 gd_status gd_calc_energy(gd_ctx *ctx, int cid, int n, const int *idx_lig, const int *idx_prt,
                          const float *movematrix, gd_energy *out) {
   if (!ctx || cid < 0 || cid >= (int)ctx->cx.size() || n < 0) return GD_ERR_INVALID;
   if (n == 0) return GD_OK;
   if (!idx_lig || !idx_prt || !movematrix || !out) return GD_ERR_INVALID;
+  return eval_poses(ctx, cid, n, idx_lig, idx_prt, movematrix, out, nullptr, 0.0f);
+}
+
+gd_status gd_sur_surrogate(gd_ctx *ctx, int cid, int n, const int *idx_lig, const int *idx_prt,
+                           const float *movematrix, float sur_scale, float *out) {
+  if (!ctx || cid < 0 || cid >= (int)ctx->cx.size() || n < 0) return GD_ERR_INVALID;
+  if (n == 0) return GD_OK;
+  if (!idx_lig || !idx_prt || !movematrix || !out) return GD_ERR_INVALID;
+  if (ctx->sur.g.size() != ctx->cx.size()) return fail(ctx, GD_ERR_STATE, "call gd_sur_build_grids first");
+  if (!ctx->sur.have_params) return fail(ctx, GD_ERR_STATE, "no surrogate parameters");
+  return eval_poses(ctx, cid, n, idx_lig, idx_prt, movematrix, nullptr, out, sur_scale);
+}
+
+static gd_status eval_poses(gd_ctx *ctx, int cid, int n, const int *idx_lig, const int *idx_prt,
+                            const float *movematrix, gd_energy *out, float *out_sur,
+                            float sur_scale) {
   HostComplex &h = ctx->cx[cid];
   for (int i = 0; i < n; ++i)
     if (idx_lig[i] < 0 || idx_lig[i] >= h.d.n_lig || idx_prt[i] < 0 || idx_prt[i] >= h.d.n_prt)
@@ -1073,15 +1101,31 @@ gd_status gd_calc_energy(gd_ctx *ctx, int cid, int n, const int *idx_lig, const 
   if (ps.block > 256) ns = 1;
   layout(P, &lds, ps.pnp_max, ps.pnk_max, ps.nch_max, ps.slots_max, ns, ps.block);
   if (lds > ctx->max_lds) return fail(ctx, GD_ERR_CAPACITY, "LDS %zu > %zu", lds, ctx->max_lds);
+  const bool sur = out_sur != nullptr;
+  if (sur) {
+    auto &S = ctx->sur;
+    P.sur_scale = sur_scale;
+    P.sur_tab = S.tab_valid ? 1 : 0;
+    P.sW1T = S.W1T;
+    P.sW2T = S.W2T;
+    P.sb2 = S.p + SP_B2;
+    P.sw3 = S.p + SP_W3;
+    P.sb3 = S.p + SP_B3;
+    P.off_da = (int)((lds + 15) & ~(size_t)15);
+    if (!P.sur_tab) lds = P.off_da + (size_t)(ps.block * ns / 64) * 6208;
+    if (lds > ctx->max_lds) return fail(ctx, GD_ERR_CAPACITY, "LDS %zu > %zu", lds, ctx->max_lds);
+  }
   Pack *d_packs = nullptr;
   int *d_slots = nullptr, *d_lig = nullptr;
   float *d_mm = nullptr;
   gd_energy *d_out = nullptr;
+  float *d_sur = nullptr;
   HIPCHK(hipMalloc(&d_packs, ps.packs.size() * sizeof(Pack)));
   HIPCHK(hipMalloc(&d_slots, ps.slots.size() * sizeof(int)));
   HIPCHK(hipMalloc(&d_lig, n * sizeof(int)));
   HIPCHK(hipMalloc(&d_mm, (size_t)n * 6 * sizeof(float)));
   HIPCHK(hipMalloc(&d_out, (size_t)n * sizeof(gd_energy)));
+  HIPCHK(hipMalloc(&d_sur, (size_t)n * sizeof(float)));
   HIPCHK(hipMemcpyAsync(d_packs, ps.packs.data(), ps.packs.size() * sizeof(Pack), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(d_slots, ps.slots.data(), ps.slots.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(d_lig, idx_lig, n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
@@ -1092,12 +1136,17 @@ gd_status gd_calc_energy(gd_ctx *ctx, int cid, int n, const int *idx_lig, const 
   P.ev_mm = d_mm;
   P.ev_lig = d_lig;
   P.ev_out = d_out;
-  P.mode = MODE_EVAL;
-  if (ps.block * ns > 512) return fail(ctx, GD_ERR_CAPACITY, "pose pack of %d lanes (> 512)", ps.block);
-  hipError_t le = launch_classes(ctx, ps.cls, P, false, ps.block * ns, lds, ctx->stream);
-  if (le == hipSuccess)
+  P.ev_sur = d_sur;
+  P.mode = sur ? MODE_SUREVAL : MODE_EVAL;
+  hipError_t le = hipSuccess;
+  if (ps.block * ns > 512) le = hipErrorInvalidValue;
+  if (le == hipSuccess) le = launch_classes(ctx, ps.cls, P, sur, ps.block * ns, lds, ctx->stream);
+  if (le == hipSuccess && out)
     le = hipMemcpyAsync(out, d_out, (size_t)n * sizeof(gd_energy), hipMemcpyDeviceToHost, ctx->stream);
+  if (le == hipSuccess && sur)
+    le = hipMemcpyAsync(out_sur, d_sur, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
   if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d_sur);
   (void)hipFree(d_packs);
   (void)hipFree(d_slots);
   (void)hipFree(d_lig);
@@ -1359,6 +1408,10 @@ gd_status gd_sur_build_grids(gd_ctx *ctx, float spacing, float half_width) {
   const size_t node_bytes = 32 * sizeof(float);
   for (float *g : S.grid)
     if (g) (void)hipFree(g);
+  for (float *g : S.tab)
+    if (g) (void)hipFree(g);
+  S.tab.assign(ctx->cx.size(), nullptr);
+  S.tab_valid = false;
   S.grid.assign(ctx->cx.size(), nullptr);
   S.g.assign(ctx->cx.size(), SurGrid{});
   for (size_t c = 0; c < ctx->cx.size(); ++c) {
@@ -1379,6 +1432,7 @@ gd_status gd_sur_build_grids(gd_ctx *ctx, float spacing, float half_width) {
     ctx->cx[c].dev.soz = G.oz;
     ctx->cx[c].dev.sh = G.h;
     ctx->cx[c].dev.sn = G.n;
+    ctx->cx[c].dev.stab = nullptr;
     ctx->cx_dirty = true;
     for (int p = 0; p < h.d.n_prt; ++p) {
       hipLaunchKernelGGL(gd_sur_grid_kernel, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0,
@@ -1433,9 +1487,63 @@ gd_status gd_sur_set_params(gd_ctx *ctx, const float *params) {
   HIPCHK(hipMemsetAsync(S.m, 0, SUR_NPARAM * sizeof(float), ctx->stream));
   HIPCHK(hipMemsetAsync(S.v, 0, SUR_NPARAM * sizeof(float), ctx->stream));
   S.adam_t = 0;
+  S.tab_valid = false;
   TRY(sur_refresh(ctx));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   S.have_params = true;
+  return GD_OK;
+}
+
+gd_status gd_sur_tabulate(gd_ctx *ctx) {
+  if (!ctx) return GD_ERR_INVALID;
+  auto &S = ctx->sur;
+  if (S.g.size() != ctx->cx.size()) return fail(ctx, GD_ERR_STATE, "call gd_sur_build_grids first");
+  if (!S.have_params) return fail(ctx, GD_ERR_STATE, "no surrogate parameters");
+  HIPCHK(hipSetDevice(ctx->device));
+  SurW W{S.W1T, S.W2T, S.W2R, S.p + SP_B2, S.p + SP_W3, S.p + SP_B3};
+  int *d_types = nullptr;
+  HIPCHK(hipMalloc(&d_types, NLT * sizeof(int)));
+  gd_status st = GD_OK;
+  for (size_t c = 0; c < ctx->cx.size() && st == GD_OK; ++c) {
+    HostComplex &h = ctx->cx[c];
+    const int n = S.g[c].n;
+    const size_t n3 = (size_t)n * n * n;
+    if (!S.tab[c]) {
+      if (hipMalloc(&S.tab[c], (size_t)h.d.n_prt * NLT * n3 * sizeof(float)) != hipSuccess) {
+        S.tab[c] = nullptr;
+        st = fail(ctx, GD_ERR_NOMEM, "surrogate table of complex %zu", c);
+        break;
+      }
+    }
+    // only the ligand's atom types are evaluated
+    std::vector<int> types;
+    for (int t = 0; t < NLT; ++t)
+      for (int l = 0; l < h.d.lna; ++l)
+        if (h.lig_t[l] == t) {
+          types.push_back(t);
+          break;
+        }
+    if (hipMemcpy(d_types, types.data(), types.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+      st = fail(ctx, GD_ERR_HIP, "type list upload");
+      break;
+    }
+    const int64_t jobs = (int64_t)((n3 + 15) / 16) * (int64_t)types.size();
+    for (int p = 0; p < h.d.n_prt; ++p) {
+      hipLaunchKernelGGL(gd_sur_tab_kernel, dim3((unsigned)((jobs + 3) / 4)), dim3(256), 0,
+                         ctx->stream, S.grid[c] + (size_t)p * n3 * 32, (int)n3, d_types,
+                         (int)types.size(), W, S.tab[c] + (size_t)p * NLT * n3);
+    }
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      st = fail(ctx, GD_ERR_HIP, "tabulation kernel");
+      break;
+    }
+    h.dev.stab = S.tab[c];
+    ctx->cx_dirty = true;
+  }
+  (void)hipFree(d_types);
+  TRY(st);
+  TRY(sync_complexes(ctx));
+  S.tab_valid = true;
   return GD_OK;
 }
 
@@ -1581,6 +1689,7 @@ gd_status gd_sur_train(gd_ctx *ctx, int pose_begin, int pose_end, int epochs, in
   TRY(sur_work(ctx, maxrows, (int64_t)S.target.size()));
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   const auto t0 = std::chrono::steady_clock::now();
+  S.tab_valid = false;  // the parameters change
   for (int ep = 0; ep < epochs; ++ep) {
     for (int p = pose_begin; p < pose_end; p += batch_poses) {
       const int q = std::min(pose_end, p + batch_poses);

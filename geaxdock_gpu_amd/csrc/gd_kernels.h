@@ -56,6 +56,9 @@ struct DevComplex {
   const float *sgrid;
   float sox, soy, soz, sh;
   int sn;
+  // network output tabulated on the same grid per ligand atom type:
+  // stab[n_prt][NLT][sn^3] (null until tabulated)
+  const float *stab;
 };
 
 // One workgroup's work: replicas (or poses) of one complex that share one
@@ -65,7 +68,8 @@ struct Pack {
   int cx, prt, n_slots, n_groups, slot_off;
 };
 
-enum { MODE_MC = 0, MODE_INIT = 1, MODE_EVAL = 2 };
+// MODE_SUREVAL: surrogate total of given poses (delayed-acceptance kernels)
+enum { MODE_MC = 0, MODE_INIT = 1, MODE_EVAL = 2, MODE_SUREVAL = 3 };
 
 struct KParams {
   const DevComplex *cxs;
@@ -97,7 +101,9 @@ struct KParams {
   // same replicas over their share of every receptor chunk and exchange
   // partial sums through LDS at off_xp (two parity buffers of xp_floats)
   int n_split, half, off_xp, xp_floats, xp_slots;
-  int off_da;  // per-wave surrogate scratch (MODE_MC with da_k > 0)
+  int off_da;  // per-wave surrogate scratch (direct network evaluation)
+  int sur_tab;  // 1: surrogate from the tabulated network outputs
+  float *ev_sur;  // MODE_SUREVAL output, indexed by slot value
   // delayed acceptance: k surrogate sub-steps per exact evaluation (0 = plain)
   int da_k;
   float sur_scale;

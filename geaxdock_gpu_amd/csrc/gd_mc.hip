@@ -493,6 +493,36 @@ __device__ __forceinline__ float sur_total(const KParams &P, const DevComplex &C
                                            const float (&z)[APL], const int (&lt)[APL],
                                            const int (&li)[APL], const bool (&val)[APL],
                                            const float mm[6]) {
+  if (P.sur_tab) {
+    // tabulated network: trilinear interpolation of the atom type's output
+    // grid (same grid geometry and clamping as the feature grid)
+    const int n = C.sn;
+    const size_t n3 = (size_t)n * n * n;
+    const float lim = (float)(n - 1) - 1e-4f;
+    const float ih = 1.0f / C.sh;
+    float ys = 0.0f;
+#pragma unroll
+    for (int k = 0; k < APL; ++k) {
+      const float ux = fminf(fmaxf((x[k] - C.sox) * ih, 0.0f), lim);
+      const float uy = fminf(fmaxf((y[k] - C.soy) * ih, 0.0f), lim);
+      const float uz = fminf(fmaxf((z[k] - C.soz) * ih, 0.0f), lim);
+      const int ix = (int)ux, iy = (int)uy, iz = (int)uz;
+      const float tx = ux - ix, ty = uy - iy, tz = uz - iz;
+      const float *g = C.stab + ((size_t)prt * NLT + lt[k]) * n3 + ((size_t)ix * n + iy) * n + iz;
+      const size_t sx = (size_t)n * n, sy = n;
+      const float c00 = fmaf(tz, g[1] - g[0], g[0]);
+      const float c01 = fmaf(tz, g[sy + 1] - g[sy], g[sy]);
+      const float c10 = fmaf(tz, g[sx + 1] - g[sx], g[sx]);
+      const float c11 = fmaf(tz, g[sx + sy + 1] - g[sx + sy], g[sx + sy]);
+      const float c0 = fmaf(ty, c01 - c00, c00);
+      const float c1 = fmaf(ty, c11 - c10, c10);
+      const float v = fmaf(tx, c1 - c0, c0);
+      ys += val[k] ? v : 0.0f;
+    }
+    ys = seg_sum(ys, lpr);
+    return P.sur_scale * (ys / (float)C.lna) +
+           cheap_terms<APL>(C, S, lpr, x, y, z, lt, li, val, mm);
+  }
   const int l = threadIdx.x & 63;
   unsigned short *Xs = reinterpret_cast<unsigned short *>(da);
   unsigned short *A1s = Xs + 16 * SUR_F;
@@ -605,7 +635,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   int flat = 0, ilig;
   float mo[6];
   gd_energy eo;
-  if (P.mode == MODE_EVAL) {
+  if (P.mode == MODE_EVAL || P.mode == MODE_SUREVAL) {
     ilig = P.ev_lig[sv];
 #pragma unroll
     for (int i = 0; i < 6; ++i) mo[i] = P.ev_mm[(size_t)sv * 6 + i];
@@ -646,6 +676,16 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
     }
   };
 
+  if constexpr (DA) {
+    if (P.mode == MODE_SUREVAL) {
+      extern __shared__ __attribute__((aligned(16))) unsigned char lds_se[];
+      unsigned char *dse = lds_se + P.off_da + (threadIdx.x >> 6) * 6208;
+      place(mo);
+      const float sv_s = sur_total<APL>(P, C, S, dse, lpr, pk.prt, x, y, z, lt, li, val, mo);
+      if (active && lead) P.ev_sur[sv] = sv_s;
+      return;
+    }
+  }
   if (P.mode != MODE_MC) {
     place(mo);
     gd_energy E;

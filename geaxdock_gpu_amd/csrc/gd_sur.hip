@@ -261,6 +261,54 @@ extern "C" __global__ void __launch_bounds__(256) gd_mlp_fwd_kernel(const unsign
     for (int i = 0; i < 4; ++i) y[r0 + 4 * (l >> 4) + i] = yy[i];
 }
 
+// Network output at every grid point for each listed ligand atom type:
+// Y[type][point] = MLP(onehot(type), features(point)), the features being
+// the grid's own values (no interpolation). One wave per 16 points of one
+// type; 4 waves per workgroup.
+extern "C" __global__ void __launch_bounds__(256) gd_sur_tab_kernel(const float *fgrid,
+                                                                    int n_pts,
+                                                                    const int *types,
+                                                                    int n_types, SurW W,
+                                                                    float *Y) {
+  __shared__ unsigned short xs[4][16 * SUR_F];
+  __shared__ unsigned short a1s[4][16 * SUR_H];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int tiles = (n_pts + 15) / 16;
+  const int64_t job = (int64_t)blockIdx.x * 4 + wv;
+  if (job >= (int64_t)tiles * n_types) return;  // whole wave leaves; no block barrier
+  const int ti = (int)(job / tiles);
+  const int p0 = (int)(job % tiles) * 16;
+  const int type = types[ti];
+  // lane l fills columns 16 (l % 4) .. +15 of row l / 4
+  {
+    const int r = l >> 2, c0 = 16 * (l & 3);
+    const int pt = min(p0 + r, n_pts - 1);
+    const float *f = fgrid + (size_t)pt * 32;
+    unsigned short *row = xs[wv] + r * SUR_F;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = c0 + q;
+      float v = 0.0f;
+      if (c < NLT) v = (c == type) ? 1.0f : 0.0f;
+      else if (c < NLT + 32) v = f[c - NLT];
+      else if (c == SUR_BIAS_COL) v = 1.0f;
+      row[c] = f2bf(v);
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  __builtin_amdgcn_wave_barrier();
+  f32x4_t h1[8], h2[8];
+  mlp_fwd16(xs[wv], 0, W, a1s[wv], h1, h2);
+  float yy[4];
+  mlp_out16(h2, W, yy);
+  if ((l & 15) == 0)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int pt = p0 + 4 * (l >> 4) + i;
+      if (pt < n_pts) Y[(size_t)type * n_pts + pt] = yy[i];
+    }
+}
+
 // Pose sums over poses [p0, p1): S = sum of row outputs, residual against the
 // target, dS = 2 (S - target) / (p1 - p0), squared residuals for the loss.
 // Row offsets are absolute; y is indexed by absolute row.

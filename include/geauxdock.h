@@ -293,6 +293,18 @@ gd_status gd_sur_predict(gd_ctx *ctx, int pose_begin, int pose_end, float *out);
  * k_sub; replica-steps count surrogate steps. */
 gd_status gd_run_da(gd_ctx *ctx, int64_t n_steps, int steps_per_exchange, int k_sub,
                     float sur_scale, gd_record_sink sink, void *user, gd_run_stats *stats);
+/* Tabulate the network on every complex's surrogate grid: for each ligand
+ * atom type, the output at each grid point (features = the grid's values),
+ * evaluated with the BF16 MFMA forward kernel. Until the parameters change,
+ * gd_run_da and gd_sur_surrogate then read the surrogate by trilinear
+ * interpolation of these tables (8 loads per atom) instead of evaluating
+ * the network per atom. No reference counterpart. */
+gd_status gd_sur_tabulate(gd_ctx *ctx);
+/* Surrogate total S of n poses exactly as the delayed-acceptance kernel
+ * computes it (tabulated or direct), out[n]. No reference counterpart. */
+gd_status gd_sur_surrogate(gd_ctx *ctx, int complex_id, int n, const int *idx_lig,
+                           const int *idx_prt, const float *movematrix, float sur_scale,
+                           float *out);
 /* Adam over contiguous batches of batch_poses poses; epoch_loss[epochs]. */
 gd_status gd_sur_train(gd_ctx *ctx, int pose_begin, int pose_end, int epochs, int batch_poses,
                        float lr, double *epoch_loss, double *seconds);

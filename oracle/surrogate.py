@@ -129,3 +129,21 @@ def loss_and_grads(params, F, pose_of_atom, target):
     g["W1"] = F.T @ dH1
     g["b1"] = dH1.sum(0)
     return loss, g
+
+
+def rows(params, F):
+    """Per-row network outputs y for feature rows F [N, 56]."""
+    S, _ = forward(params, F, np.arange(len(F)), len(F))
+    return S
+
+
+def tabulate(params, grid, types):
+    """Network output at every grid node (features = the node's values) for
+    each ligand atom type: {type: [n, n, n]}."""
+    n = grid.shape[0]
+    radial = grid.reshape(-1, grid.shape[-1])
+    out = {}
+    for t in types:
+        F = features(np.full(len(radial), int(t)), radial)
+        out[int(t)] = rows(params, F).reshape(n, n, n)
+    return out

@@ -27,11 +27,13 @@ def gd():
     return g
 
 
-def _engine(gd, cx, seed, params_seed=3, **kw):
+def _engine(gd, cx, seed, params_seed=3, tab=False, **kw):
     eng = gd.Engine(seed=seed, **kw)
     cid = eng.add_complex(cx)
     eng.sur_build_grids(0.8, 12.0)
     eng.sur_set_params(sg.pack_params(sg.init_params(params_seed)))
+    if tab:
+        eng.sur_tabulate()
     return eng, cid
 
 
@@ -66,10 +68,11 @@ def test_da_scale0_tracks_cpu_path(gd):
     assert np.all((recs["step"][recs["step"] > 0] + 1) % k == 0)
 
 
-def test_da_states_carry_exact_energy(gd):
+@pytest.mark.parametrize("tab", [False, True])
+def test_da_states_carry_exact_energy(gd, tab):
     cx = synth.config0(seed=1)
     recs = []
-    eng, cid = _engine(gd, cx, 5)
+    eng, cid = _engine(gd, cx, 5, tab=tab)
     eng.run_da(240, 20, 4, 3.0, sink=lambda c, r: recs.append(r))
     state = eng.state(cid)
     recs = np.concatenate(recs)
@@ -85,11 +88,12 @@ def test_da_states_carry_exact_energy(gd):
     eng.close()
 
 
-def test_da_deterministic_and_window_independent(gd):
+@pytest.mark.parametrize("tab", [False, True])
+def test_da_deterministic_and_window_independent(gd, tab):
     cx = synth.config0(seed=2)
     out = []
     for dump in (40, 2000):
-        eng, cid = _engine(gd, cx, 9, steps_per_dump=dump)
+        eng, cid = _engine(gd, cx, 9, tab=tab, steps_per_dump=dump)
         eng.run_da(200, 20, 4, 2.0)
         out.append(eng.state(cid))
         eng.close()
@@ -123,7 +127,7 @@ def test_da_keeps_boltzmann_mean_energy(gd):
     k = 4
     means = []
     for da in (False, True):
-        eng, cid = _engine(gd, cx, 21, record=False)
+        eng, cid = _engine(gd, cx, 21, tab=True, record=False)
         run = (lambda n: eng.run_da(n, 0, k, 1.0)) if da else (lambda n: eng.run(n, 0))
         run(3000)
         acc = []
@@ -141,3 +145,54 @@ def test_da_keeps_boltzmann_mean_energy(gd):
         se = dlt.std(ddof=1) / np.sqrt(len(dlt))
         print(f"T{t}: plain {means[0][t].mean():.4f} DA {means[1][t].mean():.4f} se {se:.4f}")
         assert abs(dlt.mean()) <= 5 * se + 1e-3, (t, dlt.mean(), se)
+
+
+@pytest.mark.parametrize("tab", [False, True])
+def test_surrogate_total_matches_oracle(gd, tab):
+    """S = scale * sum_atoms y / lna + weighted exact KDE, MCS and pocket
+    distance, as the delayed-acceptance kernel computes it, against the FP64
+    network (oracle/surrogate.py) on the FP64 feature grid. Direct mode
+    interpolates the features and evaluates the network per atom; tabulated
+    mode evaluates the network at the grid nodes and interpolates its
+    output. BF16 operands bound the agreement: |dS| <= 2e-2 * scale *
+    sum|y| / lna + 1e-4 (1 + |cheap|)."""
+    from oracle import gdo
+    from oracle import surrogate as so
+    cx = synth.config0(seed=2, void_radius=7.0)
+    spacing, half, scale = 0.8, 9.6, 7.0
+    p = so.init_params(3)
+    with gd.Engine(seed=1) as eng:
+        cid = eng.add_complex(cx)
+        eng.sur_build_grids(spacing, half)
+        eng.sur_set_params(sg.pack_params(p))
+        if tab:
+            eng.sur_tabulate()
+        rng = np.random.default_rng(4)
+        n_pose = 16
+        mm = np.zeros((n_pose, 6), np.float32)
+        mm[:, :3] = rng.normal(0, 1.5, (n_pose, 3))
+        mm[:, 3:] = rng.uniform(-np.pi, np.pi, (n_pose, 3))
+        got = eng.sur_surrogate(cid, np.zeros(n_pose), np.zeros(n_pose), mm, scale)
+    n = int(np.ceil(2 * half / spacing)) + 1
+    origin = np.float64(cx.pocket_center) - half
+    prt = np.stack([cx.prt_x, cx.prt_y, cx.prt_z], 1)[:cx.pnp]
+    grid = so.grid_features(origin, spacing, n, prt, cx.prt_t)
+    tabs = so.tabulate(p, grid, np.unique(cx.lig_t)) if tab else None
+    d = cx.desc()
+    ref = gdo.refmap_f64(d, cx)
+    w = np.asarray(cx.enepara_struct.w, np.float64)
+    worst = 0.0
+    for i in range(n_pose):
+        xyz = gdo.place_f64(d, cx, 0, mm[i]).T
+        if tab:
+            y = np.array([so.interp(tabs[int(t)][..., None], origin, spacing, xyz[k:k + 1])[0, 0]
+                          for k, t in enumerate(cx.lig_t)])
+        else:
+            y = so.rows(p, so.features(cx.lig_t, so.interp(grid, origin, spacing, xyz)))
+        o = gdo.energy_f64(d, ref, 0, 0, mm[i])
+        cheap = sum(w[j] * o.e[j] for j in (6, 7, 8))
+        want = scale * y.sum() / cx.lna + cheap
+        tol = 2e-2 * scale * np.abs(y).sum() / cx.lna + 1e-4 * (1 + abs(cheap))
+        worst = max(worst, abs(got[i] - want) / tol)
+        assert abs(got[i] - want) <= tol, (i, got[i], want, tol)
+    print(f"\ntab={tab}: worst |dS|/tol = {worst:.3f}")
