@@ -21,6 +21,7 @@ bounded sample of the same workload, on this host's cores.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -181,25 +182,34 @@ def da_compare(args, cxs, desc):
         run.sur_tabulate()
     t_tab = time.perf_counter() - t0
 
-    def best_energy():
-        return float(min(run.best(c)[0].min() for c in range(len(cxs))))
+    def best_stats():
+        b = np.concatenate([run.best(c)[0] for c in range(len(cxs))]).astype(np.float64)
+        return {"best_energy": float(b.min()), "best_p01": float(np.percentile(b, 1)),
+                "replica_best_mean": float(b.mean())}
 
-    out = {}
-    for mode in ("plain", "da"):
+    def one(mode, steps):
         run.restore(blob)
         run.reset_best()
         t0 = time.perf_counter()
         if mode == "plain":
-            st = run.run(args.steps, args.nc)
+            st = run.run(steps, args.nc)
         else:
-            st = run.run_da(args.steps, args.nc, k, sigma)
+            st = run.run_da(steps, args.nc, k, sigma)
         dt = time.perf_counter() - t0
-        out[mode] = {"replica_steps_per_s": n_rep * args.steps / dt,
-                     "kernel_replica_steps_per_s": n_rep * args.steps / st["t_compute"],
-                     "best_energy": best_energy(),
-                     "accepted_per_exact_eval": st["accepted"] / (st["replica_steps"] / (k if mode == "da" else 1)),
-                     "accepted_per_step": st["accepted"] / st["replica_steps"],
-                     "kernel_ms_avg": st["kernel_ms_avg"], "seconds": dt}
+        r = {"steps": steps, "replica_steps_per_s": n_rep * steps / dt,
+             "kernel_replica_steps_per_s": n_rep * steps / st["t_compute"],
+             "accepted_per_exact_eval": st["accepted"] / (st["replica_steps"] / (k if mode == "da" else 1)),
+             "accepted_per_step": st["accepted"] / st["replica_steps"],
+             "kernel_ms_avg": st["kernel_ms_avg"], "seconds": dt}
+        r.update(best_stats())
+        return r
+
+    out = {"plain": one("plain", args.steps), "da": one("da", args.steps)}
+    # the same wall-clock budget as the plain run
+    eq = int(args.steps * out["da"]["replica_steps_per_s"] / out["plain"]["replica_steps_per_s"])
+    step = math.lcm(k, args.nc) if args.nc > 0 else k
+    eq = max(step, eq // step * step)
+    out["da_equal_time"] = one("da", eq)
     run.close()
     speed = out["da"]["replica_steps_per_s"] / out["plain"]["replica_steps_per_s"]
     d_best = out["da"]["best_energy"] - out["plain"]["best_energy"]
@@ -213,7 +223,7 @@ def da_compare(args, cxs, desc):
         "data": "synthetic (seeded generator, geaxdock_gpu_amd/synth.py)",
         "config": dict(desc, replicas=n_rep, steps_per_exchange=args.nc, k_sub=k,
                        surrogate=args.sur_mode),
-        "plain": out["plain"], "da": out["da"],
+        "plain": out["plain"], "da": out["da"], "da_equal_time": out["da_equal_time"],
         "da_speedup": speed, "best_energy_delta": d_best,
         "target_met": bool(speed >= 3.0 and abs(d_best) <= 0.1),
         "surrogate": {"log_seconds": t_log, "label_seconds": t_data, "sigma": sigma,
