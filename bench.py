@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--lpr", type=int, default=0)
     ap.add_argument("--apl", type=int, default=0)
     ap.add_argument("--split", type=int, default=0, help="receptor split (0 = auto, 1, 2)")
+    ap.add_argument("--ladders", type=int, default=0, help="temperature ladders per workgroup")
+    ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--mode", default="plain", choices=["plain", "da"],
                     help="plain engine only, or plain vs surrogate delayed acceptance")
     ap.add_argument("--k-sub", type=int, default=4, help="surrogate steps per exact evaluation")
@@ -254,7 +256,8 @@ def main():
         print(json.dumps(da_compare(args, cxs, desc)))
         return
     eng = gd.Engine(seed=0, record=True, steps_per_dump=2000, lanes_per_replica=args.lpr,
-                    atoms_per_lane=args.apl, receptor_split=args.split)
+                    atoms_per_lane=args.apl, receptor_split=args.split,
+                    ladders_per_pack=args.ladders, verbose=args.verbose)
     for cx in cxs:
         eng.add_complex(cx)
     n_rep = sum(c.n_rep for c in cxs)

@@ -22,6 +22,15 @@ using namespace gd;
 
 namespace {
 
+// One kernel launch of a pack set: packs [begin, end) share the atoms-per-lane
+// kernel instance, the LDS layout and the block shape.
+struct LaunchClass {
+  int apl = 1, begin = 0, end = 0;
+  int block = 64;  // lanes: n_split groups of `half`
+  size_t lds = 0;  // dynamic LDS bytes
+  KParams lay{};   // LDS offsets and receptor-split fields
+};
+
 struct HostComplex {
   gd_complex_desc d;  // sizes, scalars (pointers are not kept)
   int n_rep = 0, rep_base = 0;
@@ -64,10 +73,8 @@ struct gd_ctx {
   // packs of the MC launch
   Pack *d_packs = nullptr;
   int *d_slots = nullptr;
-  int n_packs = 0, block = 0, n_split = 1;
-  int cls[5] = {0, 0, 0, 0, 0};  // packs of APL class a are [cls[a-1], cls[a])
-  size_t lds = 0;
-  KParams lay{};  // LDS layout of the MC launch
+  int n_packs = 0, n_split = 1, ladders_per_pack = 0;
+  std::vector<LaunchClass> mc_classes;  // kernel launches of the MC pack set
   // learned surrogate
   struct Sur {
     std::vector<float *> grid;  // per complex [n_prt][n^3][32]
@@ -574,53 +581,95 @@ void layout(KParams &P, size_t *bytes, int pnp_max, int pnk_max, int nch_max, in
 struct PackSet {
   std::vector<Pack> packs;
   std::vector<int> slots;
-  int block = 64, pnp_max = 0, pnk_max = 0, nch_max = 0, slots_max = 1;
-  int cls[5] = {0, 0, 0, 0, 0};  // packs of APL class a are [cls[a-1], cls[a])
+  std::vector<LaunchClass> classes;
 };
 
-void pack_finish(const gd_ctx *ctx, PackSet &ps) {
-  // grouped by APL class (one kernel instance each), longest packs first
-  // within a class so the hardware dispatcher fills the CUs in LPT order
-  std::vector<int> order(ps.packs.size());
-  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-  auto cost = [&](const Pack &p) {
+// Copy the LDS layout and split fields of a launch class into P.
+void apply_layout(KParams &P, const KParams &L) {
+  P.off_hpp = L.off_hpp;
+  P.off_psp = L.off_psp;
+  P.off_par = L.off_par;
+  P.off_kde = L.off_kde;
+  P.off_kr = L.off_kr;
+  P.off_ch = L.off_ch;
+  P.off_xe = L.off_xe;
+  P.off_xi = L.off_xi;
+  P.off_xa = L.off_xa;
+  P.off_xo = L.off_xo;
+  P.n_split = L.n_split;
+  P.half = L.half;
+  P.off_xp = L.off_xp;
+  P.xp_floats = L.xp_floats;
+  P.xp_slots = L.xp_slots;
+}
+
+// Group packs into launch classes by (atoms per lane, workgroups per CU that
+// the pack's LDS allows), each class longest packs first so the hardware
+// dispatcher fills the CUs in LPT order. A class's LDS is the largest need
+// among its packs, so small complexes keep several workgroups per CU.
+void pack_finish(const gd_ctx *ctx, PackSet &ps, int ns) {
+  const size_t n_pk = ps.packs.size();
+  std::vector<int> res(n_pk), half(n_pk);
+  for (size_t i = 0; i < n_pk; ++i) {
+    const Pack &p = ps.packs[i];
     const HostComplex &h = ctx->cx[p.cx];
-    return (double)h.d.pnp * (double)((p.n_slots * h.lpr + 63) / 64) * h.apl;
+    half[i] = ((p.n_slots * h.lpr + 63) / 64) * 64;
+    KParams L{};
+    size_t need = 0;
+    layout(L, &need, (h.d.pnp + 3) & ~3, h.d.pnk, h.n_chunks, p.n_slots, ns, half[i]);
+    res[i] = (int)std::max<size_t>(1, std::min<size_t>(8, ctx->max_lds / std::max<size_t>(need, 1)));
+  }
+  std::vector<int> order(n_pk);
+  for (size_t i = 0; i < n_pk; ++i) order[i] = (int)i;
+  auto cost = [&](int i) {
+    const Pack &p = ps.packs[i];
+    const HostComplex &h = ctx->cx[p.cx];
+    return (double)h.d.pnp * (double)half[i] * h.apl;
   };
-  auto apl = [&](const Pack &p) { return ctx->cx[p.cx].apl; };
+  auto apl = [&](int i) { return ctx->cx[ps.packs[i].cx].apl; };
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    const Pack &pa = ps.packs[a], &pb = ps.packs[b];
-    if (apl(pa) != apl(pb)) return apl(pa) < apl(pb);
-    return cost(pa) > cost(pb);
+    if (apl(a) != apl(b)) return apl(a) < apl(b);
+    if (res[a] != res[b]) return res[a] < res[b];
+    return cost(a) > cost(b);
   });
   std::vector<Pack> sorted;
   for (int i : order) sorted.push_back(ps.packs[i]);
   ps.packs.swap(sorted);
-  for (int a = 1; a <= 4; ++a) {
-    int e = ps.cls[a - 1];
-    while (e < (int)ps.packs.size() && apl(ps.packs[e]) == a) ++e;
-    ps.cls[a] = e;
+  ps.classes.clear();
+  for (size_t b = 0; b < n_pk;) {
+    size_t e = b;
+    const int a0 = apl(order[b]), r0 = res[order[b]];
+    LaunchClass lc;
+    lc.apl = a0;
+    lc.begin = (int)b;
+    int pnp_max = 0, pnk_max = 0, nch_max = 0, slots_max = 1, half_max = 64;
+    while (e < n_pk && apl(order[e]) == a0 && res[order[e]] == r0) {
+      const Pack &p = ps.packs[e];
+      const HostComplex &h = ctx->cx[p.cx];
+      pnp_max = std::max(pnp_max, h.d.pnp);
+      pnk_max = std::max(pnk_max, h.d.pnk);
+      nch_max = std::max(nch_max, h.n_chunks);
+      slots_max = std::max(slots_max, p.n_slots);
+      half_max = std::max(half_max, half[order[e]]);
+      ++e;
+    }
+    lc.end = (int)e;
+    lc.block = half_max * ns;
+    layout(lc.lay, &lc.lds, (pnp_max + 3) & ~3, pnk_max, nch_max, slots_max, ns, half_max);
+    ps.classes.push_back(lc);
+    b = e;
   }
-  for (const Pack &p : ps.packs) {
-    const HostComplex &h = ctx->cx[p.cx];
-    ps.block = std::max(ps.block, ((p.n_slots * h.lpr + 63) / 64) * 64);
-    ps.pnp_max = std::max(ps.pnp_max, h.d.pnp);
-    ps.pnk_max = std::max(ps.pnk_max, h.d.pnk);
-    ps.nch_max = std::max(ps.nch_max, h.n_chunks);
-    ps.slots_max = std::max(ps.slots_max, p.n_slots);
-  }
-  ps.pnp_max = (ps.pnp_max + 3) & ~3;
 }
 
 // MC packs: whole ladders of one (complex, protein conformer), about 256
-// lanes per workgroup and at most max_lanes.
-PackSet build_mc_packs(const gd_ctx *ctx, int max_lanes) {
+// lanes per workgroup and at most max_lanes, or `ladders` ladders when set.
+PackSet build_mc_packs(const gd_ctx *ctx, int max_lanes, int ns, int ladders) {
   PackSet ps;
   for (int c = 0; c < (int)ctx->cx.size(); ++c) {
     const HostComplex &h = ctx->cx[c];
     const int n_tmp = h.d.n_tmp, n_lig = h.d.n_lig;
     const int glanes = n_tmp * h.lpr;
-    int G = std::max(1, 256 / glanes);
+    int G = ladders > 0 ? ladders : std::max(1, 256 / glanes);
     while (G > 1 && G * glanes > max_lanes) --G;
     for (int p = 0; p < h.d.n_prt; ++p)
       for (int l0 = 0; l0 < n_lig; l0 += G) {
@@ -637,7 +686,7 @@ PackSet build_mc_packs(const gd_ctx *ctx, int max_lanes) {
         ps.packs.push_back(pk);
       }
   }
-  pack_finish(ctx, ps);
+  pack_finish(ctx, ps, ns);
   return ps;
 }
 
@@ -686,13 +735,13 @@ gd_status alloc_state(gd_ctx *ctx) {
   if (ns == 2 && !fits2)
     return fail(ctx, GD_ERR_CAPACITY, "receptor split needs ladders of at most 256 lanes");
   if (ns == 0) {
-    PackSet p1 = build_mc_packs(ctx, 512);
+    PackSet p1 = build_mc_packs(ctx, 512, 1, ctx->opts.ladders_per_pack);
     int64_t waves = 0;
     for (const Pack &pk : p1.packs) waves += (pk.n_slots * ctx->cx[pk.cx].lpr + 63) / 64;
     ns = (fits2 && waves < (int64_t)8 * ctx->n_cu) ? 2 : 1;
   }
   ctx->n_split = ns;
-  PackSet ps = build_mc_packs(ctx, ns == 2 ? 256 : 512);
+  PackSet ps = build_mc_packs(ctx, ns == 2 ? 256 : 512, ns, ctx->opts.ladders_per_pack);
   HIPCHK(hipMalloc(&ctx->d_packs, ps.packs.size() * sizeof(Pack)));
   HIPCHK(hipMalloc(&ctx->d_slots, ps.slots.size() * sizeof(int)));
   HIPCHK(hipMemcpyAsync(ctx->d_packs, ps.packs.data(), ps.packs.size() * sizeof(Pack),
@@ -700,24 +749,28 @@ gd_status alloc_state(gd_ctx *ctx) {
   HIPCHK(hipMemcpyAsync(ctx->d_slots, ps.slots.data(), ps.slots.size() * sizeof(int),
                         hipMemcpyHostToDevice, ctx->stream));
   ctx->n_packs = (int)ps.packs.size();
-  for (int a = 0; a < 5; ++a) ctx->cls[a] = ps.cls[a];
-  ctx->block = ps.block * ns;
-  layout(ctx->lay, &ctx->lds, ps.pnp_max, ps.pnk_max, ps.nch_max, ps.slots_max, ns, ps.block);
-  if (ctx->lds > ctx->max_lds)
-    return fail(ctx, GD_ERR_CAPACITY, "launch needs %zu bytes of LDS, device has %zu", ctx->lds,
-                ctx->max_lds);
-  if (ctx->block > 512)
-    return fail(ctx, GD_ERR_CAPACITY, "a temperature ladder needs %d lanes (> 512)", ctx->block);
+  ctx->mc_classes = ps.classes;
+  for (const LaunchClass &lc : ps.classes) {
+    if (lc.lds > ctx->max_lds)
+      return fail(ctx, GD_ERR_CAPACITY, "launch needs %zu bytes of LDS, device has %zu", lc.lds,
+                  ctx->max_lds);
+    if (lc.block > 512)
+      return fail(ctx, GD_ERR_CAPACITY, "a temperature ladder needs %d lanes (> 512)", lc.block);
+  }
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->state_alloc = true;
-  if (ctx->opts.verbose)
-    fprintf(stderr, "[geauxdock] %d replicas, %d packs, block %d (split %d), LDS %zu B\n", n,
-            ctx->n_packs, ctx->block, ctx->n_split, ctx->lds);
+  if (ctx->opts.verbose) {
+    fprintf(stderr, "[geauxdock] %d replicas, %d packs, split %d, %zu launch classes\n", n,
+            ctx->n_packs, ctx->n_split, ctx->mc_classes.size());
+    for (const LaunchClass &lc : ctx->mc_classes)
+      fprintf(stderr, "[geauxdock]   apl %d packs %d block %d LDS %zu B\n", lc.apl,
+              lc.end - lc.begin, lc.block, lc.lds);
+  }
   return GD_OK;
 }
 
 KParams base_params(gd_ctx *ctx) {
-  KParams P = ctx->lay;
+  KParams P{};
   P.cxs = ctx->d_cx;
   P.packs = ctx->d_packs;
   P.slots = ctx->d_slots;
@@ -746,31 +799,51 @@ const void *mc_kernel_fn(int apl, bool da) {
   }
 }
 
-// Launch every APL class of a pack set: one kernel instance per class, the
-// classes concurrently on side streams forked from and joined to `stream`.
-hipError_t launch_classes(gd_ctx *ctx, const int cls[5], const KParams &P0, bool da, int block,
-                          size_t lds, hipStream_t stream) {
-  int used[4], n_used = 0;
-  for (int a = 1; a <= 4; ++a)
-    if (cls[a] > cls[a - 1]) used[n_used++] = a;
+// Launch every class of a pack set, the classes concurrently on side streams
+// forked from and joined to `stream`. With `da`, the delayed-acceptance
+// instances run; `da_scratch` adds the per-wave LDS scratch of the direct
+// network evaluation (a class whose LDS would overflow then runs with one
+// lane group per pack: same packs, same state).
+hipError_t launch_classes(gd_ctx *ctx, const std::vector<LaunchClass> &classes, const KParams &P0,
+                          bool da, bool da_scratch, hipStream_t stream) {
+  const int n = (int)classes.size();
   hipError_t e = hipSuccess;
-  if (n_used > 1) {
+  const int n_side = std::min(n - 1, 3);
+  if (n_side > 0) {
     e = hipEventRecord(ctx->ev_fork, stream);
-    for (int i = 1; i < n_used && e == hipSuccess; ++i) e = hipStreamWaitEvent(ctx->aux[i - 1], ctx->ev_fork, 0);
+    for (int i = 0; i < n_side && e == hipSuccess; ++i) e = hipStreamWaitEvent(ctx->aux[i], ctx->ev_fork, 0);
   }
-  for (int i = 0; i < n_used && e == hipSuccess; ++i) {
-    const int a = used[i];
-    const void *f = mc_kernel_fn(a, da);
+  for (int i = 0; i < n && e == hipSuccess; ++i) {
+    const LaunchClass &lc = classes[i];
+    KParams P = P0;
+    apply_layout(P, lc.lay);
+    P.pack_base = lc.begin;
+    int block = lc.block;
+    size_t lds = lc.lds;
+    if (da && da_scratch) {
+      P.off_da = (int)((lds + 15) & ~(size_t)15);
+      lds = P.off_da + (size_t)(block / 64) * 6208;
+      if (lds > ctx->max_lds && P.n_split > 1) {
+        P.n_split = 1;
+        block = P.half;
+        lds = P.off_da + (size_t)(block / 64) * 6208;
+      }
+      if (lds > ctx->max_lds) return hipErrorInvalidValue;
+    } else if (da) {
+      P.off_da = (int)((lds + 15) & ~(size_t)15);
+    }
+    const void *f = mc_kernel_fn(lc.apl, da);
     e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) break;
-    KParams P = P0;
-    P.pack_base = cls[a - 1];
     void *args[] = {&P};
-    hipStream_t s = i == 0 ? stream : ctx->aux[i - 1];
-    e = hipLaunchKernel(f, dim3(cls[a] - cls[a - 1]), dim3(block), args, lds, s);
-    if (e == hipSuccess && i > 0) e = hipEventRecord(ctx->ev_join[i - 1], s);
+    const int side = i % 4;  // 0: the main stream
+    hipStream_t s = side == 0 ? stream : ctx->aux[side - 1];
+    e = hipLaunchKernel(f, dim3(lc.end - lc.begin), dim3(block), args, lds, s);
   }
-  for (int i = 1; i < n_used && e == hipSuccess; ++i) e = hipStreamWaitEvent(stream, ctx->ev_join[i - 1], 0);
+  for (int i = 0; i < n_side && e == hipSuccess; ++i) {
+    e = hipEventRecord(ctx->ev_join[i], ctx->aux[i]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(stream, ctx->ev_join[i], 0);
+  }
   return e;
 }
 
@@ -815,8 +888,7 @@ gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float
   P.step0 = step0;
   P.n_steps = n_steps;
   P.nc = nc;
-  size_t lds = ctx->lds;
-  int block = ctx->block;
+  bool scratch = false;
   if (da_k > 0 && mode == MODE_MC) {
     auto &S = ctx->sur;
     P.da_k = da_k;
@@ -827,21 +899,15 @@ gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float
     P.sw3 = S.p + SP_W3;
     P.sb3 = S.p + SP_B3;
     P.sur_tab = S.tab_valid ? 1 : 0;
-    P.off_da = (int)((lds + 15) & ~(size_t)15);
-    // the direct network evaluation needs per-wave LDS scratch
-    if (!P.sur_tab) lds = P.off_da + (size_t)(block / 64) * 6208;
-    if (lds > ctx->max_lds && P.n_split > 1) {
-      // the per-wave surrogate scratch of both lane groups does not fit:
-      // run this launch with one group per pack (same packs and state)
-      P.n_split = 1;
-      block = P.half;
-      lds = P.off_da + (size_t)(block / 64) * 6208;  // only the direct mode gets here
-    }
-    if (lds > ctx->max_lds)
-      return fail(ctx, GD_ERR_CAPACITY, "delayed acceptance needs %zu bytes of LDS", lds);
+    scratch = !P.sur_tab;  // the direct network evaluation needs per-wave LDS scratch
   }
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  HIPCHK(launch_classes(ctx, ctx->cls, P, P.da_k > 0, block, lds, ctx->stream));
+  {
+    const hipError_t le = launch_classes(ctx, ctx->mc_classes, P, P.da_k > 0, scratch, ctx->stream);
+    if (le == hipErrorInvalidValue)
+      return fail(ctx, GD_ERR_CAPACITY, "delayed acceptance scratch does not fit in LDS");
+    HIPCHK(le);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(hipEventSynchronize(ctx->ev1));
@@ -1092,15 +1158,13 @@ static gd_status eval_poses(gd_ctx *ctx, int cid, int n, const int *idx_lig, con
       ps.packs.push_back(pk);
     }
   }
-  pack_finish(ctx, ps);
-  KParams P{};
-  size_t lds = 0;
   // pose packs use the MC launch's receptor split (or the option's) so the
-  // same reduction order is exercised
-  int ns = ctx->state_alloc ? ctx->n_split : (ctx->opts.receptor_split == 2 ? 2 : 1);
-  if (ps.block > 256) ns = 1;
-  layout(P, &lds, ps.pnp_max, ps.pnk_max, ps.nch_max, ps.slots_max, ns, ps.block);
-  if (lds > ctx->max_lds) return fail(ctx, GD_ERR_CAPACITY, "LDS %zu > %zu", lds, ctx->max_lds);
+  // same reduction order is exercised; packs hold at most 256 lanes
+  const int ns = ctx->state_alloc ? ctx->n_split : (ctx->opts.receptor_split == 2 ? 2 : 1);
+  pack_finish(ctx, ps, ns);
+  for (const LaunchClass &lc : ps.classes)
+    if (lc.lds > ctx->max_lds) return fail(ctx, GD_ERR_CAPACITY, "LDS %zu > %zu", lc.lds, ctx->max_lds);
+  KParams P{};
   const bool sur = out_sur != nullptr;
   if (sur) {
     auto &S = ctx->sur;
@@ -1111,9 +1175,6 @@ static gd_status eval_poses(gd_ctx *ctx, int cid, int n, const int *idx_lig, con
     P.sb2 = S.p + SP_B2;
     P.sw3 = S.p + SP_W3;
     P.sb3 = S.p + SP_B3;
-    P.off_da = (int)((lds + 15) & ~(size_t)15);
-    if (!P.sur_tab) lds = P.off_da + (size_t)(ps.block * ns / 64) * 6208;
-    if (lds > ctx->max_lds) return fail(ctx, GD_ERR_CAPACITY, "LDS %zu > %zu", lds, ctx->max_lds);
   }
   Pack *d_packs = nullptr;
   int *d_slots = nullptr, *d_lig = nullptr;
@@ -1138,9 +1199,7 @@ static gd_status eval_poses(gd_ctx *ctx, int cid, int n, const int *idx_lig, con
   P.ev_out = d_out;
   P.ev_sur = d_sur;
   P.mode = sur ? MODE_SUREVAL : MODE_EVAL;
-  hipError_t le = hipSuccess;
-  if (ps.block * ns > 512) le = hipErrorInvalidValue;
-  if (le == hipSuccess) le = launch_classes(ctx, ps.cls, P, sur, ps.block * ns, lds, ctx->stream);
+  hipError_t le = launch_classes(ctx, ps.classes, P, sur, sur && !P.sur_tab, ctx->stream);
   if (le == hipSuccess && out)
     le = hipMemcpyAsync(out, d_out, (size_t)n * sizeof(gd_energy), hipMemcpyDeviceToHost, ctx->stream);
   if (le == hipSuccess && sur)

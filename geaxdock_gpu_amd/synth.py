@@ -127,17 +127,26 @@ def _chain(rng, n, bond=1.5, angle_deg=109.5, jitter_deg=10.0, dihedrals=None):
     if n > 2:
         a = ang[2]
         xyz[2] = xyz[1] + bond * np.array([-math.cos(a), math.sin(a), 0.0])
+    pts = [list(map(float, xyz[i])) for i in range(min(n, 3))]
+
+    def cross(u, v):
+        return [u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]]
+
+    def unit(u):
+        r = math.sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2])
+        return [u[0] / r, u[1] / r, u[2] / r]
+
     for i in range(3, n):
-        a, b, c = xyz[i - 3], xyz[i - 2], xyz[i - 1]
-        bc = c - b
-        bc /= np.linalg.norm(bc)
-        nrm = np.cross(b - a, bc)
-        nrm /= np.linalg.norm(nrm)
-        m = np.stack([bc, np.cross(nrm, bc), nrm], axis=1)
-        th, ph = ang[i], dih[i]
-        d2 = np.array([-bond * math.cos(th), bond * math.sin(th) * math.cos(ph),
-                       bond * math.sin(th) * math.sin(ph)])
-        xyz[i] = c + m @ d2
+        a, b, c = pts[i - 3], pts[i - 2], pts[i - 1]
+        bc = unit([c[0] - b[0], c[1] - b[1], c[2] - b[2]])
+        nrm = unit(cross([b[0] - a[0], b[1] - a[1], b[2] - a[2]], bc))
+        m1 = cross(nrm, bc)
+        th, ph = float(ang[i]), float(dih[i])
+        d0 = -bond * math.cos(th)
+        d1 = bond * math.sin(th) * math.cos(ph)
+        d2 = bond * math.sin(th) * math.sin(ph)
+        pts.append([c[k] + bc[k] * d0 + m1[k] * d1 + nrm[k] * d2 for k in range(3)])
+    xyz[:] = np.array(pts[:n])
     return xyz, dih
 
 

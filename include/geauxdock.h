@@ -164,6 +164,8 @@ typedef struct gd_opts {
   int verbose;
   int receptor_split;    /* 0 = auto; 1 = one lane group per replica; 2 = two lane
                             groups per replica, each over half of every receptor chunk */
+  int ladders_per_pack;  /* 0 = auto; else temperature ladders per workgroup (replicas
+                            per workgroup = this x n_tmp), capped by 512 lanes */
 } gd_opts;
 
 typedef struct gd_run_stats {
