@@ -609,28 +609,26 @@ void apply_layout(KParams &P, const KParams &L) {
 // among its packs, so small complexes keep several workgroups per CU.
 void pack_finish(const gd_ctx *ctx, PackSet &ps, int ns) {
   const size_t n_pk = ps.packs.size();
-  std::vector<int> res(n_pk), half(n_pk);
+  std::vector<int> res(n_pk), half(n_pk), apl(n_pk);
+  std::vector<double> cost(n_pk);
   for (size_t i = 0; i < n_pk; ++i) {
     const Pack &p = ps.packs[i];
     const HostComplex &h = ctx->cx[p.cx];
     half[i] = ((p.n_slots * h.lpr + 63) / 64) * 64;
+    apl[i] = h.apl;
+    cost[i] = (double)h.d.pnp * (double)half[i] * h.apl;
     KParams L{};
     size_t need = 0;
     layout(L, &need, (h.d.pnp + 3) & ~3, h.d.pnk, h.n_chunks, p.n_slots, ns, half[i]);
     res[i] = (int)std::max<size_t>(1, std::min<size_t>(8, ctx->max_lds / std::max<size_t>(need, 1)));
   }
+  // all per-pack keys are indexed by the pack's position before sorting
   std::vector<int> order(n_pk);
   for (size_t i = 0; i < n_pk; ++i) order[i] = (int)i;
-  auto cost = [&](int i) {
-    const Pack &p = ps.packs[i];
-    const HostComplex &h = ctx->cx[p.cx];
-    return (double)h.d.pnp * (double)half[i] * h.apl;
-  };
-  auto apl = [&](int i) { return ctx->cx[ps.packs[i].cx].apl; };
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    if (apl(a) != apl(b)) return apl(a) < apl(b);
+    if (apl[a] != apl[b]) return apl[a] < apl[b];
     if (res[a] != res[b]) return res[a] < res[b];
-    return cost(a) > cost(b);
+    return cost[a] > cost[b];
   });
   std::vector<Pack> sorted;
   for (int i : order) sorted.push_back(ps.packs[i]);
@@ -638,12 +636,12 @@ void pack_finish(const gd_ctx *ctx, PackSet &ps, int ns) {
   ps.classes.clear();
   for (size_t b = 0; b < n_pk;) {
     size_t e = b;
-    const int a0 = apl(order[b]), r0 = res[order[b]];
+    const int a0 = apl[order[b]], r0 = res[order[b]];
     LaunchClass lc;
     lc.apl = a0;
     lc.begin = (int)b;
     int pnp_max = 0, pnk_max = 0, nch_max = 0, slots_max = 1, half_max = 64;
-    while (e < n_pk && apl(order[e]) == a0 && res[order[e]] == r0) {
+    while (e < n_pk && apl[order[e]] == a0 && res[order[e]] == r0) {
       const Pack &p = ps.packs[e];
       const HostComplex &h = ctx->cx[p.cx];
       pnp_max = std::max(pnp_max, h.d.pnp);
