@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--split", type=int, default=0, help="receptor split (0 = auto, 1, 2)")
     ap.add_argument("--ladders", type=int, default=0, help="temperature ladders per workgroup")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--launch", type=int, default=0, help="GD_LAUNCH_* flags")
+    ap.add_argument("--autotune", type=int, default=-1,
+                    help="tune launch shapes first (default: on for the batch workload)")
+    ap.add_argument("--tune-budget", type=float, default=90.0, help="seconds")
     ap.add_argument("--mode", default="plain", choices=["plain", "da"],
                     help="plain engine only, or plain vs surrogate delayed acceptance")
     ap.add_argument("--k-sub", type=int, default=4, help="surrogate steps per exact evaluation")
@@ -255,9 +259,18 @@ def main():
     if args.mode == "da":
         print(json.dumps(da_compare(args, cxs, desc)))
         return
-    eng = gd.Engine(seed=0, record=True, steps_per_dump=2000, lanes_per_replica=args.lpr,
-                    atoms_per_lane=args.apl, receptor_split=args.split,
-                    ladders_per_pack=args.ladders, verbose=args.verbose)
+    opts = dict(lanes_per_replica=args.lpr, atoms_per_lane=args.apl, receptor_split=args.split,
+                ladders_per_pack=args.ladders, launch=args.launch)
+    tuned = None
+    if args.autotune == 1 or (args.autotune == -1 and args.config == "batch"):
+        from geaxdock_gpu_amd import autotune
+        t0 = time.perf_counter()
+        best, trials = autotune.tune(cxs, nc=args.nc, budget_s=args.tune_budget)
+        opts.update(best)
+        tuned = {"best": best, "seconds": time.perf_counter() - t0,
+                 "trials": [{**t["opts"], "ms_per_step": (t["s_per_step"] or 0) * 1e3}
+                            for t in trials]}
+    eng = gd.Engine(seed=0, record=True, steps_per_dump=2000, verbose=args.verbose, **opts)
     for cx in cxs:
         eng.add_complex(cx)
     n_rep = sum(c.n_rep for c in cxs)
@@ -293,6 +306,8 @@ def main():
                      "kernel": kernel_names(eng), "kernel_ms_avg": st["kernel_ms_avg"],
                      "flops_per_launch": flops_launch,
                      "note": "FP32 vector (VALU) bound; 36 flops per ligand-receptor pair"},
+        "launch_options": opts,
+        "autotune": tuned,
         "acceptance_ratio": state_accept,
         "exchange_accepts": st["exchanges_accepted"],
         "t_compute_s": st["t_compute"],

@@ -115,7 +115,8 @@ class Engine:
     """One engine context on one device (gd_create ... gd_destroy)."""
 
     def __init__(self, seed=0, device=0, steps_per_dump=2000, record=True, lanes_per_replica=0,
-                 atoms_per_lane=0, verbose=False, receptor_split=0, ladders_per_pack=0):
+                 atoms_per_lane=0, verbose=False, receptor_split=0, ladders_per_pack=0,
+                 launch=0):
         self.lib = load_library()
         o = abi.Opts()
         o.device, o.seed, o.steps_per_dump = device, seed, steps_per_dump
@@ -123,6 +124,7 @@ class Engine:
         o.lanes_per_replica, o.atoms_per_lane = lanes_per_replica, atoms_per_lane
         o.receptor_split = receptor_split
         o.ladders_per_pack = ladders_per_pack
+        o.launch = launch
         self._ctx = C.c_void_p()
         st = self.lib.gd_create(C.byref(self._ctx), C.byref(o))
         if st != 0:

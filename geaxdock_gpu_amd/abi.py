@@ -87,7 +87,12 @@ class Opts(C.Structure):
     _fields_ = [("device", C.c_int), ("seed", C.c_uint64), ("steps_per_dump", C.c_int),
                 ("record", C.c_int), ("lanes_per_replica", C.c_int),
                 ("atoms_per_lane", C.c_int), ("verbose", C.c_int),
-                ("receptor_split", C.c_int), ("ladders_per_pack", C.c_int)]
+                ("receptor_split", C.c_int), ("ladders_per_pack", C.c_int),
+                ("launch", C.c_int)]
+
+
+LAUNCH_NO_GRAPH = 1
+LAUNCH_PERSISTENT = 2
 
 
 class RunStats(C.Structure):

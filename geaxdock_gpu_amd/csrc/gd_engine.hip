@@ -75,6 +75,15 @@ struct gd_ctx {
   int *d_slots = nullptr;
   int n_packs = 0, n_split = 1, ladders_per_pack = 0;
   std::vector<LaunchClass> mc_classes;  // kernel launches of the MC pack set
+  // device control words: [0] first step of a graph replay, [1 + i] pack
+  // counter of class i in persistent launches
+  int *d_ctrl = nullptr;
+  int h_step = 0;  // host source of the step upload (outlives the async copy)
+  // captured MC launch (one graph per window shape)
+  hipGraphExec_t graph = nullptr;
+  bool graph_valid = false;
+  int g_mode = -1, g_win = 0, g_nc = 0, g_da = 0, g_tab = 0, g_persistent = 0;
+  float g_scale = 0.0f;
   // learned surrogate
   struct Sur {
     std::vector<float *> grid;  // per complex [n_prt][n^3][32]
@@ -224,7 +233,8 @@ void gd_destroy(gd_ctx *ctx) {
     for (void *p : h.bufs) (void)hipFree(p);
   void *ps[] = {ctx->d_cx, ctx->d_overflow, ctx->d_mm, ctx->d_eold, ctx->d_idx_tmp,
                 ctx->d_acs_mc, ctx->d_acs_x, ctx->d_rep_flat, ctx->d_rec, ctx->d_rec_cnt,
-                ctx->d_packs, ctx->d_slots, ctx->d_best_e, ctx->d_best_mm};
+                ctx->d_packs, ctx->d_slots, ctx->d_best_e, ctx->d_best_mm, ctx->d_ctrl};
+  if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
   for (void *p : ps)
     if (p) (void)hipFree(p);
   {
@@ -502,6 +512,7 @@ gd_status sync_complexes(gd_ctx *ctx) {
   if (!ctx->cx_dirty) return GD_OK;
   if (ctx->d_cx) HIPCHK(hipFree(ctx->d_cx));
   ctx->d_cx = nullptr;
+  ctx->graph_valid = false;  // captured launches hold the old pointer
   std::vector<DevComplex> v;
   for (auto &h : ctx->cx) v.push_back(h.dev);
   HIPCHK(hipMalloc(&ctx->d_cx, std::max<size_t>(v.size(), 1) * sizeof(DevComplex)));
@@ -575,6 +586,8 @@ void layout(KParams &P, size_t *bytes, int pnp_max, int pnk_max, int nch_max, in
   P.xp_floats = n_split > 1 ? n_split * half * 4 + n_split * P.xp_slots * 8 : 0;
   P.off_xp = (int)o;
   o = al(o + (size_t)2 * P.xp_floats * 4);
+  P.off_q = (int)o;
+  o = al(o + 16);
   *bytes = o;
 }
 
@@ -601,6 +614,7 @@ void apply_layout(KParams &P, const KParams &L) {
   P.off_xp = L.off_xp;
   P.xp_floats = L.xp_floats;
   P.xp_slots = L.xp_slots;
+  P.off_q = L.off_q;
 }
 
 // Group packs into launch classes by (atoms per lane, workgroups per CU that
@@ -748,6 +762,8 @@ gd_status alloc_state(gd_ctx *ctx) {
                         hipMemcpyHostToDevice, ctx->stream));
   ctx->n_packs = (int)ps.packs.size();
   ctx->mc_classes = ps.classes;
+  HIPCHK(hipMalloc(&ctx->d_ctrl, (1 + ps.classes.size()) * sizeof(int)));
+  HIPCHK(hipMemsetAsync(ctx->d_ctrl, 0, (1 + ps.classes.size()) * sizeof(int), ctx->stream));
   for (const LaunchClass &lc : ps.classes) {
     if (lc.lds > ctx->max_lds)
       return fail(ctx, GD_ERR_CAPACITY, "launch needs %zu bytes of LDS, device has %zu", lc.lds,
@@ -788,12 +804,23 @@ KParams base_params(gd_ctx *ctx) {
   return P;
 }
 
-const void *mc_kernel_fn(int apl, bool da) {
+// Kernel instance for atoms per lane `apl` and mode `da`: 0 plain,
+// 1 delayed acceptance with the network per atom, 2 with its tables.
+template <int A>
+const void *mc_kernel_da(int da) {
+  switch (da) {
+    case 1: return (const void *)gd_mc_kernel<A, 1>;
+    case 2: return (const void *)gd_mc_kernel<A, 2>;
+    default: return (const void *)gd_mc_kernel<A, 0>;
+  }
+}
+
+const void *mc_kernel_fn(int apl, int da) {
   switch (apl) {
-    case 1: return da ? (const void *)gd_mc_kernel<1, true> : (const void *)gd_mc_kernel<1, false>;
-    case 2: return da ? (const void *)gd_mc_kernel<2, true> : (const void *)gd_mc_kernel<2, false>;
-    case 3: return da ? (const void *)gd_mc_kernel<3, true> : (const void *)gd_mc_kernel<3, false>;
-    default: return da ? (const void *)gd_mc_kernel<4, true> : (const void *)gd_mc_kernel<4, false>;
+    case 1: return mc_kernel_da<1>(da);
+    case 2: return mc_kernel_da<2>(da);
+    case 3: return mc_kernel_da<3>(da);
+    default: return mc_kernel_da<4>(da);
   }
 }
 
@@ -802,46 +829,90 @@ const void *mc_kernel_fn(int apl, bool da) {
 // instances run; `da_scratch` adds the per-wave LDS scratch of the direct
 // network evaluation (a class whose LDS would overflow then runs with one
 // lane group per pack: same packs, same state).
-hipError_t launch_classes(gd_ctx *ctx, const std::vector<LaunchClass> &classes, const KParams &P0,
-                          bool da, bool da_scratch, hipStream_t stream) {
-  const int n = (int)classes.size();
+// Per-class kernel arguments of a launch (persistent grids when `queue`).
+struct ClassLaunch {
+  const void *f;
+  KParams P;
+  int grid, block;
+  size_t lds;
+};
+
+hipError_t plan_classes(gd_ctx *ctx, const std::vector<LaunchClass> &classes, const KParams &P0,
+                        int da, bool da_scratch, int *queue, std::vector<ClassLaunch> &out) {
+  out.clear();
+  for (size_t i = 0; i < classes.size(); ++i) {
+    const LaunchClass &lc = classes[i];
+    ClassLaunch cl;
+    cl.P = P0;
+    KParams &P = cl.P;
+    apply_layout(P, lc.lay);
+    P.pack_base = lc.begin;
+    cl.block = lc.block;
+    cl.lds = lc.lds;
+    if (da && da_scratch) {
+      P.off_da = (int)((cl.lds + 15) & ~(size_t)15);
+      cl.lds = P.off_da + (size_t)(cl.block / 64) * 6208;
+      if (cl.lds > ctx->max_lds && P.n_split > 1) {
+        P.n_split = 1;
+        cl.block = P.half;
+        cl.lds = P.off_da + (size_t)(cl.block / 64) * 6208;
+      }
+      if (cl.lds > ctx->max_lds) return hipErrorInvalidValue;
+    } else if (da) {
+      P.off_da = (int)((cl.lds + 15) & ~(size_t)15);
+    }
+    const int n_pk = lc.end - lc.begin;
+    cl.grid = n_pk;
+    if (queue) {
+      // resident workgroups: LDS-limited, and at most 8 waves per CU (the
+      // kernels' register use allows two waves per SIMD)
+      const int by_lds = (int)std::max<size_t>(1, ctx->max_lds / std::max<size_t>(cl.lds, 1));
+      const int by_waves = std::max(1, 8 / std::max(1, cl.block / 64));
+      cl.grid = std::min(n_pk, ctx->n_cu * std::min(by_lds, by_waves));
+      P.queue = queue + i;
+      P.n_queue = n_pk;
+    }
+    cl.f = mc_kernel_fn(lc.apl, da);
+    const hipError_t e = hipFuncSetAttribute(cl.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cl.lds);
+    if (e != hipSuccess) return e;
+    out.push_back(cl);
+  }
+  return hipSuccess;
+}
+
+// Enqueue planned class launches, the classes concurrently on side streams
+// forked from and joined to `stream` (capturable).
+hipError_t enqueue_classes(gd_ctx *ctx, std::vector<ClassLaunch> &cls, int *queue,
+                           hipStream_t stream) {
+  const int n = (int)cls.size();
   hipError_t e = hipSuccess;
+  if (queue && n > 0) {
+    hipLaunchKernelGGL(gd_queue_reset_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, queue, n);
+    e = hipGetLastError();
+  }
   const int n_side = std::min(n - 1, 3);
-  if (n_side > 0) {
+  if (e == hipSuccess && n_side > 0) {
     e = hipEventRecord(ctx->ev_fork, stream);
     for (int i = 0; i < n_side && e == hipSuccess; ++i) e = hipStreamWaitEvent(ctx->aux[i], ctx->ev_fork, 0);
   }
   for (int i = 0; i < n && e == hipSuccess; ++i) {
-    const LaunchClass &lc = classes[i];
-    KParams P = P0;
-    apply_layout(P, lc.lay);
-    P.pack_base = lc.begin;
-    int block = lc.block;
-    size_t lds = lc.lds;
-    if (da && da_scratch) {
-      P.off_da = (int)((lds + 15) & ~(size_t)15);
-      lds = P.off_da + (size_t)(block / 64) * 6208;
-      if (lds > ctx->max_lds && P.n_split > 1) {
-        P.n_split = 1;
-        block = P.half;
-        lds = P.off_da + (size_t)(block / 64) * 6208;
-      }
-      if (lds > ctx->max_lds) return hipErrorInvalidValue;
-    } else if (da) {
-      P.off_da = (int)((lds + 15) & ~(size_t)15);
-    }
-    const void *f = mc_kernel_fn(lc.apl, da);
-    e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) break;
-    void *args[] = {&P};
+    void *args[] = {&cls[i].P};
     const int side = i % 4;  // 0: the main stream
     hipStream_t s = side == 0 ? stream : ctx->aux[side - 1];
-    e = hipLaunchKernel(f, dim3(lc.end - lc.begin), dim3(block), args, lds, s);
+    e = hipLaunchKernel(cls[i].f, dim3(cls[i].grid), dim3(cls[i].block), args, cls[i].lds, s);
   }
   for (int i = 0; i < n_side && e == hipSuccess; ++i) {
     e = hipEventRecord(ctx->ev_join[i], ctx->aux[i]);
     if (e == hipSuccess) e = hipStreamWaitEvent(stream, ctx->ev_join[i], 0);
   }
+  return e;
+}
+
+hipError_t launch_classes(gd_ctx *ctx, const std::vector<LaunchClass> &classes, const KParams &P0,
+                          int da, bool da_scratch, hipStream_t stream) {
+  std::vector<ClassLaunch> cls;
+  hipError_t e = plan_classes(ctx, classes, P0, da, da_scratch, nullptr, cls);
+  if (e == hipSuccess) e = enqueue_classes(ctx, cls, nullptr, stream);
   return e;
 }
 
@@ -899,12 +970,55 @@ gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float
     P.sur_tab = S.tab_valid ? 1 : 0;
     scratch = !P.sur_tab;  // the direct network evaluation needs per-wave LDS scratch
   }
-  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  {
-    const hipError_t le = launch_classes(ctx, ctx->mc_classes, P, P.da_k > 0, scratch, ctx->stream);
-    if (le == hipErrorInvalidValue)
+  const int da = P.da_k > 0 ? (P.sur_tab ? 2 : 1) : 0;
+  const bool persistent = (ctx->opts.launch & GD_LAUNCH_PERSISTENT) && mode == MODE_MC;
+  const bool use_graph = !(ctx->opts.launch & GD_LAUNCH_NO_GRAPH) && mode == MODE_MC;
+  int *queue = persistent ? ctx->d_ctrl + 1 : nullptr;
+  if (use_graph) {
+    // the first step comes from device memory so one graph serves every window
+    P.step_dev = ctx->d_ctrl;
+    const bool same = ctx->graph_valid && ctx->g_mode == mode && ctx->g_win == n_steps &&
+                      ctx->g_nc == nc && ctx->g_da == P.da_k && ctx->g_tab == P.sur_tab &&
+                      ctx->g_scale == P.sur_scale && ctx->g_persistent == (int)persistent;
+    if (!same) {
+      if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+      ctx->graph = nullptr;
+      ctx->graph_valid = false;
+      std::vector<ClassLaunch> cls;
+      hipError_t e = plan_classes(ctx, ctx->mc_classes, P, da, scratch, queue, cls);
+      if (e == hipErrorInvalidValue)
+        return fail(ctx, GD_ERR_CAPACITY, "delayed acceptance scratch does not fit in LDS");
+      HIPCHK(e);
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      e = enqueue_classes(ctx, cls, queue, ctx->stream);
+      const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+      HIPCHK(e);
+      HIPCHK(e2);
+      e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHK(e);
+      ctx->graph_valid = true;
+      ctx->g_mode = mode;
+      ctx->g_win = n_steps;
+      ctx->g_nc = nc;
+      ctx->g_da = P.da_k;
+      ctx->g_tab = P.sur_tab;
+      ctx->g_scale = P.sur_scale;
+      ctx->g_persistent = (int)persistent;
+    }
+    ctx->h_step = step0;
+    HIPCHK(hipMemcpyAsync(ctx->d_ctrl, &ctx->h_step, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(hipGraphLaunch(ctx->graph, ctx->stream));
+  } else {
+    std::vector<ClassLaunch> cls;
+    hipError_t e = plan_classes(ctx, ctx->mc_classes, P, da, scratch, queue, cls);
+    if (e == hipErrorInvalidValue)
       return fail(ctx, GD_ERR_CAPACITY, "delayed acceptance scratch does not fit in LDS");
-    HIPCHK(le);
+    HIPCHK(e);
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(enqueue_classes(ctx, cls, queue, ctx->stream));
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1197,7 +1311,8 @@ static gd_status eval_poses(gd_ctx *ctx, int cid, int n, const int *idx_lig, con
   P.ev_out = d_out;
   P.ev_sur = d_sur;
   P.mode = sur ? MODE_SUREVAL : MODE_EVAL;
-  hipError_t le = launch_classes(ctx, ps.classes, P, sur, sur && !P.sur_tab, ctx->stream);
+  hipError_t le = launch_classes(ctx, ps.classes, P, sur ? (P.sur_tab ? 2 : 1) : 0,
+                                 sur && !P.sur_tab, ctx->stream);
   if (le == hipSuccess && out)
     le = hipMemcpyAsync(out, d_out, (size_t)n * sizeof(gd_energy), hipMemcpyDeviceToHost, ctx->stream);
   if (le == hipSuccess && sur)

@@ -94,6 +94,10 @@ struct KParams {
   gd_energy *ev_out;
   int mode;
   int step0, n_steps, nc;
+  const int *step_dev;  // when set, the first step is read here (graph replays)
+  int *queue;           // when set, persistent workgroups take packs from this counter
+  int n_queue;          // packs in this launch (persistent mode)
+  int off_q;            // LDS word for the queue broadcast
   uint32_t k0, k1;
   // LDS layout (bytes)
   int off_hpp, off_psp, off_par, off_kde, off_kr, off_ch, off_xe, off_xi, off_xa, off_xo;

@@ -486,14 +486,14 @@ __device__ __forceinline__ float cheap_terms(const DevComplex &C, const Lds &S, 
 // wave's atoms, 16 per MFMA tile: tile (k, sub) holds atom k of lanes
 // 16 sub .. 16 sub + 15. `da` is the wave's scratch: X tile [16][64] bf16,
 // A1 tile [16][128] bf16, y[16].
-template <int APL>
+template <int APL, int DAM>
 __device__ __forceinline__ float sur_total(const KParams &P, const DevComplex &C, const Lds &S,
                                            unsigned char *da, int lpr, int prt,
                                            const float (&x)[APL], const float (&y)[APL],
                                            const float (&z)[APL], const int (&lt)[APL],
                                            const int (&li)[APL], const bool (&val)[APL],
                                            const float mm[6]) {
-  if (P.sur_tab) {
+  if constexpr (DAM == 2) {
     // tabulated network: trilinear interpolation of the atom type's output
     // grid (same grid geometry and clamping as the feature grid)
     const int n = C.sn;
@@ -608,9 +608,11 @@ __device__ __forceinline__ void exchange_ladder(int n_tmp, int mode_t, const flo
   }
 }
 
-template <int APL, bool DA>
+// DA: 0 plain Metropolis; delayed acceptance with the surrogate from the
+// network evaluated per atom (1) or from its tabulated outputs (2).
+template <int APL, int DA>
 __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const DevComplex &C, const Lds &S,
-                         int lpr) {
+                         int lpr, int step0) {
   const int tid = threadIdx.x;
   Split sp;
   sp.ns = P.n_split;
@@ -681,7 +683,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
       extern __shared__ __attribute__((aligned(16))) unsigned char lds_se[];
       unsigned char *dse = lds_se + P.off_da + (threadIdx.x >> 6) * 6208;
       place(mo);
-      const float sv_s = sur_total<APL>(P, C, S, dse, lpr, pk.prt, x, y, z, lt, li, val, mo);
+      const float sv_s = sur_total<APL, DA>(P, C, S, dse, lpr, pk.prt, x, y, z, lt, li, val, mo);
       if (active && lead) P.ev_sur[sv] = sv_s;
       return;
     }
@@ -708,7 +710,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
             r.replica.idx_lig = ilig;
             r.energy = E;
             for (int i = 0; i < 6; ++i) r.movematrix[i] = mo[i];
-            r.step = P.step0;
+            r.step = step0;
             P.rec_cnt[sv] = rc + 1;
           }
         }
@@ -763,11 +765,11 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
     da = lds_da + P.off_da + (threadIdx.x >> 6) * 6208;
     if (wave_active) {
       place(mo);
-      s_old = sur_total<APL>(P, C, S, da, lpr, pk.prt, x, y, z, lt, li, val, mo);
+      s_old = sur_total<APL, DA>(P, C, S, da, lpr, pk.prt, x, y, z, lt, li, val, mo);
     }
   }
 
-  for (int s = P.step0; s < P.step0 + P.n_steps; s += K) {
+  for (int s = step0; s < step0 + P.n_steps; s += K) {
     if constexpr (!DA) {
       float u[8], mn[6];
       if (wave_active) {
@@ -801,7 +803,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
 #pragma unroll
         for (int i = 0; i < 6; ++i) mn[i] = yv[i] + dm[i];
         place(mn);
-        const float s_n = sur_total<APL>(P, C, S, da, lpr, pk.prt, x, y, z, lt, li, val, mn);
+        const float s_n = sur_total<APL, DA>(P, C, S, da, lpr, pk.prt, x, y, z, lt, li, val, mn);
         if (metropolis(s_n, s_y, mb, u[6])) {
 #pragma unroll
           for (int i = 0; i < 6; ++i) yv[i] = mn[i];
@@ -862,10 +864,10 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   }
 }
 
-template <int APL, bool DA>
-__device__ __forceinline__ void mc_kernel_body(const KParams &P) {
+template <int APL, int DA>
+__device__ __forceinline__ void mc_kernel_body(const KParams &P, int idx, int step0) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-  const Pack pk = P.packs[P.pack_base + blockIdx.x];
+  const Pack pk = P.packs[P.pack_base + idx];
   const DevComplex &C = P.cxs[pk.cx];
   Lds S;
   S.xyze = reinterpret_cast<const float4 *>(lds);
@@ -912,16 +914,39 @@ __device__ __forceinline__ void mc_kernel_body(const KParams &P) {
   }
   __syncthreads();
 
-  run_pack<APL, DA>(P, pk, C, S, C.lpr);
+  run_pack<APL, DA>(P, pk, C, S, C.lpr, step0);
 }
 
 // MC kernel for packs whose complexes place APL ligand atoms on each lane:
 // plain MC, start-pose evaluation and pose evaluation (DA = false), or
 // delayed acceptance with surrogate sub-steps (DA = true). One instance per
 // APL keeps each variant's register allocation to what it needs.
-template <int APL, bool DA>
+template <int APL, int DA>
 __global__ void __launch_bounds__(512) gd_mc_kernel(KParams P) {
-  mc_kernel_body<APL, DA>(P);
+  const int step0 = P.step_dev ? *P.step_dev : P.step0;
+  // persistent launches: each workgroup takes packs in LPT order from a
+  // counter until none is left; otherwise one pack per workgroup
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_q[];
+  int *next = reinterpret_cast<int *>(lds_q + P.off_q);
+  const bool persistent = P.queue != nullptr;
+  int idx = blockIdx.x;
+  for (;;) {
+    if (persistent) {
+      if (threadIdx.x == 0) *next = atomicAdd(P.queue, 1);
+      __syncthreads();
+      idx = *next;
+      __syncthreads();  // every thread has read the index; the previous pack's LDS is free
+      if (idx >= P.n_queue) break;
+    }
+    mc_kernel_body<APL, DA>(P, idx, step0);
+    if (!persistent) break;
+  }
+}
+
+// Zero the per-class pack counters of persistent launches.
+extern "C" __global__ void gd_queue_reset_kernel(int *queue, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) queue[i] = 0;
 }
 
 // Best-so-far := current state, for every replica.

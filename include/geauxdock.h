@@ -166,7 +166,16 @@ typedef struct gd_opts {
                             groups per replica, each over half of every receptor chunk */
   int ladders_per_pack;  /* 0 = auto; else temperature ladders per workgroup (replicas
                             per workgroup = this x n_tmp), capped by 512 lanes */
+  int launch;            /* GD_LAUNCH_* flags; 0 = graph-captured, non-persistent */
 } gd_opts;
+
+/* gd_opts.launch flags. MC windows are captured once into a HIP graph (all
+ * launch classes, forked over side streams) and replayed; the first step of
+ * a window is read from device memory. PERSISTENT launches each class with
+ * as many workgroups as stay resident, taking packs from a device counter in
+ * longest-first order. */
+#define GD_LAUNCH_NO_GRAPH 1
+#define GD_LAUNCH_PERSISTENT 2
 
 typedef struct gd_run_stats {
   double t_compute; /* seconds in MC kernels (HIP events), the reference's McLog.t0 */

@@ -228,3 +228,60 @@ def test_run_deterministic_and_window_independent(eng_mod):
     _, s2, _ = _run_gpu(eng_mod, cx, 9, 300, 20, steps_per_dump=2000)
     for k in s1:
         assert np.array_equal(s1[k], s2[k]), k
+
+
+def _batch_states(eng_mod, cxs, seed, n_steps, nc, **kw):
+    with eng_mod.Engine(seed=seed, record=False, **kw) as eng:
+        for cx in cxs:
+            eng.add_complex(cx)
+        eng.run(n_steps, nc)
+        return [eng.state(c) for c in range(len(cxs))]
+
+
+def test_launch_modes_bit_identical(eng_mod):
+    """Graph-captured, eager, persistent and non-persistent launches of an
+    uneven batch give bit-identical states (graph replays included)."""
+    cxs = synth.config_batch(3, seed=5, lna_range=(12, 40), pnp_range=(800, 1600))
+    ref = None
+    for launch in (0, abi.LAUNCH_NO_GRAPH, abi.LAUNCH_PERSISTENT,
+                   abi.LAUNCH_PERSISTENT | abi.LAUNCH_NO_GRAPH):
+        st = _batch_states(eng_mod, cxs, 3, 120, 20, launch=launch, steps_per_dump=40)
+        if ref is None:
+            ref = st
+            continue
+        for a, b in zip(ref, st):
+            for k in a:
+                assert np.array_equal(a[k], b[k]), (launch, k)
+
+
+@pytest.mark.parametrize("split,ladders", [(1, 0), (2, 1), (1, 3)])
+def test_batch_tracks_cpu_path(oracle, eng_mod, split, ladders):
+    """An uneven batch in one engine: every complex follows the FP32 CPU path
+    of that complex (its own Philox streams, keyed by complex id)."""
+    cxs = synth.config_batch(3, seed=6, lna_range=(12, 40), pnp_range=(800, 1600), n_lig=2,
+                             n_prt=2, n_tmp=4)
+    n_steps, nc = 200, 20
+    recs = {c: [] for c in range(len(cxs))}
+    with eng_mod.Engine(seed=4, receptor_split=split, ladders_per_pack=ladders) as eng:
+        for cx in cxs:
+            eng.add_complex(cx)
+        eng.run(n_steps, nc, sink=lambda c, r: recs[c].append(r))
+        states = [eng.state(c) for c in range(len(cxs))]
+    total = same = 0
+    for c, cx in enumerate(cxs):
+        g_all = np.concatenate(recs[c])
+        d = cx.desc()
+        cpu = oracle.McState(cx, rec_cap=n_steps + 1)
+        oracle.mc_run(d, cpu, seed=4, cid=c, init=True, step0=0, n_steps=0, nc=nc)
+        oracle.mc_run(d, cpu, seed=4, cid=c, init=False, step0=0, n_steps=n_steps, nc=nc)
+        for r in range(cx.n_rep):
+            g = g_all[g_all["replica"][:, 0] == r]
+            cr = cpu.records(r)
+            total += 1
+            if len(g) == len(cr) and np.array_equal(g["step"], cr["step"]):
+                same += 1
+                np.testing.assert_allclose(g["energy"]["e"][:, 9], cr["energy"]["e"][:, 9],
+                                           rtol=2e-4, atol=2e-4)
+        assert np.array_equal(states[c]["replica"][:, 2], cpu.idx_tmp) or same < total
+    print(f"\nsplit {split} ladders {ladders}: {same}/{total} replicas with identical decisions")
+    assert same >= total - 2, (same, total)
