@@ -41,7 +41,8 @@ def parse():
     ap.add_argument("--nc", type=int, default=20, help="steps per exchange")
     ap.add_argument("--config", default="ensemble", choices=["ensemble", "single", "batch"])
     ap.add_argument("--batch", type=int, default=256, help="complexes in the batch workload")
-    ap.add_argument("--cpu-sample-steps", type=int, default=30)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="length of the CPU-path sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lpr", type=int, default=0)
     ap.add_argument("--apl", type=int, default=0)
@@ -118,10 +119,17 @@ def cpu_baseline(cxs, args):
     threads = os.cpu_count() or 1
     threads = min(threads, 16)
     gdo.mc_run(d, st, seed=0, init=True, step0=0, n_steps=0, nc=args.nc, threads=threads)
-    steps = args.cpu_sample_steps
+    # windows of exchange segments until about cpu_seconds of work
+    chunk = max(args.nc, 20)
+    steps = 0
     t0 = time.perf_counter()
-    gdo.mc_run(d, st, seed=0, init=False, step0=0, n_steps=steps, nc=args.nc, threads=threads)
-    dt = time.perf_counter() - t0
+    while True:
+        gdo.mc_run(d, st, seed=0, init=False, step0=steps, n_steps=chunk, nc=args.nc,
+                   threads=threads)
+        steps += chunk
+        dt = time.perf_counter() - t0
+        if dt >= args.cpu_seconds:
+            break
     rs = sub.n_rep * steps
     # per replica-step cost scales with lna * pnp; rescale to the workload's mean
     w_sub = sub.lna * sub.pnp
@@ -135,7 +143,7 @@ def cpu_baseline(cxs, args):
 def kernel_names(eng):
     """Kernel instances the MC launch uses (one per atoms-per-lane class)."""
     apls = sorted({eng.shape(c)[1] for c in range(len(eng.complexes))})
-    return ", ".join(f"gd::gd_mc_kernel<{a}, false>" for a in apls)
+    return ", ".join(f"gd::gd_mc_kernel<{a}, 0>" for a in apls)
 
 
 def da_compare(args, cxs, desc):

@@ -285,3 +285,36 @@ def test_batch_tracks_cpu_path(oracle, eng_mod, split, ladders):
         assert np.array_equal(states[c]["replica"][:, 2], cpu.idx_tmp) or same < total
     print(f"\nsplit {split} ladders {ladders}: {same}/{total} replicas with identical decisions")
     assert same >= total - 2, (same, total)
+
+
+def test_statistics_match_cpu_path_independent_seeds(oracle, eng_mod):
+    """Independent chains (different seeds) of the GPU engine and the FP32 CPU
+    path on one complex, exchanges off: at every temperature the per-replica
+    acceptance ratios and minimum energies come from the same distributions
+    (two-sample Kolmogorov-Smirnov, p > 1e-4 for each of the 8 comparisons)."""
+    from scipy.stats import ks_2samp
+    cx = synth.make_complex(seed=7, lna=24, pnp=900, n_lig=32, n_prt=1, n_tmp=4,
+                            t_floor=0.5, t_ceiling=5.0)
+    n_steps = 400
+    with eng_mod.Engine(seed=101, record=False) as eng:
+        cid = eng.add_complex(cx)
+        eng.run(n_steps, 0)
+        st = eng.state(cid)
+        g_best = eng.best(cid)[0].astype(np.float64)
+    g_acc = st["acs_mc"] / n_steps
+    d = cx.desc()
+    cpu = oracle.McState(cx, rec_cap=n_steps + 1)
+    oracle.mc_run(d, cpu, seed=202, init=True, step0=0, n_steps=0, nc=0)
+    oracle.mc_run(d, cpu, seed=202, init=False, step0=0, n_steps=n_steps, nc=0)
+    c_acc = cpu.acs_mc / n_steps
+    c_best = np.array([cpu.records(r)["energy"]["e"][:, 9].min() for r in range(cx.n_rep)],
+                      np.float64)
+    temp = (np.arange(cx.n_rep) // cx.n_lig) % cx.n_tmp
+    for t in range(cx.n_tmp):
+        sel = temp == t
+        p_acc = ks_2samp(g_acc[sel], c_acc[sel]).pvalue
+        p_min = ks_2samp(g_best[sel], c_best[sel]).pvalue
+        print(f"\nT{t}: acceptance GPU {g_acc[sel].mean():.4f} CPU {c_acc[sel].mean():.4f} "
+              f"p={p_acc:.3f}; min energy GPU {g_best[sel].mean():.4f} CPU "
+              f"{c_best[sel].mean():.4f} p={p_min:.3f}")
+        assert p_acc > 1e-4 and p_min > 1e-4, (t, p_acc, p_min)
