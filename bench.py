@@ -1,16 +1,20 @@
 #!/usr/bin/env python3
-"""Benchmark: MC replica-steps/s of the plain REMC engine on one MI355X.
+"""Benchmark: MC replica-steps/s, plain vs surrogate delayed acceptance, on one MI355X.
 
 Workload (default): the ensemble configuration -- one synthetic complex,
 3,000-point receptor with 11 protein conformers, 48-atom ligand with 20
 conformers, 16 temperatures (3,520 replicas), KDE 200, MCS 8 x 40, an exchange
 attempt every 20 steps, FP32. A "step" is one MC step of every replica.
 
-value = replicas * steps / seconds of the timed region (the steps run with all
-inputs resident in HBM; accepted-state records are written to HBM as the
-reference's kernel does, and not copied to the host).
+Default mode (da): a plain warm-up, 1,000 logged plain steps, surrogate
+training on the GPU, then from one checkpoint `steps` plain steps and `steps`
+delayed-acceptance steps (k surrogate steps per exact evaluation), both with
+all inputs resident in HBM and no records. value = the DA run's replica-steps
+per second of wall time; the plain figure, the speed-up and the best total
+energy each run reached are reported beside it. --mode plain times the plain
+engine alone (records written to HBM as the reference's kernel does).
 
-roofline: algorithmic FP32 work of the fused MC kernel, 36 flops per
+roofline: algorithmic FP32 work of the plain MC kernel, 36 flops per
 ligand-atom x receptor-point pair per replica-step (distance 10, soft LJ 13,
 far-field electrostatics 5, PMF 8; conditional terms not counted), divided by
 the kernel's mean duration from HIP events on the engine's stream; peak is
@@ -53,8 +57,9 @@ def parse():
     ap.add_argument("--autotune", type=int, default=-1,
                     help="tune launch shapes first (default: on for the batch workload)")
     ap.add_argument("--tune-budget", type=float, default=90.0, help="seconds")
-    ap.add_argument("--mode", default="plain", choices=["plain", "da"],
-                    help="plain engine only, or plain vs surrogate delayed acceptance")
+    ap.add_argument("--mode", default="da", choices=["plain", "da"],
+                    help="plain vs surrogate delayed acceptance (default), or plain engine only")
+    ap.add_argument("--no-record", action="store_true", help="plain mode: keep no records")
     ap.add_argument("--k-sub", type=int, default=4, help="surrogate steps per exact evaluation")
     ap.add_argument("--log-steps", type=int, default=1000, help="plain steps logged for training")
     ap.add_argument("--log-poses", type=int, default=400000)
@@ -165,7 +170,8 @@ def da_compare(args, cxs, desc):
     if args.steps % k or args.warmup % k or args.log_steps % k or args.nc % k:
         raise SystemExit("steps, warmup, log-steps and nc must be multiples of k-sub")
     n_rep = sum(c.n_rep for c in cxs)
-    eng = gd.Engine(seed=0, record=True, steps_per_dump=500, receptor_split=args.split)
+    opts, tuned = launch_options(args, cxs)
+    eng = gd.Engine(seed=0, record=True, steps_per_dump=500, **opts)
     for cx in cxs:
         eng.add_complex(cx)
     eng.run(args.warmup, args.nc)
@@ -185,7 +191,7 @@ def da_compare(args, cxs, desc):
     blob = eng.checkpoint()
     eng.close()
 
-    run = gd.Engine(seed=0, record=False, steps_per_dump=2000, receptor_split=args.split)
+    run = gd.Engine(seed=0, record=False, steps_per_dump=2000, **opts)
     for cx in cxs:
         run.add_complex(cx)
     run.restore(blob)
@@ -224,9 +230,16 @@ def da_compare(args, cxs, desc):
     step = math.lcm(k, args.nc) if args.nc > 0 else k
     eq = max(step, eq // step * step)
     out["da_equal_time"] = one("da", eq)
+    kname = kernel_names(run)
     run.close()
     speed = out["da"]["replica_steps_per_s"] / out["plain"]["replica_steps_per_s"]
     d_best = out["da"]["best_energy"] - out["plain"]["best_energy"]
+    pairs_per_step = sum(c.n_rep * c.lna * c.pnp for c in cxs)
+    roof = roofline(pairs_per_step, min(args.steps, 2000), out["plain"]["kernel_ms_avg"], kname,
+                    desc)
+    # the DA kernel's exact work: one pose evaluation per k surrogate steps
+    roof_da = roofline(pairs_per_step / k, min(args.steps, 2000), out["da"]["kernel_ms_avg"],
+                       kname.replace(", 0>", ", 2>"), None)
     return {
         "metric": "MC replica-steps/s, plain vs surrogate-DA, at matched best-pose energy",
         "value": out["da"]["replica_steps_per_s"],
@@ -237,6 +250,9 @@ def da_compare(args, cxs, desc):
         "data": "synthetic (seeded generator, geaxdock_gpu_amd/synth.py)",
         "config": dict(desc, replicas=n_rep, steps_per_exchange=args.nc, k_sub=k,
                        surrogate=args.sur_mode),
+        "roofline": roof,
+        "roofline_da_exact_part": roof_da,
+        "launch_options": opts, "autotune": tuned,
         "plain": out["plain"], "da": out["da"], "da_equal_time": out["da_equal_time"],
         "da_speedup": speed, "best_energy_delta": d_best,
         "target_met": bool(speed >= 3.0 and abs(d_best) <= 0.1),
@@ -248,6 +264,43 @@ def da_compare(args, cxs, desc):
                       "test_rmse_energy": fit["test_rmse_energy"],
                       "test_energy_std": fit["test_energy_std"]},
     }
+
+
+# HBM traffic of one 2,000-step plain MC launch on the ensemble workload
+# without records (rocprofv3 FETCH_SIZE x 2 and WRITE_SIZE x 1, the factors a
+# plain 256 MiB device copy gives); see profiles/gd_mc_kernel_apl3_ensemble_*
+TRAFFIC_ENSEMBLE_2000 = None
+
+
+def roofline(pairs_per_step, steps_per_launch, kernel_ms, kname, desc):
+    """FP32 VALU roofline of an MC launch: 36 flops per ligand-atom x
+    receptor-point pair, divided by the launch's HIP-event duration."""
+    flops = FLOPS_PER_PAIR * pairs_per_step * steps_per_launch
+    ach = flops / (kernel_ms * 1e-3) if kernel_ms > 0 else 0.0
+    traffic = None
+    if desc is not None and desc.get("workload", "").startswith("ensemble") and \
+            steps_per_launch == 2000:
+        traffic = TRAFFIC_ENSEMBLE_2000
+    return {"bound": "mfma", "unit": "TFLOP/s", "achieved": ach / 1e12, "peak": PEAK_FP32 / 1e12,
+            "frac": ach / PEAK_FP32, "frac_unpacked_fp32": ach / 78.6e12, "traffic": traffic,
+            "kernel": kname, "kernel_ms_avg": kernel_ms, "flops_per_launch": flops,
+            "note": "FP32 vector (VALU) bound; 36 flops per ligand-receptor pair; peak is the "
+                    "packed-FP32 157.3 TFLOP/s, frac_unpacked_fp32 uses 78.6"}
+
+
+def launch_options(args, cxs):
+    opts = dict(lanes_per_replica=args.lpr, atoms_per_lane=args.apl, receptor_split=args.split,
+                ladders_per_pack=args.ladders, launch=args.launch)
+    tuned = None
+    if args.autotune == 1 or (args.autotune == -1 and args.config == "batch"):
+        from geaxdock_gpu_amd import autotune
+        t0 = time.perf_counter()
+        best, trials = autotune.tune(cxs, nc=args.nc, budget_s=args.tune_budget)
+        opts.update(best)
+        tuned = {"best": best, "seconds": time.perf_counter() - t0,
+                 "trials": [{**t["opts"], "ms_per_step": (t["s_per_step"] or 0) * 1e3}
+                            for t in trials]}
+    return opts, tuned
 
 
 def main():
@@ -265,20 +318,17 @@ def main():
 
     cxs, desc = workload(args)
     if args.mode == "da":
-        print(json.dumps(da_compare(args, cxs, desc)))
+        out = da_compare(args, cxs, desc)
+        if not args.no_cpu:
+            try:
+                out["cpu_baseline"] = cpu_baseline(cxs, args)
+            except Exception as ex:  # the GPU figures stand without it
+                out["cpu_baseline"] = {"error": str(ex)}
+        print(json.dumps(out))
         return
-    opts = dict(lanes_per_replica=args.lpr, atoms_per_lane=args.apl, receptor_split=args.split,
-                ladders_per_pack=args.ladders, launch=args.launch)
-    tuned = None
-    if args.autotune == 1 or (args.autotune == -1 and args.config == "batch"):
-        from geaxdock_gpu_amd import autotune
-        t0 = time.perf_counter()
-        best, trials = autotune.tune(cxs, nc=args.nc, budget_s=args.tune_budget)
-        opts.update(best)
-        tuned = {"best": best, "seconds": time.perf_counter() - t0,
-                 "trials": [{**t["opts"], "ms_per_step": (t["s_per_step"] or 0) * 1e3}
-                            for t in trials]}
-    eng = gd.Engine(seed=0, record=True, steps_per_dump=2000, verbose=args.verbose, **opts)
+    opts, tuned = launch_options(args, cxs)
+    eng = gd.Engine(seed=0, record=not args.no_record, steps_per_dump=2000, verbose=args.verbose,
+                    **opts)
     for cx in cxs:
         eng.add_complex(cx)
     n_rep = sum(c.n_rep for c in cxs)
@@ -290,10 +340,8 @@ def main():
     value = n_rep * args.steps / dt
     pairs_per_step = sum(c.n_rep * c.lna * c.pnp for c in cxs)
     # dominant kernel: one launch = min(steps, 2000) steps of every replica
-    steps_per_launch = min(args.steps, 2000)
-    kernel_s = st["kernel_ms_avg"] * 1e-3
-    flops_launch = FLOPS_PER_PAIR * pairs_per_step * steps_per_launch
-    achieved = flops_launch / kernel_s if kernel_s > 0 else 0.0
+    roof = roofline(pairs_per_step, min(args.steps, 2000), st["kernel_ms_avg"], kernel_names(eng),
+                    desc if args.no_record else None)
     state_accept = st["accepted"] / max(1, st["replica_steps"])
     out = {
         "metric": "MC replica-steps/s (plain REMC engine)",
@@ -309,11 +357,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded generator, geaxdock_gpu_amd/synth.py)",
         "config": dict(desc, replicas=n_rep, steps_per_exchange=args.nc),
-        "roofline": {"bound": "mfma", "unit": "TFLOP/s", "achieved": achieved / 1e12,
-                     "peak": PEAK_FP32 / 1e12, "frac": achieved / PEAK_FP32, "traffic": None,
-                     "kernel": kernel_names(eng), "kernel_ms_avg": st["kernel_ms_avg"],
-                     "flops_per_launch": flops_launch,
-                     "note": "FP32 vector (VALU) bound; 36 flops per ligand-receptor pair"},
+        "roofline": roof,
         "launch_options": opts,
         "autotune": tuned,
         "acceptance_ratio": state_accept,

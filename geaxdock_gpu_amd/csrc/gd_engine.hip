@@ -1552,6 +1552,8 @@ gd_status sur_batch(gd_ctx *ctx, int64_t p0, int64_t p1, bool grad, double *loss
                        (const unsigned short *)S.dH2T, S.work_rows, nr, R, S.p2, S.pb2);
     hipLaunchKernelGGL(gd_mlp_gradsum_kernel, dim3((SUR_NPARAM + 255) / 256), dim3(256), 0,
                        ctx->stream, S.p1, nch, S.p2, nch, S.pb2, S.pw3, nr / 16, S.grad);
+    hipLaunchKernelGGL(gd_mlp_w3sum_kernel, dim3(SUR_H + 1), dim3(256), 0, ctx->stream, S.pw3,
+                       (int64_t)(nr / 16), S.grad);
   }
   HIPCHK(hipGetLastError());
   if (loss) {

@@ -469,10 +469,27 @@ extern "C" __global__ void gd_mlp_gradsum_kernel(const float *p1, int n1, const 
   } else if (q < SP_W3) {
     for (int c = 0; c < n2; ++c) s += pb2[(int64_t)c * SUR_H + (q - SP_B2)];
   } else {
-    const int j = q - SP_W3;  // w3[j] or b3 (j == 128)
-    for (int64_t w = 0; w < nw; ++w) s += pw3[w * (SUR_H + 1) + j];
+    return;  // w3 and b3: gd_mlp_w3sum_kernel
   }
   g[q] = s;
+}
+
+// w3[j] and b3 (j == 128) gradients from the per-wave partials: one
+// workgroup per output, strided partial sums then a fixed-shape tree
+// (deterministic).
+extern "C" __global__ void __launch_bounds__(256) gd_mlp_w3sum_kernel(const float *pw3, int64_t nw,
+                                                                      float *g) {
+  __shared__ float red[256];
+  const int j = blockIdx.x;
+  float s = 0.0f;
+  for (int64_t w = threadIdx.x; w < nw; w += 256) s += pw3[w * (SUR_H + 1) + j];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) g[SP_W3 + j] = red[0];
 }
 
 // Adam on the FP32 master parameters; refreshes the bf16 layouts.
