@@ -196,3 +196,30 @@ def test_surrogate_total_matches_oracle(gd, tab):
         worst = max(worst, abs(got[i] - want) / tol)
         assert abs(got[i] - want) <= tol, (i, got[i], want, tol)
     print(f"\ntab={tab}: worst |dS|/tol = {worst:.3f}")
+
+
+@pytest.mark.parametrize("tab", [False, True])
+def test_da_equals_plain_when_surrogate_is_exact(gd, tab):
+    """Receptor-pair weights zero: the exact energy is the surrogate's cheap
+    part (scale 0), every composite move is accepted, and the DA engine passes
+    through the plain engine's states every k steps, exchanges included
+    (rounding between the two code paths may split at most one replica)."""
+    w = [0.0] * 6 + [0.7, 0.4, 1.0, 0.0]
+    cx = synth.make_complex(seed=12, lna=12, pnp=300, n_lig=4, n_prt=2, n_tmp=4, box=20.0,
+                            void_radius=6.0, weights=w, t_floor=0.5, t_ceiling=5.0)
+    states = []
+    for da in (False, True):
+        eng, cid = _engine(gd, cx, 9, tab=tab, record=False)
+        if da:
+            eng.run_da(400, 20, 4, 0.0)
+        else:
+            eng.run(400, 20)
+        states.append(eng.state(cid))
+        eng.close()
+    p, q = states
+    same = np.all(p["movematrix"] == q["movematrix"], axis=1) & \
+        np.all(p["energy"]["e"] == q["energy"]["e"], axis=1)
+    print(f"\ntab={tab}: {same.sum()} of {len(same)} replicas identical; plain acceptance "
+          f"{p['acs_mc'].mean() / 400:.3f}")
+    assert same.sum() >= len(same) - 1, same
+    assert np.array_equal(p["replica"][:, 2], q["replica"][:, 2]) or same.sum() < len(same)

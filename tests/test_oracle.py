@@ -447,3 +447,38 @@ def test_cpu_mc_path_deterministic_and_consistent(oracle):
     e = oracle.energy_f32(d, ref, 0, 0, r["movematrix"])
     assert np.float32(e.e[9]) == r["energy"]["e"][9]
     assert sorted(a.idx_tmp.tolist()) == list(range(8))
+
+
+def test_da_chain_reduces_to_plain_when_surrogate_is_exact(oracle):
+    """Pins gdo_mc_run_da: with every receptor-pair weight zero the exact
+    energy equals the surrogate (the cheap terms), so every composite move is
+    accepted and the delayed-acceptance chain must pass through exactly the
+    plain chain's states every k steps (same uniforms per sub-step). A wrong
+    sign or a dropped surrogate correction in the composite test, or wrong
+    sub-step uniforms, breaks the equality."""
+    w = [0.0] * 6 + [0.7, 0.4, 1.0, 0.0]
+    cx = synth.make_complex(seed=12, lna=12, pnp=300, n_lig=4, n_prt=1, n_tmp=4, box=20.0,
+                            void_radius=6.0, weights=w, t_floor=0.5, t_ceiling=5.0)
+    d = cx.desc()
+    k, n = 4, 160
+    plain = oracle.McState(cx, rec_cap=n + 1)
+    oracle.mc_run(d, plain, seed=9, init=True, step0=0, n_steps=0, nc=0)
+    da = oracle.McState(cx, rec_cap=n + 1)
+    oracle.mc_run(d, da, seed=9, init=True, step0=0, n_steps=0, nc=0)
+    moved = 0
+    for s0 in range(0, n, k):
+        before = plain.mm.copy()
+        oracle.mc_run(d, plain, seed=9, init=False, step0=s0, n_steps=k, nc=0)
+        oracle.mc_run_da(d, da, seed=9, step0=s0, n_steps=k, nc=0, k_sub=k)
+        assert np.array_equal(plain.mm, da.mm), s0
+        assert np.array_equal(plain.e_old, da.e_old), s0
+        moved += int(np.any(plain.mm != before, axis=1).sum())
+    assert moved > n // k  # the chains did move
+    # a chain with exchanges: composite groups align with exchange segments
+    p2 = oracle.McState(cx, rec_cap=0)
+    d2 = oracle.McState(cx, rec_cap=0)
+    for st in (p2, d2):
+        oracle.mc_run(d, st, seed=3, init=True, step0=0, n_steps=0, nc=20)
+    oracle.mc_run(d, p2, seed=3, init=False, step0=0, n_steps=200, nc=20)
+    oracle.mc_run_da(d, d2, seed=3, step0=0, n_steps=200, nc=20, k_sub=k)
+    assert np.array_equal(p2.mm, d2.mm) and np.array_equal(p2.idx_tmp, d2.idx_tmp)
