@@ -62,7 +62,8 @@ def parse():
     ap.add_argument("--no-record", action="store_true", help="plain mode: keep no records")
     ap.add_argument("--k-sub", type=int, default=4, help="surrogate steps per exact evaluation")
     ap.add_argument("--log-steps", type=int, default=1000, help="plain steps logged for training")
-    ap.add_argument("--log-poses", type=int, default=400000)
+    ap.add_argument("--log-snapshots", type=int, default=50,
+                    help="state snapshots taken over the logged steps")
     ap.add_argument("--train-poses", type=int, default=200000)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--sur-mode", default="tab", choices=["tab", "direct"],
@@ -154,7 +155,7 @@ def kernel_names(eng):
 def da_compare(args, cxs, desc):
     """Plain vs delayed acceptance on the same complexes, seed and start state.
 
-    1. plain warmup, then log accepted states of a plain run (reservoir);
+    1. plain warmup, then snapshots of the replicas' states during a plain run;
     2. label training poses (logged states and states moved by 1-4
        proposals) with the exact energies, train the MLP on the GPU, report
        the held-out RMSE;
@@ -171,15 +172,15 @@ def da_compare(args, cxs, desc):
         raise SystemExit("steps, warmup, log-steps and nc must be multiples of k-sub")
     n_rep = sum(c.n_rep for c in cxs)
     opts, tuned = launch_options(args, cxs)
-    eng = gd.Engine(seed=0, record=True, steps_per_dump=500, **opts)
+    eng = gd.Engine(seed=0, record=False, steps_per_dump=2000, **opts)
     for cx in cxs:
         eng.add_complex(cx)
     eng.run(args.warmup, args.nc)
-    log = sg.PoseLog(args.log_poses, seed=0)
     t0 = time.perf_counter()
-    eng.run(args.log_steps, args.nc, sink=log)
+    # state snapshots every log_every steps over log_steps plain steps
+    every = max(args.nc, args.log_steps // max(1, args.log_snapshots))
+    recs, rcid = sg.log_states(eng, max(1, args.log_steps // every), every, args.nc)
     t_log = time.perf_counter() - t0
-    recs, rcid = log.records()
     t0 = time.perf_counter()
     eng.sur_build_grids(args.grid_spacing, args.grid_half)
     ts = sg.build_training_set(eng, recs, rcid, args.train_poses, seed=1)
@@ -202,12 +203,16 @@ def da_compare(args, cxs, desc):
         run.sur_tabulate()
     t_tab = time.perf_counter() - t0
 
-    def best_stats():
-        b = np.concatenate([run.best(c)[0] for c in range(len(cxs))]).astype(np.float64)
+    per_cx = {}
+
+    def best_stats(tag):
+        bs = [run.best(c)[0].astype(np.float64) for c in range(len(cxs))]
+        per_cx[tag] = np.array([b.min() for b in bs])
+        b = np.concatenate(bs)
         return {"best_energy": float(b.min()), "best_p01": float(np.percentile(b, 1)),
                 "replica_best_mean": float(b.mean())}
 
-    def one(mode, steps):
+    def one(mode, steps, tag):
         run.restore(blob)
         run.reset_best()
         t0 = time.perf_counter()
@@ -221,19 +226,25 @@ def da_compare(args, cxs, desc):
              "accepted_per_exact_eval": st["accepted"] / (st["replica_steps"] / (k if mode == "da" else 1)),
              "accepted_per_step": st["accepted"] / st["replica_steps"],
              "kernel_ms_avg": st["kernel_ms_avg"], "seconds": dt}
-        r.update(best_stats())
+        r.update(best_stats(tag))
         return r
 
-    out = {"plain": one("plain", args.steps), "da": one("da", args.steps)}
+    out = {"plain": one("plain", args.steps, "plain"), "da": one("da", args.steps, "da")}
     # the same wall-clock budget as the plain run
     eq = int(args.steps * out["da"]["replica_steps_per_s"] / out["plain"]["replica_steps_per_s"])
     step = math.lcm(k, args.nc) if args.nc > 0 else k
     eq = max(step, eq // step * step)
-    out["da_equal_time"] = one("da", eq)
+    out["da_equal_time"] = one("da", eq, "da_eq")
     kname = kernel_names(run)
     run.close()
     speed = out["da"]["replica_steps_per_s"] / out["plain"]["replica_steps_per_s"]
-    d_best = out["da"]["best_energy"] - out["plain"]["best_energy"]
+    # best-pose energy per complex (energies of different complexes do not compare)
+    dcx = per_cx["da"] - per_cx["plain"]
+    d_best = float(dcx.mean()) if len(cxs) > 1 else float(dcx[0])
+    best_cmp = {"complexes": len(cxs), "delta_mean": float(dcx.mean()),
+                "delta_max": float(dcx.max()), "delta_min": float(dcx.min()),
+                "within_0p1": float(np.mean(np.abs(dcx) <= 0.1)),
+                "delta_equal_time_mean": float((per_cx["da_eq"] - per_cx["plain"]).mean())}
     pairs_per_step = sum(c.n_rep * c.lna * c.pnp for c in cxs)
     roof = roofline(pairs_per_step, min(args.steps, 2000), out["plain"]["kernel_ms_avg"], kname,
                     desc)
@@ -254,7 +265,7 @@ def da_compare(args, cxs, desc):
         "roofline_da_exact_part": roof_da,
         "launch_options": opts, "autotune": tuned,
         "plain": out["plain"], "da": out["da"], "da_equal_time": out["da_equal_time"],
-        "da_speedup": speed, "best_energy_delta": d_best,
+        "da_speedup": speed, "best_energy_delta": d_best, "best_energy_per_complex": best_cmp,
         "target_met": bool(speed >= 3.0 and abs(d_best) <= 0.1),
         "surrogate": {"log_seconds": t_log, "label_seconds": t_data, "sigma": sigma,
                       "tabulate_seconds": t_tab,

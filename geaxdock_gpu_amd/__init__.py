@@ -83,6 +83,9 @@ def load_library():
     lib.gd_sur_dataset_reserve.argtypes = [vp, C.c_int64]
     lib.gd_sur_dataset_add.argtypes = [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_int),
                                        P(C.c_float), P(C.c_float)]
+    lib.gd_sur_dataset_add_multi.argtypes = [vp, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int),
+                                             P(C.c_float), P(C.c_float)]
+    lib.gd_sur_dataset_add_multi.restype = C.c_int
     lib.gd_sur_dataset_size.argtypes = [vp]
     lib.gd_sur_dataset_size.restype = C.c_int64
     lib.gd_sur_loss_grad.argtypes = [vp, C.c_int, C.c_int, P(C.c_double), P(C.c_float)]
@@ -322,6 +325,16 @@ class Engine:
         t = np.ascontiguousarray(target, np.float32)
         self._check(self.lib.gd_sur_dataset_add(self._ctx, cid, len(mm), _i(il), _i(ip), _f(mm),
                                                 _f(t)))
+
+    def sur_dataset_add_multi(self, cids, idx_lig, idx_prt, movematrix, target):
+        """Append poses of several complexes in the given order."""
+        cc = np.ascontiguousarray(cids, np.int32)
+        il = np.ascontiguousarray(idx_lig, np.int32)
+        ip = np.ascontiguousarray(idx_prt, np.int32)
+        mm = np.ascontiguousarray(movematrix, np.float32).reshape(-1, 6)
+        t = np.ascontiguousarray(target, np.float32)
+        self._check(self.lib.gd_sur_dataset_add_multi(self._ctx, len(mm), _i(cc), _i(il), _i(ip),
+                                                      _f(mm), _f(t)))
 
     def sur_dataset_size(self):
         return self.lib.gd_sur_dataset_size(self._ctx)

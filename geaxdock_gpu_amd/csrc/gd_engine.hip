@@ -1808,6 +1808,56 @@ gd_status gd_sur_dataset_add(gd_ctx *ctx, int cid, int n, const int *idx_lig, co
   return GD_OK;
 }
 
+gd_status gd_sur_dataset_add_multi(gd_ctx *ctx, int n, const int *cid, const int *idx_lig,
+                                   const int *idx_prt, const float *movematrix,
+                                   const float *target) {
+  if (!ctx || n < 0) return GD_ERR_INVALID;
+  if (n == 0) return GD_OK;
+  if (!cid || !idx_lig || !idx_prt || !movematrix || !target) return GD_ERR_INVALID;
+  auto &S = ctx->sur;
+  if (S.g.size() != ctx->cx.size()) return fail(ctx, GD_ERR_STATE, "call gd_sur_build_grids first");
+  std::vector<int64_t> row_of(n);
+  int64_t add = 0;
+  for (int i = 0; i < n; ++i) {
+    if (cid[i] < 0 || cid[i] >= (int)ctx->cx.size())
+      return fail(ctx, GD_ERR_INVALID, "pose %d: complex id out of range", i);
+    const HostComplex &h = ctx->cx[cid[i]];
+    if (idx_lig[i] < 0 || idx_lig[i] >= h.d.n_lig || idx_prt[i] < 0 || idx_prt[i] >= h.d.n_prt)
+      return fail(ctx, GD_ERR_INVALID, "pose %d: conformer index out of range", i);
+    row_of[i] = S.rows + add;
+    add += h.d.lna;
+  }
+  if (S.rows + add > S.cap) TRY(gd_sur_dataset_reserve(ctx, std::max<int64_t>(S.rows + add, 2 * S.cap)));
+  HIPCHK(hipSetDevice(ctx->device));
+  TRY(sync_complexes(ctx));
+  std::vector<void *> keep;
+  int *dc, *dl, *dp;
+  float *dm;
+  int64_t *dr;
+  gd_status s = GD_OK;
+  if ((s = tmp_dev(ctx, keep, &dc, cid, (size_t)n)) || (s = tmp_dev(ctx, keep, &dl, idx_lig, (size_t)n)) ||
+      (s = tmp_dev(ctx, keep, &dp, idx_prt, (size_t)n)) ||
+      (s = tmp_dev(ctx, keep, &dm, movematrix, (size_t)n * 6)) ||
+      (s = tmp_dev(ctx, keep, &dr, row_of.data(), (size_t)n))) {
+    free_all(keep);
+    return s;
+  }
+  hipLaunchKernelGGL(gd_sur_feat_multi_kernel, dim3((unsigned)n), dim3(64), 0, ctx->stream,
+                     ctx->d_cx, n, dc, dl, dp, dm, dr, S.X, S.XT, S.cap + 64);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  free_all(keep);
+  if (e != hipSuccess) return fail(ctx, GD_ERR_HIP, "gd_sur_dataset_add_multi: %s", hipGetErrorString(e));
+  for (int i = 0; i < n; ++i) {
+    S.off.push_back(S.off.back() + ctx->cx[cid[i]].d.lna);
+    S.target.push_back(target[i]);
+    S.pose_cx.push_back(cid[i]);
+  }
+  S.rows += add;
+  S.dirty = true;
+  return GD_OK;
+}
+
 int64_t gd_sur_dataset_size(const gd_ctx *ctx) { return ctx ? (int64_t)ctx->sur.target.size() : -1; }
 
 gd_status gd_sur_loss_grad(gd_ctx *ctx, int pose_begin, int pose_end, double *loss, float *grad) {

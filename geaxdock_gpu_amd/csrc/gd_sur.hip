@@ -137,29 +137,22 @@ __device__ inline void grid_radial(const SurGrid &G, int prt, float x, float y, 
   }
 }
 
-// Feature rows of poses: one thread per ligand atom of each pose. Writes
-// bf16 rows X [row][64] and, when xt != null, the transpose XT [64][ld].
-extern "C" __global__ void gd_sur_feat_kernel(const DevComplex *cxs, int cxi, SurGrid G, int n,
-                                              const int *idx_lig, const int *idx_prt,
-                                              const float *mm, int64_t row0,
-                                              unsigned short *X, unsigned short *XT,
-                                              int64_t ld, float *xf) {
-  const DevComplex &C = cxs[cxi];
+// Feature row of ligand atom l of one pose (movematrix m, ligand conformer
+// il, protein conformer ip): bf16 into X [row][64] and, when xt != null, the
+// transpose XT [64][ld]; FP32 features into xf when given.
+__device__ inline void feat_row(const DevComplex &C, const SurGrid &G, const float *m, int il,
+                                int ip, int l, int64_t row, unsigned short *X,
+                                unsigned short *XT, int64_t ld, float *xf) {
   const int lna = C.lna;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)n * lna) return;
-  const int pose = (int)(t / lna), l = (int)(t % lna);
-  const float *m = mm + (size_t)pose * 6;
   const Rot R = euler(m[3], m[4], m[5]);
-  const float *orig = C.lig_orig + (size_t)idx_lig[pose] * 3 * lna;
+  const float *orig = C.lig_orig + (size_t)il * 3 * lna;
   const float ox = orig[l], oy = orig[lna + l], oz = orig[2 * lna + l];
   const float x = R.r00 * ox + R.r01 * oy + R.r02 * oz + m[0] + C.pocket[0];
   const float y = R.r10 * ox + R.r11 * oy + R.r12 * oz + m[1] + C.pocket[1];
   const float z = R.r20 * ox + R.r21 * oy + R.r22 * oz + m[2] + C.pocket[2];
   float f[32];
-  grid_radial(G, idx_prt[pose], x, y, z, f);
+  grid_radial(G, ip, x, y, z, f);
   const int lt = C.lig_t[l];
-  const int64_t row = row0 + t;
   float v[SUR_F];
 #pragma unroll
   for (int q = 0; q < SUR_F; ++q) v[q] = 0.0f;
@@ -176,7 +169,45 @@ extern "C" __global__ void gd_sur_feat_kernel(const DevComplex *cxs, int cxi, Su
     for (int q = 0; q < SUR_F; ++q) XT[(int64_t)q * ld + row] = f2bf(v[q]);
   if (xf)
 #pragma unroll
-    for (int q = 0; q < SUR_NF; ++q) xf[(size_t)t * SUR_NF + q] = v[q];
+    for (int q = 0; q < SUR_NF; ++q) xf[q] = v[q];
+}
+
+// Feature rows of poses of one complex: one thread per ligand atom of each
+// pose, rows from row0 on.
+extern "C" __global__ void gd_sur_feat_kernel(const DevComplex *cxs, int cxi, SurGrid G, int n,
+                                              const int *idx_lig, const int *idx_prt,
+                                              const float *mm, int64_t row0,
+                                              unsigned short *X, unsigned short *XT,
+                                              int64_t ld, float *xf) {
+  const DevComplex &C = cxs[cxi];
+  const int lna = C.lna;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * lna) return;
+  const int pose = (int)(t / lna), l = (int)(t % lna);
+  feat_row(C, G, mm + (size_t)pose * 6, idx_lig[pose], idx_prt[pose], l, row0 + t, X, XT, ld,
+           xf ? xf + (size_t)t * SUR_NF : nullptr);
+}
+
+// Feature rows of poses of many complexes: one workgroup per pose, its rows
+// from row_of[pose] on; the grid geometry comes from the complex record.
+extern "C" __global__ void gd_sur_feat_multi_kernel(const DevComplex *cxs, int n, const int *cid,
+                                                    const int *idx_lig, const int *idx_prt,
+                                                    const float *mm, const int64_t *row_of,
+                                                    unsigned short *X, unsigned short *XT,
+                                                    int64_t ld) {
+  const int pose = blockIdx.x;
+  if (pose >= n) return;
+  const DevComplex &C = cxs[cid[pose]];
+  SurGrid G;
+  G.grid = C.sgrid;
+  G.ox = C.sox;
+  G.oy = C.soy;
+  G.oz = C.soz;
+  G.h = C.sh;
+  G.n = C.sn;
+  for (int l = threadIdx.x; l < C.lna; l += blockDim.x)
+    feat_row(C, G, mm + (size_t)pose * 6, idx_lig[pose], idx_prt[pose], l, row_of[pose] + l, X, XT,
+             ld, nullptr);
 }
 
 // bf16 weight copies in the layouts the fragments read:

@@ -15,6 +15,8 @@ zero; then W2 [128][128], b2 [128], w3 [128], b3 [1].
 """
 import numpy as np
 
+from . import abi
+
 N_FEAT = 56
 N_IN = 64
 N_HID = 128
@@ -132,6 +134,25 @@ class PoseLog:
         return self.buf[:self.fill], self.cid[:self.fill]
 
 
+def log_states(eng, n_snapshots, every, nc=20, da=None):
+    """Accepted states of a plain run: after every `every` steps, each
+    replica's current pose and its exact energy (a record array with the
+    gd_record layout, and the complex of each entry). Memory is bounded by
+    n_snapshots x replicas, whatever the run length."""
+    out, cids = [], []
+    for _ in range(n_snapshots):
+        eng.run(every, nc)
+        for c in range(len(eng.complexes)):
+            st = eng.state(c)
+            r = np.zeros(len(st["movematrix"]), abi.RECORD_DTYPE)
+            r["replica"] = st["replica"]
+            r["energy"] = st["energy"]
+            r["movematrix"] = st["movematrix"]
+            out.append(r)
+            cids.append(np.full(len(r), c, np.int32))
+    return np.concatenate(out), np.concatenate(cids)
+
+
 def build_training_set(eng, recs, rec_cid, n_poses, perturbed=0.5, max_moves=4, seed=0):
     """Poses for training: logged accepted states, and the same states moved
     by 1..max_moves proposals (the surrogate ranks proposals). Labels are the
@@ -176,18 +197,8 @@ def load_training_set(eng, ts, clip_pct=99.5):
     cap = np.percentile(t, clip_pct)
     t = np.minimum(t, cap).astype(np.float32)
     ts["t"] = t
-    eng.sur_dataset_reserve(int(eng.sur_dataset_size() + lna.sum() + 64 * len(lna)))
-    # consecutive runs of one complex go in one call, keeping the pose order
-    cid = ts["cid"]
-    b = 0
-    n = len(cid)
-    while b < n:
-        e = b
-        while e < n and cid[e] == cid[b]:
-            e += 1
-        eng.sur_dataset_add(int(cid[b]), ts["idx_lig"][b:e], ts["idx_prt"][b:e],
-                            ts["movematrix"][b:e], t[b:e])
-        b = e
+    eng.sur_dataset_reserve(int(eng.sur_dataset_size() + lna.sum() + 64))
+    eng.sur_dataset_add_multi(ts["cid"], ts["idx_lig"], ts["idx_prt"], ts["movematrix"], t)
     return m, sigma
 
 

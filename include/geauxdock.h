@@ -290,6 +290,10 @@ gd_status gd_sur_dataset_clear(gd_ctx *ctx);
 gd_status gd_sur_dataset_reserve(gd_ctx *ctx, int64_t rows);
 gd_status gd_sur_dataset_add(gd_ctx *ctx, int complex_id, int n, const int *idx_lig,
                              const int *idx_prt, const float *movematrix, const float *target);
+/* The same for poses of several complexes, in the given order (complex_id[n]). */
+gd_status gd_sur_dataset_add_multi(gd_ctx *ctx, int n, const int *complex_id, const int *idx_lig,
+                                   const int *idx_prt, const float *movematrix,
+                                   const float *target);
 int64_t gd_sur_dataset_size(const gd_ctx *ctx);
 /* Mean squared error over poses [pose_begin, pose_end) and its gradient. */
 gd_status gd_sur_loss_grad(gd_ctx *ctx, int pose_begin, int pose_end, double *loss, float *grad);

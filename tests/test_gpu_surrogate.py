@@ -129,3 +129,36 @@ def test_training_reduces_loss(setup):
     assert hist[-1] < 0.35 * l0
     pred = eng.sur_predict(0, n)
     assert np.mean((pred - target) ** 2) == pytest.approx(hist[-1], rel=1e-3)
+
+
+def test_dataset_add_multi_matches_per_complex_adds():
+    """Poses of several complexes appended in one interleaved call give the
+    same rows (hence predictions) as per-complex appends."""
+    import geaxdock_gpu_amd as gd
+    cxs = synth.config_batch(2, seed=8, lna_range=(12, 30), pnp_range=(800, 1200))
+    rng = np.random.default_rng(3)
+    n = 40
+    cid = rng.integers(0, 2, n).astype(np.int32)
+    il = np.array([rng.integers(0, cxs[c].n_lig) for c in cid], np.int32)
+    ip = np.array([rng.integers(0, cxs[c].n_prt) for c in cid], np.int32)
+    mm = poses(rng, n, sigma=1.0)
+    t = rng.normal(0, 1, n).astype(np.float32)
+    preds = []
+    for multi in (True, False):
+        with gd.Engine(seed=1) as eng:
+            for cx in cxs:
+                eng.add_complex(cx)
+            eng.sur_build_grids(0.8, 9.6)
+            eng.sur_set_params(to_gpu_params(so.init_params(2)))
+            if multi:
+                eng.sur_dataset_add_multi(cid, il, ip, mm, t)
+                preds.append(eng.sur_predict(0, n))
+            else:
+                order = np.concatenate([np.nonzero(cid == c)[0] for c in (0, 1)])
+                for c in (0, 1):
+                    sel = np.nonzero(cid == c)[0]
+                    eng.sur_dataset_add(c, il[sel], ip[sel], mm[sel], t[sel])
+                p = np.zeros(n, np.float32)
+                p[order] = eng.sur_predict(0, n)
+                preds.append(p)
+    np.testing.assert_array_equal(preds[0], preds[1])
