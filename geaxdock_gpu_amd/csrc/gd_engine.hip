@@ -326,36 +326,16 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
   if (h.lpr * h.apl < lna || h.apl < 1 || h.apl > 4)
     return fail(ctx, GD_ERR_INVALID, "no lane shape for lna %d (lpr %d apl %d)", lna, h.lpr, h.apl);
 
-  // receptor sorted by (type, kind, distance from the pocket centre in
-  // conformer 0); kind: 2 = class 2, 1 = hydrophobic flag. The distance
-  // order lets the kernel cut each chunk into near / mid / far ranges.
+  // receptor sorted by (type, kind); kind: 2 = class 2, 1 = hydrophobic flag
   std::vector<int> kind(pnp), perm(pnp);
-  std::vector<double> rad0(pnp);
   for (int p = 0; p < pnp; ++p) {
     kind[p] = d->prt_c[p] == 2 ? KIND_PSP : (d->prt_c0_and_d12_or_c2[p] == 1 ? KIND_HPC : KIND_PLAIN);
     perm[p] = p;
-    const double dx = (double)d->prt_x[p] - d->pocket_center[0];
-    const double dy = (double)d->prt_y[p] - d->pocket_center[1];
-    const double dz = (double)d->prt_z[p] - d->pocket_center[2];
-    rad0[p] = std::sqrt(dx * dx + dy * dy + dz * dz);
   }
   std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
     if (d->prt_t[a] != d->prt_t[b]) return d->prt_t[a] < d->prt_t[b];
-    if (kind[a] != kind[b]) return kind[a] < kind[b];
-    return rad0[a] < rad0[b];
+    return kind[a] < kind[b];
   });
-  // other conformers move points by at most rad_slack in distance
-  double slack = 0.0;
-  for (int c = 1; c < d->n_prt; ++c)
-    for (int p = 0; p < pnp; ++p) {
-      const size_t o = (size_t)c * pnp + p;
-      const double dx = (double)d->prt_x[o] - d->pocket_center[0];
-      const double dy = (double)d->prt_y[o] - d->pocket_center[1];
-      const double dz = (double)d->prt_z[o] - d->pocket_center[2];
-      slack = std::max(slack, std::fabs(std::sqrt(dx * dx + dy * dy + dz * dz) - rad0[p]));
-    }
-  std::vector<float> prad(pnp);
-  for (int i = 0; i < pnp; ++i) prad[i] = (float)rad0[perm[i]];
   std::vector<float4> xyze((size_t)d->n_prt * pnp);
   for (int c = 0; c < d->n_prt; ++c)
     for (int i = 0; i < pnp; ++i) {
@@ -381,33 +361,6 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
     i = j;
   }
   h.n_chunks = (int)chunks.size();
-
-  // per-chunk radii over the ligand's atom types: contacts and the near
-  // electrostatic form lie within rn of an atom; beyond rf the PMF
-  // correction underflows (1 + e rounds to 1 for e <= 2^-25) and the H-bond
-  // Gaussian is below 2^-60
-  std::vector<float2> crad(h.n_chunks);
-  {
-    const gd_enepara &ep0 = *d->enepara;
-    std::vector<bool> has(NLT, false);
-    for (int l = 0; l < lna; ++l) has[d->lig_t[l]] = true;
-    const double hs = std::sqrt(0.5 * 1.4426950408889634);
-    for (int c = 0; c < h.n_chunks; ++c) {
-      const int pt = chunks[c].z;
-      double rn = 0.0, rf = 0.0;
-      for (int lt = 0; lt < NLT; ++lt) {
-        if (!has[lt]) continue;
-        const double pmf0 = ep0.pmf0[lt][pt];
-        rn = std::max(rn, std::max(pmf0, 1.0 / (double)ep0.el1));
-        const double u = 0.5 * pmf0 - 6.0;
-        double f = (6.0 + 0.5 * pmf0) + std::sqrt(u * u + 50.0 * std::log(2.0));
-        const double h0 = ep0.hdb0[lt][pt], h1 = ep0.hdb1[lt][pt];
-        if (h0 > 0.1 && h1 != 0.0) f = std::max(f, h0 + std::sqrt(60.0) / (std::fabs(h1) * hs));
-        rf = std::max(rf, f);
-      }
-      crad[c] = make_float2((float)(rn * 1.001 + 1e-3), (float)(rf * 1.001 + 1e-3));
-    }
-  }
 
   // type-pair parameter records [receptor type][ligand type]
   const gd_enepara &ep = *d->enepara;
@@ -477,7 +430,6 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
   D.n_prt = d->n_prt;
   D.n_tmp = d->n_tmp;
   D.n_chunks = h.n_chunks;
-  D.rad_slack = (float)(slack * 1.001 + 1e-3);
   D.lpr = h.lpr;
   D.apl = h.apl;
   D.move_kind = d->move_kind;
@@ -522,8 +474,6 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
     TRY(dev_upload(ctx, h, (int **)&D.prt_cls, cls.data(), cls.size()));
   }
   TRY(dev_upload(ctx, h, (int4 **)&D.chunks, chunks.data(), chunks.size()));
-  TRY(dev_upload(ctx, h, (float2 **)&D.chunk_rad, crad.data(), crad.size()));
-  TRY(dev_upload(ctx, h, (float **)&D.prt_rad, prad.data(), prad.size()));
   TRY(dev_upload(ctx, h, (float **)&D.params, par.data(), par.size()));
   TRY(dev_upload(ctx, h, (float **)&D.lig_orig, orig.data(), orig.size()));
   TRY(dev_upload(ctx, h, (int **)&D.lig_t, d->lig_t, (size_t)lna));

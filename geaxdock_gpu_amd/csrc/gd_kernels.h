@@ -39,9 +39,6 @@ struct DevComplex {
   const float *prt_hpp;    // [pnp] hpp where c0_and_d12_or_c2, else 0
   const int *prt_psp;      // [pnp] seq3r where class 2, else -1
   const int4 *chunks;      // [n_chunks] start, length, type, kind
-  const float2 *chunk_rad; // [n_chunks] near radius, far radius (over the ligand's types)
-  const float *prt_rad;    // [pnp] distance from the pocket centre in conformer 0 (sorted order)
-  float rad_slack;         // bound on how far other conformers move a point's distance
   const float *params;     // [NPT * NLT * NPAR]
   const float *lig_orig;   // [n_lig][3][lna]
   const int *lig_t;        // [lna]

@@ -66,19 +66,9 @@ struct PairConst {
 // Per-lane parameters of one chunk (one receptor type) for one ligand atom.
 struct ChunkPar {
   float p1a, p2a, pmf0, pmf1, pb, pc, ha, hb, hc;
+  float nr2;  // contacts and the near electrostatic form lie within sqrt(nr2)
+  float rf2;  // beyond sqrt(rf2) PMF equals pmf1 exactly and H-bond < 1e-18
 };
-
-// Radial regions of a chunk. Points inside a chunk are sorted by their
-// distance from the pocket centre; for the wave's current poses the host's
-// per-chunk radii split each chunk into three ranges:
-//   NEAR  some atom of the wave may be within contact range (d <= pmf0) or
-//         within the near electrostatic form (el1 d < 1): every term;
-//   MID   no contact is possible: vdW, far electrostatics, PMF, H-bond, HPC;
-//   FAR   every atom is beyond the chunk's far radius, where
-//         1 + exp2(PMF exponent) rounds to 1 (PMF = pmf1 exactly, counted
-//         per chunk) and the H-bond Gaussian is below 1e-18: vdW and
-//         electrostatics only.
-enum { REG_NEAR = 0, REG_MID = 1, REG_FAR = 2 };
 
 // Receptor points [p, p + NP) of one chunk against the lane's APL atoms.
 // All terms of src/kernel_cuda_l2_calcenergy.cu:105-184 plus the contact
@@ -94,27 +84,32 @@ enum { REG_NEAR = 0, REG_MID = 1, REG_FAR = 2 };
 //   hpc  hpp * switch(d^2) for d^2 <= 81 (KIND >= 1);
 //   psp  psp[lig_t][seq3r] when class 2 and d <= pmf0 (KIND == 2);
 //   contacts  d <= pmf0.
-// The region fixes at compile time which terms are evaluated, so the loop
-// body has no branch.
-template <int APL, int KIND, int NP, int REG>
-__device__ __forceinline__ void pair_group(const Lds &S, int p, const PairConst &K,
+// Two wave-uniform branches skip work no lane of the wave needs: when every
+// lane is beyond its far radius the exponentials are not evaluated (1 + e
+// rounds to 1 there, so PMF is pmf1 bit for bit); contacts, the near
+// electrostatic form and PSP run only when some lane is within contact range.
+template <int APL, int KIND, int NP>
+__device__ __forceinline__ int pair_group(const Lds &S, int p, const PairConst &K,
                                            const float (&x)[APL], const float (&y)[APL],
                                            const float (&z)[APL], const ChunkPar (&cp)[APL],
                                            const float *const (&psprow)[APL],
                                            float (&vdw)[APL], float (&ele)[APL],
                                            float (&pmf)[APL], float (&psp)[APL],
                                            float (&hdb)[APL], float (&hpc)[APL],
-                                           int (&cnt)[APL]) {
+                                           int (&cnt)[APL], float (&vdwf)[APL]) {
   const float pa = -0.5f * LOG2E;
+  int nfar = 0;
   float4 P[NP];
   float hp[NP];
   int ps[NP];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     P[j] = S.xyze[p + j];
-    hp[j] = (KIND >= KIND_HPC && REG != REG_FAR) ? S.hpp[p + j] : 0.0f;
-    ps[j] = (KIND == KIND_PSP && REG == REG_NEAR) ? S.psp[p + j] : 0;
+    hp[j] = (KIND >= KIND_HPC) ? S.hpp[p + j] : 0.0f;
+    ps[j] = (KIND == KIND_PSP) ? S.psp[p + j] : 0;
   }
+  float d2[NP][APL], rd[NP][APL], dd[NP][APL], num[NP][APL], den[NP][APL];
+  bool mid = false, near = false;
 #pragma unroll
   for (int j = 0; j < NP; ++j)
 #pragma unroll
@@ -122,65 +117,67 @@ __device__ __forceinline__ void pair_group(const Lds &S, int p, const PairConst 
       const float q2 = pair_d2(x[k], y[k], z[k], P[j].x, P[j].y, P[j].z);
       const float r = fast_rsq(q2);
       const float d = q2 * r;
+      d2[j][k] = q2;
+      rd[j][k] = r;
+      dd[j][k] = d;
+      mid |= q2 < cp[k].rf2;
+      near |= q2 <= cp[k].nr2;
       const float d3 = q2 * d;
       const float d6 = d3 * d3;
       const float d9 = d6 * d3;
-      const float num = fmaf(-cp[k].p2a, d3, cp[k].p1a);
-      const float den = fmaf(cp[k].p1a, fmaf(K.lj01, q2, K.lj0), d9);
+      num[j][k] = fmaf(-cp[k].p2a, d3, cp[k].p1a);
+      den[j][k] = fmaf(cp[k].p1a, fmaf(K.lj01, q2, K.lj0), d9);
       ele[k] = fmaf(P[j].w, r, ele[k]);
-      if constexpr (REG == REG_FAR) {
-        vdw[k] = fmaf(num, fast_rcp(den), vdw[k]);
-      } else {
+    }
+  if (__ballot(mid) != 0) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+#pragma unroll
+      for (int k = 0; k < APL; ++k) {
+        const float q2 = d2[j][k], d = dd[j][k];
         const float wv = fminf(fmaf(pa, q2, fmaf(cp[k].pb, d, cp[k].pc)), 64.0f);
         const float e1 = 1.0f + fast_exp2(wv);
-        const float rr = fast_rcp(den * e1);
-        vdw[k] = fmaf(num * e1, rr, vdw[k]);
-        pmf[k] = fmaf(cp[k].pmf1 * den, rr, pmf[k]);
+        const float r = fast_rcp(den[j][k] * e1);
+        vdw[k] = fmaf(num[j][k] * e1, r, vdw[k]);
+        pmf[k] = fmaf(cp[k].pmf1 * den[j][k], r, pmf[k]);
         const float h = fmaf(d, cp[k].ha, cp[k].hb);
         hdb[k] = fmaf(cp[k].hc, fast_exp2(-(h * h)), hdb[k]);
-        if constexpr (KIND >= KIND_HPC) {
+        if (KIND >= KIND_HPC) {
           const float poly = fmaf(q2, fmaf(q2, fmaf(q2, fmaf(q2, H4, H3), H2), H1), 1.0f);
           hpc[k] += (q2 <= 81.0f) ? hp[j] * poly : 0.0f;
         }
       }
-      if constexpr (REG == REG_NEAR) {
+  } else {
+    // far field: PMF contributes pmf1 per point, added once per chunk
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+#pragma unroll
+      for (int k = 0; k < APL; ++k) vdwf[k] = fmaf(num[j][k], fast_rcp(den[j][k]), vdwf[k]);
+    nfar = NP;
+  }
+  if (__ballot(near) != 0) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+#pragma unroll
+      for (int k = 0; k < APL; ++k) {
+        const float d = dd[j][k];
         const bool c = d <= cp[k].pmf0;
         cnt[k] += c ? 1 : 0;
-        const float sv = K.el1 * d;
-        if (sv < 1.0f) {
-          const float gn = fmaf(sv * sv, fmaf(K.b1, sv, K.a1), K.el0);
-          ele[k] = fmaf(P[j].w, fmaf(K.el1, gn, -r), ele[k]);
+        const float s = K.el1 * d;
+        if (s < 1.0f) {
+          const float gn = fmaf(s * s, fmaf(K.b1, s, K.a1), K.el0);
+          ele[k] = fmaf(P[j].w, fmaf(K.el1, gn, -rd[j][k]), ele[k]);
         }
-        if constexpr (KIND == KIND_PSP) {
+        if (KIND == KIND_PSP) {
           if (c) psp[k] += psprow[k][ps[j]];
         }
       }
-    }
+  }
+  return nfar;
 }
 
-// Points [beg, end) of one region of a chunk.
-template <int APL, int KIND, int REG>
-__device__ __forceinline__ void region_pairs(const Lds &S, int beg, int end, const PairConst &K,
-                                             const float (&x)[APL], const float (&y)[APL],
-                                             const float (&z)[APL], const ChunkPar (&cp)[APL],
-                                             const float *const (&psprow)[APL],
-                                             float (&vdw)[APL], float (&ele)[APL],
-                                             float (&pmf)[APL], float (&psp)[APL],
-                                             float (&hdb)[APL], float (&hpc)[APL],
-                                             int (&cnt)[APL]) {
-  // points per group: enough independent pairs per lane to cover latency
-  constexpr int NP = APL >= 3 ? 1 : (APL == 2 ? 2 : 4);
-  int p = beg;
-  for (; p + NP <= end; p += NP)
-    pair_group<APL, KIND, NP, REG>(S, p, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc, cnt);
-  for (; p < end; ++p)
-    pair_group<APL, KIND, 1, REG>(S, p, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc, cnt);
-}
-
-// One chunk: the three regions, each shared among the receptor-split groups.
 template <int APL, int KIND>
-__device__ __forceinline__ void chunk_pairs(const Lds &S, const Split &sp, int beg, int i_near,
-                                            int i_far, int end, const PairConst &K,
+__device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, const PairConst &K,
                                             const float (&x)[APL], const float (&y)[APL],
                                             const float (&z)[APL], const ChunkPar (&cp)[APL],
                                             const float *const (&psprow)[APL],
@@ -188,40 +185,26 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, const Split &sp, int b
                                             float (&pmf)[APL], float (&psp)[APL],
                                             float (&hdb)[APL], float (&hpc)[APL],
                                             int (&cnt)[APL]) {
-  auto part = [&](int a, int b, int &pb, int &pe) {
-    pb = a + ((b - a) * sp.h) / sp.ns;
-    pe = a + ((b - a) * (sp.h + 1)) / sp.ns;
-  };
-  int pb, pe;
-  part(beg, i_near, pb, pe);
-  region_pairs<APL, KIND, REG_NEAR>(S, pb, pe, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
-                                    hpc, cnt);
-  part(i_near, i_far, pb, pe);
-  region_pairs<APL, KIND, REG_MID>(S, pb, pe, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
-                                   hpc, cnt);
-  part(i_far, end, pb, pe);
+  // points per group: enough independent pairs per lane to cover latency
+  constexpr int NP = APL >= 3 ? 1 : (APL == 2 ? 2 : 4);
+  int p = beg, nfar = 0;
+  // far-field vdW has its own accumulators so that no accumulator is
+  // updated on both sides of the far/mid branch (keeps them in place)
   float vdwf[APL];
 #pragma unroll
   for (int k = 0; k < APL; ++k) vdwf[k] = 0.0f;
-  region_pairs<APL, KIND, REG_FAR>(S, pb, pe, K, x, y, z, cp, psprow, vdwf, ele, pmf, psp, hdb,
-                                   hpc, cnt);
-  // far field: PMF contributes pmf1 per point
-  const float nf = (float)(pe - pb);
+  for (; p + NP <= end; p += NP)
+    nfar += pair_group<APL, KIND, NP>(S, p, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc,
+                                      cnt, vdwf);
+  for (; p < end; ++p)
+    nfar += pair_group<APL, KIND, 1>(S, p, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc,
+                                     cnt, vdwf);
+  const float nf = (float)nfar;
 #pragma unroll
   for (int k = 0; k < APL; ++k) {
     pmf[k] = fmaf(cp[k].pmf1, nf, pmf[k]);
     vdw[k] += vdwf[k];
   }
-}
-
-// First index in [b, e) whose radius exceeds r (radii ascending), or e.
-__device__ __forceinline__ int upper_index(const float *rad, int b, int e, float r) {
-  while (b < e) {
-    const int m = (b + e) >> 1;
-    if (rad[m] <= r) b = m + 1;
-    else e = m;
-  }
-  return b;
 }
 
 // Energy, RMSD and MCC of one pose held by `lpr` lanes (CalcEnergy_d +
@@ -260,27 +243,14 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
     psprow[k] = C.psp + (size_t)lt[k] * C.n_res;
   }
 
-  // the wave's atoms lie within rw of the pocket centre
-  float rw = 0.0f;
-#pragma unroll
-  for (int k = 0; k < APL; ++k) {
-    const float dx = x[k] - C.pocket[0], dy = y[k] - C.pocket[1], dz = z[k] - C.pocket[2];
-    rw = fmaxf(rw, dx * dx + dy * dy + dz * dz);
-  }
-  for (int m = 1; m < 64; m <<= 1) rw = fmaxf(rw, __shfl_xor(rw, m));
-  rw = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(sqrtf(rw)))) + C.rad_slack;
-
   for (int ci = 0; live && ci < n_chunks; ++ci) {
     const int4 ch = S.ch[ci];
-    const int beg = __builtin_amdgcn_readfirstlane(ch.x);
-    const int end = beg + __builtin_amdgcn_readfirstlane(ch.y);
+    const int cb = __builtin_amdgcn_readfirstlane(ch.x);
+    const int cl = __builtin_amdgcn_readfirstlane(ch.y);
+    const int beg = cb + (cl * sp.h) / sp.ns;
+    const int end = cb + (cl * (sp.h + 1)) / sp.ns;
     const int type = __builtin_amdgcn_readfirstlane(ch.z);
     const int kind = __builtin_amdgcn_readfirstlane(ch.w);
-    const float2 cr = C.chunk_rad[ci];
-    const float rn = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cr.x)));
-    const float rf = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cr.y)));
-    const int i_near = __builtin_amdgcn_readfirstlane(upper_index(C.prt_rad, beg, end, rw + rn));
-    const int i_far = __builtin_amdgcn_readfirstlane(upper_index(C.prt_rad, i_near, end, rw + rf));
     ChunkPar cp[APL];
 #pragma unroll
     for (int k = 0; k < APL; ++k) {
@@ -297,16 +267,27 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
       // (6 - d/2)(d - pmf0) log2(e) = A d^2 + B d + C
       cp[k].pb = LOG2E * fmaf(0.5f, a.z, 6.0f);
       cp[k].pc = -6.0f * LOG2E * a.z;
+      // contacts (d <= pmf0) and the near electrostatic form (d < 1/el1)
+      // lie within this radius; the margin covers the rounding of d
+      const float rn = fmaxf(a.z, C.inv_el1) * 1.0001f;
+      cp[k].nr2 = rn * rn;
+      // far radius: (d/2 - 6)(d - pmf0) >= 25 ln 2 makes exp2 of the PMF
+      // exponent <= 2^-25, so 1 + e == 1; the H-bond exponent -h^2 <= -60
+      const float u = 0.5f * a.z - 6.0f;
+      float rf = (6.0f + 0.5f * a.z) + sqrtf(u * u + 2.0f * 25.0f * 0.69314718f);
+      if (b.z != 0.0f) rf = fmaxf(rf, (7.7459667f - b.y) / b.x);
+      rf *= 1.0001f;
+      cp[k].rf2 = rf * rf;
     }
     if (kind == KIND_PLAIN)
-      chunk_pairs<APL, KIND_PLAIN>(S, sp, beg, i_near, i_far, end, K, x, y, z, cp, psprow, vdw,
-                                   ele, pmf, psp, hdb, hpc, cnt);
+      chunk_pairs<APL, KIND_PLAIN>(S, beg, end, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
+                                   hpc, cnt);
     else if (kind == KIND_HPC)
-      chunk_pairs<APL, KIND_HPC>(S, sp, beg, i_near, i_far, end, K, x, y, z, cp, psprow, vdw, ele,
-                                 pmf, psp, hdb, hpc, cnt);
+      chunk_pairs<APL, KIND_HPC>(S, beg, end, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
+                                 hpc, cnt);
     else
-      chunk_pairs<APL, KIND_PSP>(S, sp, beg, i_near, i_far, end, K, x, y, z, cp, psprow, vdw, ele,
-                                 pmf, psp, hdb, hpc, cnt);
+      chunk_pairs<APL, KIND_PSP>(S, beg, end, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
+                                 hpc, cnt);
   }
 
   // pair sums that are linear in the receptor points
