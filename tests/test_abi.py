@@ -14,7 +14,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    src = open(os.path.join(ROOT, "include", "geauxdock.h")).read()
+    src = "".join(open(os.path.join(ROOT, "include", h)).read()
+                  for h in sorted(os.listdir(os.path.join(ROOT, "include"))) if h.endswith(".h"))
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(gd_[a-z_0-9]+)\s*\(", src)))
 
@@ -23,6 +24,7 @@ def test_library_exports_every_declared_symbol():
     lib = gd.load_library()
     names = declared_symbols()
     assert "gd_run" in names and "gd_calc_energy" in names and len(names) >= 13
+    assert "gd_load_complex" in names and "gd_sdf_read" in names
     for n in names:
         assert hasattr(lib, n), n
 

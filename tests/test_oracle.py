@@ -482,3 +482,47 @@ def test_da_chain_reduces_to_plain_when_surrogate_is_exact(oracle):
     oracle.mc_run(d, p2, seed=3, init=False, step0=0, n_steps=200, nc=20)
     oracle.mc_run_da(d, d2, seed=3, step0=0, n_steps=200, nc=20, k_sub=k)
     assert np.array_equal(p2.mm, d2.mm) and np.array_equal(p2.idx_tmp, d2.idx_tmp)
+
+
+def _load_trajectory():
+    import os
+    from geaxdock_gpu_amd import io
+    g = os.path.join(os.path.dirname(__file__), "golden", "inputs")
+    cx = io.load_complex(os.path.join(g, "1a07C1", "1a07C1.sdf"),
+                         os.path.join(g, "1a07C1", "1a07C.pdb"),
+                         os.path.join(g, "1a07C1", "1a07C1-0.8.ff"),
+                         os.path.join(g, "parameters", "paras"), complex_id="1a07C1")
+    with open(os.path.join(g, "trajectory.out")) as f:
+        rows = [l.split() for l in f.read().split("\n")[1:] if l.strip()]
+    ip = np.array([int(r[0]) for r in rows])
+    il = np.array([int(r[1]) for r in rows])
+    mm = np.array([[float(v) for v in r[2:8]] for r in rows], np.float32)
+    cms = np.array([float(r[8]) for r in rows])
+    ener = np.array([float(r[9]) for r in rows])
+    return cx, ip, il, mm, cms, ener
+
+
+def test_reference_trajectory_known_answers(oracle):
+    """data/trajectory.out: 2,666 poses of a reference run on 1a07C1 (receptor
+    conformer, ligand conformer, move vector) with the MCC and total energy
+    the reference printed (6 decimals). Loaded through the library's input
+    loader, the FP64 oracle reproduces every total energy within 1e-5 and
+    every MCC within 2e-6 (print rounding 5e-7); the FP32 CPU path within
+    2e-5 and 2e-6."""
+    cx, ip, il, mm, cms, ener = _load_trajectory()
+    assert len(ener) == 2666
+    d = cx.desc()
+    r64, r32 = oracle.refmap_f64(d, cx), oracle.refmap_f32(d, cx)
+    worst = np.zeros(4)
+    for i in range(len(ener)):
+        o = oracle.energy_f64(d, r64, int(ip[i]), int(il[i]), mm[i])
+        f = oracle.energy_f32(d, r32, int(ip[i]), int(il[i]), mm[i])
+        errs = [abs(o.e[9] - ener[i]), abs(o.cms - cms[i]), abs(f.e[9] - ener[i]),
+                abs(f.cms - cms[i])]
+        worst = np.maximum(worst, errs)
+        assert errs[0] <= 1e-5 and errs[1] <= 2e-6, (i, o.e[9], ener[i], o.cms, cms[i])
+        assert errs[2] <= 2e-5 and errs[3] <= 2e-6, (i, f.e[9], ener[i], f.cms, cms[i])
+    print("\nworst |d ener| f64 %.2e, |d cms| f64 %.2e, |d ener| f32 %.2e, |d cms| f32 %.2e"
+          % tuple(worst))
+    # the file spans a real descent: a broken term could not match all rows
+    assert ener.max() - ener.min() > 2.0
