@@ -6,7 +6,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgeauxdock.so")
 SOURCES = ["gd_engine.hip", "gd_io.cpp"]
-HEADERS = ["gd_mc.hip", "gd_sur.hip", "gd_device.h", "gd_kernels.h", "../../include/geauxdock.h",
+HEADERS = ["gd_mc.hip", "gd_sur.hip", "gd_post.hip", "gd_device.h", "gd_kernels.h", "../../include/geauxdock.h",
            "../../include/geauxdock_io.h"]
 
 HIPCC_FLAGS = [

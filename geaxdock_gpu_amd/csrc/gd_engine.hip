@@ -1946,3 +1946,5 @@ gd_status gd_sur_train(gd_ctx *ctx, int pose_begin, int pose_end, int epochs, in
 }
 
 }  // extern "C"
+
+#include "gd_post.hip"

@@ -764,4 +764,190 @@ gd_status gd_sdf_read(const char *path, int record, int centre_each, gd_sdf_reco
   return GD_OK;
 }
 
+// ---- trajectory output --------------------------------------------------------
+
+gd_status gd_write_states_csv(const char *path, const gd_record *recs, int64_t n, char *err,
+                              size_t err_len) {
+  Err e{err, err_len};
+  if (err && err_len) err[0] = 0;
+  if (!path || (n > 0 && !recs) || n < 0) return e.fail(GD_ERR_INVALID, "bad arguments");
+  FILE *f = std::fopen(path, "w");
+  if (!f) return e.fail(GD_ERR_INVALID, "cannot write %s", path);
+  std::fputs("lig prt cms rmsd mv1 mv2 mv3 mv4 mv5 mv6 "
+             "vdw ele pmf psp hdb hpc kde lhm edst ener\n", f);
+  for (int64_t i = 0; i < n; ++i) {
+    const gd_record &r = recs[i];
+    std::fprintf(f, "%d %d %.4f %.4f", r.replica.idx_lig, r.replica.idx_prt, r.energy.cms,
+                 r.energy.rmsd);
+    for (int j = 0; j < 6; ++j) std::fprintf(f, " %.4f", r.movematrix[j]);
+    for (int j = 0; j < GD_MAXWEI; ++j) std::fprintf(f, " %.4f", r.energy.e[j]);
+    std::fputc('\n', f);
+  }
+  const bool ok = std::fclose(f) == 0;
+  return ok ? GD_OK : e.fail(GD_ERR_INVALID, "error writing %s", path);
+}
+
+gd_status gd_write_medoids_csv(const char *path, const gd_record *recs, const int *cluster_size,
+                               int64_t n, char *err, size_t err_len) {
+  Err e{err, err_len};
+  if (err && err_len) err[0] = 0;
+  if (!path || (n > 0 && (!recs || !cluster_size)) || n < 0)
+    return e.fail(GD_ERR_INVALID, "bad arguments");
+  FILE *f = std::fopen(path, "w");
+  if (!f) return e.fail(GD_ERR_INVALID, "cannot write %s", path);
+  std::fputs("lig prt t0 t1 t2 r0 r1 r2 vdw ele pmf psp hdb hpc kde lhm edst "
+             "cms rmsd ener cluster_sz\n", f);
+  for (int64_t i = 0; i < n; ++i) {
+    const gd_record &r = recs[i];
+    std::fprintf(f, "%d %d ", r.replica.idx_lig, r.replica.idx_prt);
+    for (int j = 0; j < 6; ++j) std::fprintf(f, "%f ", r.movematrix[j]);
+    for (int j = 0; j < GD_MAXWEI - 1; ++j) std::fprintf(f, "%f ", r.energy.e[j]);
+    std::fprintf(f, "%f %f %f %d \n", r.energy.cms, r.energy.rmsd, r.energy.e[GD_MAXWEI - 1],
+                 cluster_size[i]);
+  }
+  const bool ok = std::fclose(f) == 0;
+  return ok ? GD_OK : e.fail(GD_ERR_INVALID, "error writing %s", path);
+}
+
+// ---- flat record file -----------------------------------------------------------
+// 64-byte header, then gd_record entries (little-endian, 92 bytes each) in
+// append order: the content of the reference's per-dump LigRecord datasets
+// (src/hdf5_datastruct_ligrecord_create.h:53-84) without the padding of the
+// fixed STEPS_PER_DUMP slots.
+namespace {
+struct RecHeader {
+  char magic[8];       // "GDREC001"
+  uint32_t rec_size;   // sizeof(gd_record)
+  uint32_t version;    // 1
+  uint64_t n_records;
+  char pad[40];
+};
+static_assert(sizeof(RecHeader) == 64, "record file header is 64 bytes");
+static_assert(sizeof(gd_record) == 92, "gd_record is 92 bytes");
+
+gd_status read_header(FILE *f, const char *path, RecHeader &h, Err &e) {
+  if (std::fread(&h, sizeof h, 1, f) != 1) return e.fail(GD_ERR_INVALID, "%s: no header", path);
+  if (std::memcmp(h.magic, "GDREC001", 8) != 0 || h.rec_size != sizeof(gd_record) || h.version != 1)
+    return e.fail(GD_ERR_INVALID, "%s: not a record file of this layout", path);
+  return GD_OK;
+}
+}  // namespace
+
+gd_status gd_records_write(const char *path, int append, const gd_record *recs, int64_t n,
+                           char *err, size_t err_len) {
+  Err e{err, err_len};
+  if (err && err_len) err[0] = 0;
+  if (!path || n < 0 || (n > 0 && !recs)) return e.fail(GD_ERR_INVALID, "bad arguments");
+  RecHeader h{};
+  FILE *f = append ? std::fopen(path, "r+b") : nullptr;
+  if (f) {
+    gd_status s = read_header(f, path, h, e);
+    if (s != GD_OK) {
+      std::fclose(f);
+      return s;
+    }
+  } else {
+    f = std::fopen(path, "w+b");
+    if (!f) return e.fail(GD_ERR_INVALID, "cannot write %s", path);
+    std::memcpy(h.magic, "GDREC001", 8);
+    h.rec_size = sizeof(gd_record);
+    h.version = 1;
+    h.n_records = 0;
+  }
+  bool ok = std::fseek(f, (long)(sizeof h + h.n_records * sizeof(gd_record)), SEEK_SET) == 0;
+  ok = ok && (n == 0 || std::fwrite(recs, sizeof(gd_record), (size_t)n, f) == (size_t)n);
+  h.n_records += (uint64_t)n;
+  ok = ok && std::fseek(f, 0, SEEK_SET) == 0 && std::fwrite(&h, sizeof h, 1, f) == 1;
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? GD_OK : e.fail(GD_ERR_INVALID, "error writing %s", path);
+}
+
+gd_status gd_records_count(const char *path, int64_t *n, char *err, size_t err_len) {
+  Err e{err, err_len};
+  if (err && err_len) err[0] = 0;
+  if (!path || !n) return e.fail(GD_ERR_INVALID, "bad arguments");
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return e.fail(GD_ERR_INVALID, "cannot open %s", path);
+  RecHeader h{};
+  gd_status s = read_header(f, path, h, e);
+  std::fclose(f);
+  if (s == GD_OK) *n = (int64_t)h.n_records;
+  return s;
+}
+
+gd_status gd_records_read(const char *path, int64_t first, int64_t n, gd_record *out, char *err,
+                          size_t err_len) {
+  Err e{err, err_len};
+  if (err && err_len) err[0] = 0;
+  if (!path || first < 0 || n < 0 || (n > 0 && !out)) return e.fail(GD_ERR_INVALID, "bad arguments");
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return e.fail(GD_ERR_INVALID, "cannot open %s", path);
+  RecHeader h{};
+  gd_status s = read_header(f, path, h, e);
+  if (s == GD_OK && (uint64_t)(first + n) > h.n_records)
+    s = e.fail(GD_ERR_INVALID, "%s: records [%lld, %lld) past the %llu stored", path,
+               (long long)first, (long long)(first + n), (unsigned long long)h.n_records);
+  if (s == GD_OK) {
+    const bool ok = std::fseek(f, (long)(sizeof h + (uint64_t)first * sizeof(gd_record)), SEEK_SET) == 0 &&
+                    (n == 0 || std::fread(out, sizeof(gd_record), (size_t)n, f) == (size_t)n);
+    if (!ok) s = e.fail(GD_ERR_INVALID, "%s: short read", path);
+  }
+  std::fclose(f);
+  return s;
+}
+
+// ---- trace tool: a pose as ligand coordinates and as an SDF -----------------------
+
+// PlaceLigand (src/util.C:456-497): the conformer (in its centroid frame)
+// rotated by the Euler angles mm[3..5], then moved to the pocket centre plus
+// mm[0..2]. FP32, as the reference's host code.
+gd_status gd_pose_coords(const gd_complex_desc *d, int idx_lig, const float mm[6], float *x,
+                         float *y, float *z) {
+  if (!d || !mm || !x || !y || !z || idx_lig < 0 || idx_lig >= d->n_lig) return GD_ERR_INVALID;
+  const float s1 = sinf(mm[3]), c1 = cosf(mm[3]);
+  const float s2 = sinf(mm[4]), c2 = cosf(mm[4]);
+  const float s3 = sinf(mm[5]), c3 = cosf(mm[5]);
+  const float r00 = c1 * c2, r01 = c1 * s2 * s3 - c3 * s1, r02 = s1 * s3 + c1 * c3 * s2;
+  const float r10 = c2 * s1, r11 = c1 * c3 + s1 * s2 * s3, r12 = c3 * s1 * s2 - c1 * s3;
+  const float r20 = -s2, r21 = c2 * s3, r22 = c2 * c3;
+  const size_t o = (size_t)idx_lig * d->lna;
+  for (int l = 0; l < d->lna; ++l) {
+    const float a = d->lig_x[o + l], b = d->lig_y[o + l], c = d->lig_z[o + l];
+    x[l] = r00 * a + r01 * b + r02 * c + mm[0] + d->pocket_center[0];
+    y[l] = r10 * a + r11 * b + r12 * c + mm[1] + d->pocket_center[1];
+    z[l] = r20 * a + r21 * b + r22 * c + mm[2] + d->pocket_center[2];
+  }
+  return GD_OK;
+}
+
+// replaceLigandCoords (src/util.C:504-549): the SDF with lines 5..4+lna
+// (the first record's atom block) starting with the new coordinates as
+// %10.4f%10.4f%10.4f; every other line unchanged.
+gd_status gd_write_pose_sdf(const char *ligand_sdf, int lna, const float *x, const float *y,
+                            const float *z, const char *out_path, char *err, size_t err_len) {
+  Err e{err, err_len};
+  if (err && err_len) err[0] = 0;
+  if (!ligand_sdf || !out_path || !x || !y || !z || lna <= 0)
+    return e.fail(GD_ERR_INVALID, "bad arguments");
+  std::ifstream in(ligand_sdf);
+  if (!in.is_open()) return e.fail(GD_ERR_INVALID, "cannot open %s", ligand_sdf);
+  std::vector<std::string> lines;
+  for (std::string l; std::getline(in, l);) lines.push_back(l);
+  if ((int)lines.size() < 4 + lna) return e.fail(GD_ERR_INVALID, "%s: fewer than %d atom lines", ligand_sdf, lna);
+  for (int a = 0; a < lna; ++a)
+    if (lines[4 + a].size() < 30) return e.fail(GD_ERR_INVALID, "%s: atom line %d is short", ligand_sdf, a + 1);
+  FILE *f = std::fopen(out_path, "w");
+  if (!f) return e.fail(GD_ERR_INVALID, "cannot write %s", out_path);
+  for (size_t i = 0; i < lines.size(); ++i) {
+    if (i >= 4 && i < (size_t)(4 + lna)) {
+      const int a = (int)i - 4;
+      std::fprintf(f, "%10.4f%10.4f%10.4f%s\n", x[a], y[a], z[a], lines[i].c_str() + 30);
+    } else {
+      std::fprintf(f, "%s\n", lines[i].c_str());
+    }
+  }
+  const bool ok = std::fclose(f) == 0;
+  return ok ? GD_OK : e.fail(GD_ERR_INVALID, "error writing %s", out_path);
+}
+
 }  // extern "C"

@@ -48,6 +48,19 @@ def _lib():
                                     C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_char_p, C.c_size_t]
         lib.gd_sdf_read.restype = C.c_int
+        E = [C.c_char_p, C.c_size_t]
+        lib.gd_write_states_csv.argtypes = [C.c_char_p, C.c_void_p, C.c_int64] + E
+        lib.gd_write_medoids_csv.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_int64] + E
+        lib.gd_records_write.argtypes = [C.c_char_p, C.c_int, C.c_void_p, C.c_int64] + E
+        lib.gd_records_count.argtypes = [C.c_char_p, C.POINTER(C.c_int64)] + E
+        lib.gd_records_read.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.c_void_p] + E
+        lib.gd_pose_coords.argtypes = [C.POINTER(abi.ComplexDesc), C.c_int, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.gd_write_pose_sdf.argtypes = [C.c_char_p, C.c_int, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_char_p] + E
+        for f in ("gd_write_states_csv", "gd_write_medoids_csv", "gd_records_write",
+                  "gd_records_count", "gd_records_read", "gd_pose_coords", "gd_write_pose_sdf"):
+            getattr(lib, f).restype = C.c_int
         lib._io_bound = True
     return lib
 
@@ -160,3 +173,118 @@ def _from_desc(d):
     cx.move_kind = d.move_kind
     cx.seed = None
     return cx
+
+
+def _call(fn, *args):
+    err = C.create_string_buffer(512)
+    st = fn(*args, err, 512)
+    if st != 0:
+        from . import GeauxDockError
+        raise GeauxDockError(st, err.value.decode())
+
+
+def _records(recs):
+    return np.ascontiguousarray(recs, abi.RECORD_DTYPE)
+
+
+def write_states_csv(path, records):
+    """printStates of accepted steps (src/util.C:1504-1550)."""
+    r = _records(records)
+    _call(_lib().gd_write_states_csv, _b(path), r.ctypes.data, len(r))
+
+
+def write_medoids_csv(path, records, cluster_size):
+    """printStates of medoids (src/util.C:1464-1502)."""
+    r = _records(records)
+    cs = np.ascontiguousarray(cluster_size, np.int32)
+    if len(cs) != len(r):
+        raise ValueError("one cluster size per record")
+    _call(_lib().gd_write_medoids_csv, _b(path), r.ctypes.data, cs.ctypes.data, len(r))
+
+
+class RecordFile:
+    """Flat record file (gd_records_*): a 64-byte header and gd_record
+    entries in append order. `sink` fits Engine.run(sink=...)."""
+
+    HEADER = 64
+
+    def __init__(self, path, append=False):
+        self.path = path
+        if not append:
+            self.write([], append=False)
+
+    def write(self, records, append=True):
+        r = _records(records)
+        _call(_lib().gd_records_write, _b(self.path), int(bool(append)), r.ctypes.data, len(r))
+
+    def sink(self, cid, records):
+        self.write(records)
+
+    def __len__(self):
+        n = C.c_int64(0)
+        _call(_lib().gd_records_count, _b(self.path), C.byref(n))
+        return n.value
+
+    def read(self, first=0, n=None):
+        if n is None:
+            n = len(self) - first
+        out = np.zeros(n, abi.RECORD_DTYPE)
+        _call(_lib().gd_records_read, _b(self.path), int(first), int(n), out.ctypes.data)
+        return out
+
+    def memmap(self):
+        """The records as a read-only numpy view of the file."""
+        return np.memmap(self.path, dtype=abi.RECORD_DTYPE, mode="r", offset=self.HEADER,
+                         shape=(len(self),))
+
+
+def pose_coords(cx, idx_lig, movematrix):
+    """PlaceLigand (src/util.C:456-497): (lna, 3) coordinates of conformer
+    idx_lig under the move vector."""
+    d = cx.desc()
+    mm = np.ascontiguousarray(movematrix, np.float32).reshape(6)
+    x, y, z = (np.zeros(cx.lna, np.float32) for _ in range(3))
+    st = _lib().gd_pose_coords(C.byref(d), int(idx_lig), mm.ctypes.data, x.ctypes.data,
+                               y.ctypes.data, z.ctypes.data)
+    if st != 0:
+        from . import GeauxDockError
+        raise GeauxDockError(st, "gd_pose_coords: bad arguments")
+    return np.stack([x, y, z], axis=1)
+
+
+def write_pose_sdf(ligand_sdf, coords, out_path):
+    """replaceLigandCoords (src/util.C:504-549) written to out_path."""
+    c = np.ascontiguousarray(coords, np.float32).reshape(-1, 3)
+    x, y, z = (np.ascontiguousarray(c[:, i]) for i in range(3))
+    _call(_lib().gd_write_pose_sdf, _b(ligand_sdf), len(c), x.ctypes.data, y.ctypes.data,
+          z.ctypes.data, _b(out_path))
+
+
+def read_trace(path):
+    """read2D of a trace file: rows of whitespace-separated numbers (a header
+    row that is not numeric is skipped)."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if not t:
+                continue
+            try:
+                rows.append([float(v) for v in t])
+            except ValueError:
+                continue
+    return rows
+
+
+def trace(cx, ligand_sdf, rows, prefix):
+    """The trace tool (src/trace.C:90-119): for each row (protein conformer,
+    ligand conformer, six move values, ...), the ligand conformer placed by
+    the move vector and written as <prefix>_<row>.sdf. Returns the paths."""
+    out = []
+    for i, r in enumerate(rows):
+        if len(r) < 8:
+            raise ValueError(f"trace row {i} has {len(r)} values, needs 8")
+        path = f"{prefix}_{i}.sdf"
+        write_pose_sdf(ligand_sdf, pose_coords(cx, int(r[1]), r[2:8]), path)
+        out.append(path)
+    return out

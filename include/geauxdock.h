@@ -324,6 +324,31 @@ gd_status gd_sur_surrogate(gd_ctx *ctx, int complex_id, int n, const int *idx_li
 gd_status gd_sur_train(gd_ctx *ctx, int pose_begin, int pose_end, int epochs, int batch_poses,
                        float lr, double *epoch_loss, double *seconds);
 
+/* ---- post-MC clustering (device k-means) ---------------------------------
+ * clusterByKmeans (src/util.C:1801-1880) over seq_kmeans (src/seq_kmeans.c)
+ * on the nine energy terms e[0..8] of each object: centers start at the first
+ * k objects of a seeded random permutation, Lloyd passes run while the share
+ * of objects that change cluster exceeds `threshold` (the reference: 0.001,
+ * at most 501 passes), and each non-empty cluster reports its member nearest
+ * the final center (ties: lower index). A group with fewer than k objects
+ * reports every object as its own medoid. Cluster sums are exact (fixed
+ * point), so results are bit-reproducible. Limits: k <= 512, 2^19 objects
+ * per group. */
+
+/* groups: objects [group_off[g], group_off[g+1]) of features (row-major,
+ * 9 floats per object). medoid, size: [n_groups * k], group-local index or
+ * -1 for an empty cluster; passes (optional): [n_groups]. */
+gd_status gd_kmeans_medoids(gd_ctx *ctx, int n_groups, const int64_t *group_off,
+                            const float *features, int k, float threshold, int max_pass,
+                            uint64_t seed, int *medoid, int *size, int *passes);
+
+/* post_mc (src/post_mc.C:40-101): the records of each replica (ascending
+ * replica id, in the order given) clustered into k_replica clusters, then
+ * all their medoids, in replica and cluster order, into k_final. Writes up to
+ * k_final medoids as record indices with their level-2 cluster sizes. */
+gd_status gd_post_mc(gd_ctx *ctx, const gd_record *recs, int64_t n, int k_replica, int k_final,
+                     uint64_t seed, int64_t *medoid_rec, int *medoid_size, int *n_medoids);
+
 #ifdef __cplusplus
 }
 #endif

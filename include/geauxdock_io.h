@@ -88,6 +88,42 @@ gd_status gd_sdf_read(const char *path, int record, int centre_each, gd_sdf_reco
                       float *xyz, int max_conf, float *rmsd, int max_rmsd, int *types,
                       float *charges, char *err, size_t err_len);
 
+/* ---- trajectory output ------------------------------------------------------ */
+
+/* printStates for accepted steps (src/util.C:1504-1550): header
+ * "lig prt cms rmsd mv1..mv6 vdw ele pmf psp hdb hpc kde lhm edst ener", one
+ * line per record, values with 4 decimals. */
+gd_status gd_write_states_csv(const char *path, const gd_record *recs, int64_t n, char *err,
+                              size_t err_len);
+/* printStates for medoids (src/util.C:1464-1502): move vector, nine terms,
+ * cms, rmsd, total and the cluster size, values with 6 decimals. */
+gd_status gd_write_medoids_csv(const char *path, const gd_record *recs, const int *cluster_size,
+                               int64_t n, char *err, size_t err_len);
+
+/* ---- record file ---------------------------------------------------------------
+ * Replaces the reference's per-dump HDF5 LigRecord files (DumpLigRecord /
+ * ReadLigRecord, src/hdf5io.C:11-55; layout
+ * src/hdf5_datastruct_ligrecord_create.h:53-84) with one flat file: a 64-byte
+ * header ("GDREC001", record size 92, version 1, record count) followed by
+ * gd_record entries in append order. */
+gd_status gd_records_write(const char *path, int append, const gd_record *recs, int64_t n,
+                           char *err, size_t err_len);
+gd_status gd_records_count(const char *path, int64_t *n, char *err, size_t err_len);
+gd_status gd_records_read(const char *path, int64_t first, int64_t n, gd_record *out, char *err,
+                          size_t err_len);
+
+/* ---- trace tool (src/trace.C:90-119) --------------------------------------------- */
+
+/* PlaceLigand (src/util.C:456-497): coordinates of ligand conformer idx_lig
+ * of a complex under move vector mm (translation from the pocket centre,
+ * Euler angles); x, y, z hold lna values. */
+gd_status gd_pose_coords(const gd_complex_desc *d, int idx_lig, const float mm[6], float *x,
+                         float *y, float *z);
+/* replaceLigandCoords (src/util.C:504-549): ligand_sdf with the first
+ * record's atom coordinates replaced, written to out_path. */
+gd_status gd_write_pose_sdf(const char *ligand_sdf, int lna, const float *x, const float *y,
+                            const float *z, const char *out_path, char *err, size_t err_len);
+
 #ifdef __cplusplus
 }
 #endif

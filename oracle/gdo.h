@@ -97,4 +97,11 @@ int gdo_mc_run_da(const gd_complex_desc *cx, uint64_t seed, int cid, int step0, 
 }
 #endif
 
+/* post-MC clustering (post.c) */
+void gdo_km_init(int n, int k, int group, int level, uint64_t seed, int *out);
+int gdo_kmeans(const float *f, int n, int k, float threshold, int max_pass, int group, int level,
+               uint64_t seed, int *memb, int *medoid, int *size);
+int gdo_post_mc(const gd_record *recs, int64_t n, int k_rep, int k_final, uint64_t seed,
+                int64_t *medoid_rec, int *medoid_size);
+
 #endif

@@ -53,6 +53,14 @@ def load():
     lib.gdo_mc_run_da.argtypes = [C.POINTER(abi.ComplexDesc), C.c_uint64, C.c_int, C.c_int,
                                   C.c_int, C.c_int, C.c_int, C.POINTER(McIO), C.c_int]
     lib.gdo_mc_run_da.restype = C.c_int
+    ip = C.POINTER(C.c_int)
+    lib.gdo_km_init.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, ip]
+    lib.gdo_kmeans.argtypes = [fp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_int,
+                               C.c_uint64, ip, ip, ip]
+    lib.gdo_kmeans.restype = C.c_int
+    lib.gdo_post_mc.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_uint64,
+                                C.POINTER(C.c_int64), ip]
+    lib.gdo_post_mc.restype = C.c_int
     return lib
 
 
@@ -178,3 +186,34 @@ def mc_run_da(desc, state, seed, cid=0, step0=0, n_steps=0, nc=0, k_sub=1, threa
                              threads)
     if rc != 0:
         raise RuntimeError(f"gdo_mc_run_da failed: {rc}")
+
+
+def kmeans(features, k, threshold=0.001, max_pass=501, group=0, level=0, seed=0):
+    """One group: (membership, medoid[k], size[k], passes)."""
+    f = np.ascontiguousarray(features, np.float32).reshape(-1, 9)
+    n = len(f)
+    memb = np.zeros(max(n, 1), np.int32)
+    med = np.zeros(k, np.int32)
+    sz = np.zeros(k, np.int32)
+    ip = C.POINTER(C.c_int)
+    p = lib().gdo_kmeans(f.ctypes.data_as(C.POINTER(C.c_float)), n, k, threshold, max_pass,
+                         group, level, seed, memb.ctypes.data_as(ip), med.ctypes.data_as(ip),
+                         sz.ctypes.data_as(ip))
+    return memb[:n], med, sz, p
+
+
+def km_init(n, k, group=0, level=0, seed=0):
+    out = np.zeros(k, np.int32)
+    lib().gdo_km_init(n, k, group, level, seed, out.ctypes.data_as(C.POINTER(C.c_int)))
+    return out
+
+
+def post_mc(records, k_rep=50, k_final=500, seed=0):
+    """(record indices of the medoids, their cluster sizes)."""
+    r = np.ascontiguousarray(records)
+    mr = np.zeros(k_final, np.int64)
+    ms = np.zeros(k_final, np.int32)
+    m = lib().gdo_post_mc(r.ctypes.data, len(r), k_rep, k_final, seed,
+                          mr.ctypes.data_as(C.POINTER(C.c_int64)),
+                          ms.ctypes.data_as(C.POINTER(C.c_int)))
+    return mr[:m], ms[:m]
