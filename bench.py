@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--autotune", type=int, default=-1,
                     help="tune launch shapes first (default: on for the batch workload)")
     ap.add_argument("--tune-budget", type=float, default=90.0, help="seconds")
-    ap.add_argument("--mode", default="da", choices=["plain", "da"],
+    ap.add_argument("--mode", default="da", choices=["plain", "da", "post"],
                     help="plain vs surrogate delayed acceptance (default), or plain engine only")
     ap.add_argument("--no-record", action="store_true", help="plain mode: keep no records")
     ap.add_argument("--k-sub", type=int, default=4, help="surrogate steps per exact evaluation")
@@ -315,6 +315,91 @@ def launch_options(args, cxs):
     return opts, tuned
 
 
+def post_bench(args, cxs, desc):
+    """Post-MC clustering of the workload's accepted-step records: k-means
+    medoids per replica (k 50), then over all of those (k 500), on the device.
+    value = records clustered per second of wall time (grouping, the copy to
+    HBM and both levels); the device-only rate and a roofline of the assign
+    pass (27 flops per object-center distance) are reported beside it."""
+    import numpy as np
+    import geaxdock_gpu_amd as gd
+    from geaxdock_gpu_amd import abi
+    recs = []
+    with gd.Engine(seed=0, record=True, steps_per_dump=2000) as eng:
+        for cx in cxs:
+            eng.add_complex(cx)
+        if args.warmup > 0:
+            eng.run(args.warmup, args.nc)
+        eng.run(args.steps, args.nc, sink=lambda c, r: recs.append(r))
+        allr = np.concatenate(recs)
+        eng.post_mc(allr[: min(len(allr), 200000)], 50, 500, seed=1)  # warm-up
+        tm = {}
+        t0 = time.perf_counter()
+        idx, sz = eng.post_mc(allr, 50, 500, seed=0, timing=tm)
+        dt = time.perf_counter() - t0
+        # level 1 alone for the roofline: records grouped by replica
+        order = np.argsort(allr["replica"][:, 0], kind="stable")
+        rep = allr["replica"][order, 0]
+        counts = np.bincount(rep)
+        counts = counts[counts > 0]
+        off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        feat = allr["energy"]["e"][order, :9]
+        t1 = {}
+        _, _, passes = eng.kmeans_medoids(feat, off, 50, seed=0, timing=t1)
+    live = counts >= 50
+    obj_passes = float((counts[live] * passes[live]).sum())
+    flops = obj_passes * 50 * 27.0
+    achieved = flops / (t1["device_ms"] * 1e-3)
+    out = {
+        "metric": "post-MC records clustered/s (k-means medoids per replica, then over medoids)",
+        "value": len(allr) / dt,
+        "unit": "records/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (distances), int64 (cluster sums)",
+        "data": "synthetic (seeded generator, geaxdock_gpu_amd/synth.py), records of a plain run",
+        "config": dict(desc, records=int(len(allr)), replicas=int(len(counts)), k_replica=50,
+                       k_final=500, medoids=int(len(idx))),
+        "device_ms": tm["device_ms"],
+        "device_records_per_s": len(allr) / (tm["device_ms"] * 1e-3),
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "kernel": "gd::gd_km_assign_kernel (+ update), level 1",
+                     "work": f"{obj_passes:.3e} object-passes x 50 centers x 27 flops",
+                     "passes_mean": float(passes[live].mean()), "passes_max": int(passes.max())},
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = post_cpu_baseline(allr, args)
+        except Exception as ex:
+            out["cpu_baseline"] = {"error": str(ex)}
+    return out
+
+
+def post_cpu_baseline(allr, args):
+    """The oracle's post_mc (one thread) on the records of the first replicas,
+    grown until about cpu_seconds of work."""
+    import numpy as np
+    from oracle import gdo
+    reps = np.unique(allr["replica"][:, 0])
+    n_r = 8
+    while True:
+        sub = allr[np.isin(allr["replica"][:, 0], reps[:n_r])]
+        t0 = time.perf_counter()
+        gdo.post_mc(sub, 50, 500, seed=0)
+        dt = time.perf_counter() - t0
+        if dt >= args.cpu_seconds or n_r >= len(reps):
+            break
+        n_r = min(len(reps), int(n_r * max(2.0, args.cpu_seconds / max(dt, 1e-3))))
+    return {"value": len(sub) / dt, "unit": "records/s", "cores": 1, "kind": "port",
+            "sample": f"{len(sub)} records of {n_r} replicas, {dt:.1f} s"}
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -329,6 +414,9 @@ def main():
     import geaxdock_gpu_amd as gd
 
     cxs, desc = workload(args)
+    if args.mode == "post":
+        print(json.dumps(post_bench(args, cxs, desc)))
+        return
     if args.mode == "da":
         out = da_compare(args, cxs, desc)
         if not args.no_cpu:

@@ -94,9 +94,9 @@ def load_library():
                                  P(C.c_double), P(C.c_double)]
     lib.gd_kmeans_medoids.argtypes = [vp, C.c_int, P(C.c_int64), P(C.c_float), C.c_int,
                                       C.c_float, C.c_int, C.c_uint64, P(C.c_int), P(C.c_int),
-                                      P(C.c_int)]
+                                      P(C.c_int), P(C.c_double)]
     lib.gd_post_mc.argtypes = [vp, vp, C.c_int64, C.c_int, C.c_int, C.c_uint64, P(C.c_int64),
-                               P(C.c_int), P(C.c_int)]
+                               P(C.c_int), P(C.c_int), P(C.c_double)]
     for f in ("gd_kmeans_medoids", "gd_post_mc", "gd_create", "gd_add_complex", "gd_run", "gd_get_state", "gd_checkpoint",
               "gd_restore", "gd_calc_energy", "gd_accept", "gd_exchange", "gd_philox",
               "gd_sur_build_grids", "gd_sur_features", "gd_sur_set_params", "gd_sur_get_params",
@@ -371,31 +371,40 @@ class Engine:
         self._check(self.lib.gd_philox(self._ctx, ctr.shape[0], _u(ctr), _u(key), _u(out)))
         return out
 
-    def kmeans_medoids(self, features, group_off, k, threshold=0.001, max_pass=501, seed=0):
+    def kmeans_medoids(self, features, group_off, k, threshold=0.001, max_pass=501, seed=0,
+                       timing=None):
         """Device k-means per group (gd_kmeans_medoids): (medoid[g][k] group-local
-        index or -1, size[g][k], passes[g])."""
+        index or -1, size[g][k], passes[g]). `timing` (a dict) receives the
+        device milliseconds."""
         f = np.ascontiguousarray(features, np.float32).reshape(-1, abi.MAXWEI - 1)
         off = np.ascontiguousarray(group_off, np.int64)
         g = len(off) - 1
         med = np.zeros((max(g, 1), k), np.int32)
         sz = np.zeros((max(g, 1), k), np.int32)
         ps = np.zeros(max(g, 1), np.int32)
+        ms = C.c_double(0.0)
         self._check(self.lib.gd_kmeans_medoids(
             self._ctx, g, off.ctypes.data_as(C.POINTER(C.c_int64)), _f(f), int(k),
-            float(threshold), int(max_pass), int(seed), _i(med), _i(sz), _i(ps)))
+            float(threshold), int(max_pass), int(seed), _i(med), _i(sz), _i(ps), C.byref(ms)))
+        if timing is not None:
+            timing["device_ms"] = ms.value
         return med[:g], sz[:g], ps[:g]
 
-    def post_mc(self, records, k_replica=50, k_final=500, seed=0):
+    def post_mc(self, records, k_replica=50, k_final=500, seed=0, timing=None):
         """post_mc on the device (gd_post_mc): (record indices of the medoids,
-        their cluster sizes)."""
+        their cluster sizes). `timing` (a dict) receives the device
+        milliseconds of both levels."""
         r = np.ascontiguousarray(records, abi.RECORD_DTYPE)
         mr = np.zeros(k_final, np.int64)
         ms = np.zeros(k_final, np.int32)
         m = C.c_int(0)
+        dms = C.c_double(0.0)
         self._check(self.lib.gd_post_mc(self._ctx, r.ctypes.data, len(r), int(k_replica),
                                         int(k_final), int(seed),
                                         mr.ctypes.data_as(C.POINTER(C.c_int64)), _i(ms),
-                                        C.byref(m)))
+                                        C.byref(m), C.byref(dms)))
+        if timing is not None:
+            timing["device_ms"] = dms.value
         return mr[:m.value], ms[:m.value]
 
 

@@ -65,31 +65,33 @@ __host__ __device__ inline long long km_fixed(float v) {
   return (long long)rint((double)v * 65536.0);
 }
 
-// Squared Euclidean distance, terms added in order, no fused multiply-add.
+// Squared Euclidean distance: terms in order, each square added with one
+// fused multiply-add (the CPU restatement calls fmaf the same way).
 __host__ __device__ inline float km_dist2(const float *a, const float *b) {
 #pragma clang fp contract(off)
   float s = 0.0f;
+#pragma unroll
   for (int d = 0; d < KM_DIM; ++d) {
     const float t = a[d] - b[d];
-    s = s + t * t;
+    s = fmaf(t, t, s);
   }
   return s;
 }
 
-__global__ void __launch_bounds__(256) gd_km_assign_kernel(KmArgs A) {
+// The centers are read-only here and wave-uniform: passed as their own
+// restrict pointer they are fetched with scalar loads and feed the distance
+// arithmetic as scalar operands (no LDS traffic for them).
+__global__ void __launch_bounds__(256) gd_km_assign_kernel(KmArgs A,
+                                                           const float *__restrict__ cent) {
   const int g = A.tile_g[blockIdx.x];
   if (A.done[g]) return;
   const int k = A.k;
   extern __shared__ __align__(16) unsigned char km_lds[];
-  long long *sum = (long long *)km_lds;                 // [k][KM_DIM]
-  float *cen = (float *)(sum + (size_t)k * KM_DIM);     // [k][KM_DIM]
-  int *cnt = (int *)(cen + (size_t)k * KM_DIM);         // [k]
+  long long *sum = (long long *)km_lds;  // [k][KM_DIM]
+  int *cnt = (int *)(sum + (size_t)k * KM_DIM);  // [k]
   int *dlt = cnt + k;
-  const float *gc = A.cent + (size_t)g * k * KM_DIM;
-  for (int j = threadIdx.x; j < k * KM_DIM; j += blockDim.x) {
-    cen[j] = gc[j];
-    sum[j] = 0;
-  }
+  const float *cen = cent + (size_t)g * k * KM_DIM;
+  for (int j = threadIdx.x; j < k * KM_DIM; j += blockDim.x) sum[j] = 0;
   for (int c = threadIdx.x; c < k; c += blockDim.x) cnt[c] = 0;
   if (threadIdx.x == 0) *dlt = 0;
   __syncthreads();
@@ -98,28 +100,46 @@ __global__ void __launch_bounds__(256) gd_km_assign_kernel(KmArgs A) {
   const int s0 = A.tile_s[blockIdx.x];
   const int s1 = min(n, s0 + KM_TILE);
   int changed = 0;
-  for (int i = s0 + (int)threadIdx.x; i < s1; i += blockDim.x) {
-    float f[KM_DIM];
+  // two objects per lane share each center's scalar loads
+  const int stride = 2 * (int)blockDim.x;
+  for (int i0 = s0 + (int)threadIdx.x; i0 < s1; i0 += stride) {
+    const int i1 = i0 + (int)blockDim.x;
+    const bool has1 = i1 < s1;
+    float f0[KM_DIM], f1[KM_DIM];
 #pragma unroll
-    for (int d = 0; d < KM_DIM; ++d) f[d] = A.feat[(size_t)d * A.n_total + base + i];
-    int bc = 0;
-    float bd = km_dist2(f, cen);
+    for (int d = 0; d < KM_DIM; ++d) {
+      f0[d] = A.feat[(size_t)d * A.n_total + base + i0];
+      f1[d] = has1 ? A.feat[(size_t)d * A.n_total + base + i1] : f0[d];
+    }
+    int b0 = 0, b1 = 0;
+    float d0 = km_dist2(f0, cen), d1 = km_dist2(f1, cen);
+#pragma unroll 2
     for (int c = 1; c < k; ++c) {
-      const float dd = km_dist2(f, cen + (size_t)c * KM_DIM);
-      if (dd < bd) {
-        bd = dd;
-        bc = c;
+      const float *cc = cen + (size_t)c * KM_DIM;
+      const float e0 = km_dist2(f0, cc), e1 = km_dist2(f1, cc);
+      if (e0 < d0) {
+        d0 = e0;
+        b0 = c;
+      }
+      if (e1 < d1) {
+        d1 = e1;
+        b1 = c;
       }
     }
-    int *m = A.memb + base + i;
-    if (*m != bc) {
-      changed += 1;
-      *m = bc;
-    }
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1 && !has1) break;
+      const int i = h ? i1 : i0, bc = h ? b1 : b0;
+      const float *f = h ? f1 : f0;
+      int *m = A.memb + base + i;
+      if (*m != bc) {
+        changed += 1;
+        *m = bc;
+      }
 #pragma unroll
-    for (int d = 0; d < KM_DIM; ++d) atomicAdd((unsigned long long *)&sum[bc * KM_DIM + d],
-                                               (unsigned long long)km_fixed(f[d]));
-    atomicAdd(&cnt[bc], 1);
+      for (int d = 0; d < KM_DIM; ++d)
+        atomicAdd((unsigned long long *)&sum[bc * KM_DIM + d], (unsigned long long)km_fixed(f[d]));
+      atomicAdd(&cnt[bc], 1);
+    }
   }
   if (changed) atomicAdd(dlt, changed);
   __syncthreads();
@@ -220,7 +240,7 @@ namespace {
 // the host). medoid/size: [n_groups][k], group-local member index or -1.
 gd_status km_run(gd_ctx *ctx, int n_groups, const int64_t *off, const float *features, int k,
                  float threshold, int max_pass, uint64_t seed, int level, int *medoid, int *size,
-                 int *passes) {
+                 int *passes, double *device_ms) {
   using namespace gd;
   if (n_groups < 0 || !off || (!features && off[n_groups] > 0) || !medoid || !size)
     return fail(ctx, GD_ERR_INVALID, "kmeans: bad arguments");
@@ -325,15 +345,17 @@ gd_status km_run(gd_ctx *ctx, int n_groups, const int64_t *off, const float *fea
   A.k = k;
   A.threshold = threshold;
   A.max_pass = max_pass;
-  const size_t lds_assign = (size_t)k * KM_DIM * 12 + (size_t)k * 4 + 16;
+  const size_t lds_assign = (size_t)k * KM_DIM * 8 + (size_t)k * 4 + 16;
   const size_t lds_medoid = (size_t)k * KM_DIM * 4;
   HIPCHK(hipFuncSetAttribute((const void *)gd_km_assign_kernel,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_assign));
+  HIPCHK(hipEventRecord(ctx->ev0, st));
   for (int p = 0; p < max_pass;) {
     const int batch = std::min(8, max_pass - p);
     for (int b = 0; b < batch; ++b) {
       if (b == batch - 1) HIPCHK(hipMemsetAsync(A.active, 0, 4, st));
-      hipLaunchKernelGGL(gd_km_assign_kernel, dim3(n_tiles), dim3(256), lds_assign, st, A);
+      hipLaunchKernelGGL(gd_km_assign_kernel, dim3(n_tiles), dim3(256), lds_assign, st, A,
+                         (const float *)A.cent);
       HIPCHK(hipGetLastError());
       hipLaunchKernelGGL(gd_km_update_kernel, dim3(n_groups), dim3(256), 0, st, A);
       HIPCHK(hipGetLastError());
@@ -346,12 +368,18 @@ gd_status km_run(gd_ctx *ctx, int n_groups, const int64_t *off, const float *fea
   }
   hipLaunchKernelGGL(gd_km_medoid_kernel, dim3(n_tiles), dim3(256), lds_medoid, st, A);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev1, st));
   std::vector<unsigned long long> best(gk);
   std::vector<int> sz(gk), pass(n_groups);
   HIPCHK(hipMemcpyAsync(best.data(), A.best, gk * 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(sz.data(), A.size, gk * 4, hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(pass.data(), A.pass, (size_t)n_groups * 4, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  if (device_ms) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *device_ms += ms;
+  }
   for (int g = 0; g < n_groups; ++g) {
     if (off[g + 1] - off[g] < k) continue;
     if (passes) passes[g] = pass[g];
@@ -372,16 +400,20 @@ extern "C" {
 
 gd_status gd_kmeans_medoids(gd_ctx *ctx, int n_groups, const int64_t *group_off,
                             const float *features, int k, float threshold, int max_pass,
-                            uint64_t seed, int *medoid, int *size, int *passes) {
+                            uint64_t seed, int *medoid, int *size, int *passes,
+                            double *device_ms) {
   if (!ctx) return GD_ERR_INVALID;
+  if (device_ms) *device_ms = 0.0;
   return km_run(ctx, n_groups, group_off, features, k, threshold, max_pass, seed, 0, medoid, size,
-                passes);
+                passes, device_ms);
 }
 
 gd_status gd_post_mc(gd_ctx *ctx, const gd_record *recs, int64_t n, int k_replica, int k_final,
-                     uint64_t seed, int64_t *medoid_rec, int *medoid_size, int *n_medoids) {
+                     uint64_t seed, int64_t *medoid_rec, int *medoid_size, int *n_medoids,
+                     double *device_ms) {
   using namespace gd;
   if (!ctx) return GD_ERR_INVALID;
+  if (device_ms) *device_ms = 0.0;
   if (n < 0 || (n > 0 && !recs) || !medoid_rec || !medoid_size || !n_medoids)
     return fail(ctx, GD_ERR_INVALID, "post_mc: bad arguments");
   *n_medoids = 0;
@@ -413,7 +445,7 @@ gd_status gd_post_mc(gd_ctx *ctx, const gd_record *recs, int64_t n, int k_replic
     for (int d = 0; d < KM_DIM; ++d) feat[(size_t)i * KM_DIM + d] = recs[order[i]].energy.e[d];
   std::vector<int> med((size_t)n_groups * k_replica), sz((size_t)n_groups * k_replica);
   TRY(km_run(ctx, n_groups, off.data(), feat.data(), k_replica, 0.001f, 501, seed, 1, med.data(),
-             sz.data(), nullptr));
+             sz.data(), nullptr, device_ms));
   // level 2 over the level-1 medoids, in replica and cluster order
   std::vector<int64_t> l1;
   for (int g = 0; g < n_groups; ++g)
@@ -427,7 +459,7 @@ gd_status gd_post_mc(gd_ctx *ctx, const gd_record *recs, int64_t n, int k_replic
   const int64_t off2[2] = {0, (int64_t)l1.size()};
   std::vector<int> med2(k_final), sz2(k_final);
   TRY(km_run(ctx, 1, off2, f2.data(), k_final, 0.001f, 501, seed, 2, med2.data(), sz2.data(),
-             nullptr));
+             nullptr, device_ms));
   int m = 0;
   for (int c = 0; c < k_final; ++c)
     if (med2[c] >= 0) {

@@ -337,17 +337,21 @@ gd_status gd_sur_train(gd_ctx *ctx, int pose_begin, int pose_end, int epochs, in
 
 /* groups: objects [group_off[g], group_off[g+1]) of features (row-major,
  * 9 floats per object). medoid, size: [n_groups * k], group-local index or
- * -1 for an empty cluster; passes (optional): [n_groups]. */
+ * -1 for an empty cluster; passes (optional): [n_groups]; device_ms
+ * (optional): device time of the passes and the medoid search. */
 gd_status gd_kmeans_medoids(gd_ctx *ctx, int n_groups, const int64_t *group_off,
                             const float *features, int k, float threshold, int max_pass,
-                            uint64_t seed, int *medoid, int *size, int *passes);
+                            uint64_t seed, int *medoid, int *size, int *passes,
+                            double *device_ms);
 
 /* post_mc (src/post_mc.C:40-101): the records of each replica (ascending
  * replica id, in the order given) clustered into k_replica clusters, then
  * all their medoids, in replica and cluster order, into k_final. Writes up to
- * k_final medoids as record indices with their level-2 cluster sizes. */
+ * k_final medoids as record indices with their level-2 cluster sizes;
+ * device_ms (optional): device time of both levels. */
 gd_status gd_post_mc(gd_ctx *ctx, const gd_record *recs, int64_t n, int k_replica, int k_final,
-                     uint64_t seed, int64_t *medoid_rec, int *medoid_size, int *n_medoids);
+                     uint64_t seed, int64_t *medoid_rec, int *medoid_size, int *n_medoids,
+                     double *device_ms);
 
 #ifdef __cplusplus
 }

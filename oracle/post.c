@@ -30,7 +30,7 @@ static float dist2(const float *a, const float *b) {
   float s = 0.0f;
   for (int d = 0; d < KM_DIM; ++d) {
     const float t = a[d] - b[d];
-    s = s + t * t;
+    s = fmaf(t, t, s);
   }
   return s;
 }
