@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "geauxdock.h"
+#include "geauxdock_io.h"
 
 namespace geauxdock {
 
@@ -252,21 +253,29 @@ void RunT(const Ligand *lig, const Protein *prt, const Psp *psp, const Kde *kde,
   mclog->t1 += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_wall0).count();
 }
 
-// printStates (src/util.C:1504-1550): one line per record,
-// "lig prt cms rmsd mv1..mv6 vdw ele pmf psp hdb hpc kde lhm edst ener", 4 decimals.
+// printStates (src/util.C:1504-1550) through gd_write_states_csv: one line per
+// record, "lig prt cms rmsd mv1..mv6 vdw ele pmf psp hdb hpc kde lhm edst ener",
+// 4 decimals.
 template <class Record>
 void WriteStates(const std::vector<Record> &steps, const std::string &path) {
-  std::ofstream f(path, std::ofstream::out);
-  if (!f) throw Error(GD_ERR_INVALID, "cannot open " + path);
-  f << std::fixed << std::setprecision(4);
-  f << "lig prt cms rmsd mv1 mv2 mv3 mv4 mv5 mv6 vdw ele pmf psp hdb hpc kde lhm edst ener\n";
-  for (const Record &s : steps) {
-    f << s.replica.idx_lig << " " << s.replica.idx_prt << " " << s.energy.cms << " "
-      << s.energy.rmsd << " ";
-    for (int i = 0; i < 6; ++i) f << s.movematrix[i] << " ";
-    for (int i = 0; i < 9; ++i) f << s.energy.e[i] << " ";
-    f << s.energy.e[9] << "\n";
+  std::vector<gd_record> recs(steps.size());
+  for (size_t i = 0; i < steps.size(); ++i) {
+    const Record &s = steps[i];
+    gd_record &r = recs[i];
+    r.replica.idx_rep = s.replica.idx_rep;
+    r.replica.idx_prt = s.replica.idx_prt;
+    r.replica.idx_tmp = s.replica.idx_tmp;
+    r.replica.idx_lig = s.replica.idx_lig;
+    for (int k = 0; k < GD_MAXWEI; ++k) r.energy.e[k] = s.energy.e[k];
+    r.energy.cms = s.energy.cms;
+    r.energy.rmsd = s.energy.rmsd;
+    for (int k = 0; k < 6; ++k) r.movematrix[k] = s.movematrix[k];
+    r.step = s.step;
   }
+  char err[512];
+  const gd_status st = gd_write_states_csv(path.c_str(), recs.data(), (int64_t)recs.size(), err,
+                                           sizeof err);
+  if (st != GD_OK) throw Error(st, err);
 }
 
 }  // namespace geauxdock
