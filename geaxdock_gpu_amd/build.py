@@ -13,6 +13,10 @@ HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # keep packed FP32 (v_pk_*_f32) out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # branches on wave-uniform conditions (ballots in the pair loop) stay plain
+    # scalar branches: without this the control-flow structurizer routes them
+    # through flow blocks and copies every accumulator at the join
+    "-mllvm", "-structurizecfg-skip-uniform-regions=true",
 ]
 
 
