@@ -281,7 +281,7 @@ def da_compare(args, cxs, desc):
 # records: rocprofv3 FETCH_SIZE 3681.1 KiB x 2 + WRITE_SIZE 786.8 KiB x 1 (the
 # factors a plain 256 MiB device copy gives), from
 # profiles/gd_mc_kernel_apl3_ensemble_3520rep_traffic.md
-TRAFFIC_ENSEMBLE_2000 = (3681.125 * 2 + 786.8125) * 1024
+TRAFFIC_ENSEMBLE_2000 = (3690.125 * 2 + 786.8125) * 1024
 
 
 def roofline(pairs_per_step, steps_per_launch, kernel_ms, kname, desc):
