@@ -13,6 +13,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <memory>
+#include <new>
+#include <stdexcept>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -489,6 +492,10 @@ bool is_number(const std::string &s) {
   return end && *end == 0;
 }
 
+// Sanity bounds on indices read from files (they size host tables).
+constexpr int kMaxResidue = 99999;
+constexpr int kMaxMcsField = 4095;
+
 gd_status load_ff(const char *path, const char *complex_id, LhmData &H, Err &e) {
   std::vector<std::string> lines;
   if (!read_lines(path, lines)) return e.fail(GD_ERR_INVALID, "cannot open %s", path);
@@ -517,7 +524,10 @@ gd_status load_ff(const char *path, const char *complex_id, LhmData &H, Err &e) 
     } else if (t[0] == "PSP" && t.size() >= 4) {
       const int c = lig_code(t[2]);
       if (c < 0) continue;
-      H.psp_res.push_back(std::atoi(t[1].c_str()));
+      const int r = std::atoi(t[1].c_str());
+      if (r < 0 || r > kMaxResidue)
+        return e.fail(GD_ERR_INVALID, "%s: PSP residue %d outside [0, %d]", path, r, kMaxResidue);
+      H.psp_res.push_back(r);
       H.psp_t.push_back(c);
       H.psp_v.push_back(fnum(t[3]));
     } else if (t[0] == "MCS" && t.size() >= 4) {
@@ -532,7 +542,10 @@ gd_status load_ff(const char *path, const char *complex_id, LhmData &H, Err &e) 
       if ((int)t.size() < (int)(o + 2 + 4 * n))
         return e.fail(GD_ERR_INVALID, "%s: MCS record with %d fields is short", path, n);
       for (int k = 0; k < n; ++k) {
-        m.idx.push_back(std::atoi(t[o + 2 + 4 * k].c_str()));
+        const int ai = std::atoi(t[o + 2 + 4 * k].c_str());
+        if (ai < -1 || ai > kMaxMcsField)
+          return e.fail(GD_ERR_INVALID, "%s: MCS field %d outside [-1, %d]", path, ai, kMaxMcsField);
+        m.idx.push_back(ai);
         m.x.push_back(fnum(t[o + 3 + 4 * k]));
         m.y.push_back(fnum(t[o + 4 + 4 * k]));
         m.z.push_back(fnum(t[o + 5 + 4 * k]));
@@ -563,14 +576,45 @@ struct gd_loaded {
 
 extern "C" {
 
+namespace {
+gd_status load_complex_impl(const gd_input_files *files, const gd_dock_params *params,
+                            gd_loaded **out, Err &e);
+gd_status sdf_read_impl(const char *path, int record, int centre_each, gd_sdf_record *info,
+                        float *xyz, int max_conf, float *rmsd, int max_rmsd, int *types,
+                        float *charges, Err &e);
+gd_status write_pose_sdf_impl(const char *ligand_sdf, int lna, const float *x, const float *y,
+                              const float *z, const char *out_path, Err &e);
+}  // namespace
+
+// The loaders allocate host tables sized by the files; an allocation failure
+// or a library exception becomes a status, never an exception across the ABI.
+#define GD_IO_GUARD(call)                                                   \
+  try {                                                                     \
+    return call;                                                            \
+  } catch (const std::bad_alloc &) {                                        \
+    return e.fail(GD_ERR_NOMEM, "out of host memory");                      \
+  } catch (const std::exception &x) {                                       \
+    return e.fail(GD_ERR_INVALID, "input error: %s", x.what());             \
+  }
+
 gd_status gd_load_complex(const gd_input_files *files, const gd_dock_params *params,
                           gd_loaded **out, char *err, size_t err_len) {
   Err e{err, err_len};
   if (err && err_len) err[0] = 0;
+  if (out) *out = nullptr;
+  GD_IO_GUARD(load_complex_impl(files, params, out, e))
+}
+
+}  // extern "C"
+
+namespace {
+gd_status load_complex_impl(const gd_input_files *files, const gd_dock_params *params,
+                            gd_loaded **out, Err &e) {
   if (!files || !params || !out || !files->ligand_sdf || !files->protein_pdb || !files->lhm_ff ||
       !files->enepara)
     return e.fail(GD_ERR_INVALID, "missing input file path");
-  if (params->n_tmp < 1 || params->n_tmp > GD_MAX_NTMP || !(params->floor_temp > 0.0f))
+  if (params->n_tmp < 1 || params->n_tmp > GD_MAX_NTMP || !(params->floor_temp > 0.0f) ||
+      (params->n_tmp > 1 && !(params->ceiling_temp > 0.0f)))
     return e.fail(GD_ERR_INVALID, "bad temperature parameters");
   *out = nullptr;
   RawPara raw;
@@ -586,7 +630,8 @@ gd_status gd_load_complex(const gd_input_files *files, const gd_dock_params *par
   if ((s = load_pdb(files->protein_pdb, P, e)) != GD_OK) return s;
   if ((s = load_ff(files->lhm_ff, files->complex_id, H, e)) != GD_OK) return s;
 
-  gd_loaded *g = new gd_loaded;
+  std::unique_ptr<gd_loaded> owner(new gd_loaded);
+  gd_loaded *g = owner.get();
   g->sections = L.sections;
   g->ens_total = L.ens_total;
   g->residues = P.n_res;
@@ -625,10 +670,8 @@ gd_status gd_load_complex(const gd_input_files *files, const gd_dock_params *par
     g->prt_z.insert(g->prt_z.end(), P.z[c].begin(), P.z[c].end());
   }
   for (int s3 : g->prt_seq3r)
-    if (s3 < 0) {
-      delete g;
-      return e.fail(GD_ERR_INVALID, "negative residue serial number");
-    }
+    if (s3 < 0 || s3 > kMaxResidue)
+      return e.fail(GD_ERR_INVALID, "residue serial number %d outside [0, %d]", s3, kMaxResidue);
 
   // PSP table [ligand type][residue serial]
   int n_res = max_seq + 1;
@@ -716,9 +759,12 @@ gd_status gd_load_complex(const gd_input_files *files, const gd_dock_params *par
   for (int i = 0; i < 3; ++i) d.move_scale[i] = params->ts;
   for (int i = 3; i < 6; ++i) d.move_scale[i] = params->rs;
   d.move_kind = params->move_kind;
-  *out = g;
+  *out = owner.release();
   return GD_OK;
 }
+}  // namespace
+
+extern "C" {
 
 const gd_complex_desc *gd_loaded_desc(const gd_loaded *l) { return l ? &l->d : nullptr; }
 
@@ -738,6 +784,16 @@ gd_status gd_sdf_read(const char *path, int record, int centre_each, gd_sdf_reco
                       float *charges, char *err, size_t err_len) {
   Err e{err, err_len};
   if (err && err_len) err[0] = 0;
+  GD_IO_GUARD(sdf_read_impl(path, record, centre_each, info, xyz, max_conf, rmsd, max_rmsd, types,
+                            charges, e))
+}
+
+}  // extern "C"
+
+namespace {
+gd_status sdf_read_impl(const char *path, int record, int centre_each, gd_sdf_record *info,
+                        float *xyz, int max_conf, float *rmsd, int max_rmsd, int *types,
+                        float *charges, Err &e) {
   if (!path || !info) return e.fail(GD_ERR_INVALID, "missing path or info");
   LigandData L;
   gd_status s;
@@ -763,6 +819,9 @@ gd_status gd_sdf_read(const char *path, int record, int centre_each, gd_sdf_reco
   if (charges) std::copy(L.c.begin(), L.c.end(), charges);
   return GD_OK;
 }
+}  // namespace
+
+extern "C" {
 
 // ---- trajectory output --------------------------------------------------------
 
@@ -927,6 +986,14 @@ gd_status gd_write_pose_sdf(const char *ligand_sdf, int lna, const float *x, con
                             const float *z, const char *out_path, char *err, size_t err_len) {
   Err e{err, err_len};
   if (err && err_len) err[0] = 0;
+  GD_IO_GUARD(write_pose_sdf_impl(ligand_sdf, lna, x, y, z, out_path, e))
+}
+
+}  // extern "C"
+
+namespace {
+gd_status write_pose_sdf_impl(const char *ligand_sdf, int lna, const float *x, const float *y,
+                              const float *z, const char *out_path, Err &e) {
   if (!ligand_sdf || !out_path || !x || !y || !z || lna <= 0)
     return e.fail(GD_ERR_INVALID, "bad arguments");
   std::ifstream in(ligand_sdf);
@@ -949,5 +1016,4 @@ gd_status gd_write_pose_sdf(const char *ligand_sdf, int lna, const float *x, con
   const bool ok = std::fclose(f) == 0;
   return ok ? GD_OK : e.fail(GD_ERR_INVALID, "error writing %s", out_path);
 }
-
-}  // extern "C"
+}  // namespace

@@ -292,3 +292,19 @@ def test_ca_points_are_ca_atoms_per_model(files):
                 if s == seqs[r] and nm not in (" N  ", " CA ", " C  ", " O  ")]
         np.testing.assert_allclose([px[0, i], py[0, i], pz[0, i]], np.mean(side, axis=0),
                                    atol=1e-4)
+
+
+@pytest.mark.parametrize("line,msg", [("PSP 1000000000 C.3 0.5", "PSP residue"),
+                                      ("MCS 1a07C1 0.5 1 99999 1.0 2.0 3.0", "MCS field")])
+def test_out_of_range_indices_in_ff_are_errors(tmp_path, line, msg):
+    """Indices that size host tables are bounded: a corrupt .ff line is an
+    error, not a huge allocation."""
+    ff = tmp_path / "bad.ff"
+    ff.write_text(open(A07[2]).read().rstrip("\n") + "\n" + line + "\n")
+    with pytest.raises(GeauxDockError, match=msg):
+        io.load_complex(A07[0], A07[1], str(ff), PARAS, complex_id="1a07C1")
+
+
+def test_bad_temperature_ladder_is_an_error():
+    with pytest.raises(GeauxDockError, match="temperature"):
+        io.load_complex(*A07, PARAS, n_tmp=4, floor_temp=0.3, ceiling_temp=0.0)
