@@ -30,6 +30,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <new>
 #include <vector>
 
 namespace gd {
@@ -404,18 +405,49 @@ gd_status gd_kmeans_medoids(gd_ctx *ctx, int n_groups, const int64_t *group_off,
                             double *device_ms) {
   if (!ctx) return GD_ERR_INVALID;
   if (device_ms) *device_ms = 0.0;
-  return km_run(ctx, n_groups, group_off, features, k, threshold, max_pass, seed, 0, medoid, size,
-                passes, device_ms);
+  try {
+    return km_run(ctx, n_groups, group_off, features, k, threshold, max_pass, seed, 0, medoid,
+                  size, passes, device_ms);
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, GD_ERR_NOMEM, "kmeans: out of host memory");
+  }
 }
+
+}  // extern "C"
+
+namespace {
+gd_status post_mc_impl(gd_ctx *ctx, const gd_record *recs, int64_t n, int k_replica, int k_final,
+                       uint64_t seed, int64_t *medoid_rec, int *medoid_size, int *n_medoids,
+                       double *device_ms);
+}  // namespace
+
+extern "C" {
 
 gd_status gd_post_mc(gd_ctx *ctx, const gd_record *recs, int64_t n, int k_replica, int k_final,
                      uint64_t seed, int64_t *medoid_rec, int *medoid_size, int *n_medoids,
                      double *device_ms) {
-  using namespace gd;
   if (!ctx) return GD_ERR_INVALID;
+  try {
+    return post_mc_impl(ctx, recs, n, k_replica, k_final, seed, medoid_rec, medoid_size,
+                        n_medoids, device_ms);
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, GD_ERR_NOMEM, "post_mc: out of host memory");
+  }
+}
+
+}  // extern "C"
+
+namespace {
+gd_status post_mc_impl(gd_ctx *ctx, const gd_record *recs, int64_t n, int k_replica, int k_final,
+                       uint64_t seed, int64_t *medoid_rec, int *medoid_size, int *n_medoids,
+                       double *device_ms) {
+  using namespace gd;
   if (device_ms) *device_ms = 0.0;
   if (n < 0 || (n > 0 && !recs) || !medoid_rec || !medoid_size || !n_medoids)
     return fail(ctx, GD_ERR_INVALID, "post_mc: bad arguments");
+  if (k_replica < 1 || k_replica > KM_MAX_K || k_final < 1 || k_final > KM_MAX_K)
+    return fail(ctx, GD_ERR_CAPACITY, "post_mc: k %d / %d outside [1, %d]", k_replica, k_final,
+                KM_MAX_K);
   *n_medoids = 0;
   if (n == 0) return GD_OK;
   // records grouped by replica, ascending replica, each in the given order
@@ -470,5 +502,4 @@ gd_status gd_post_mc(gd_ctx *ctx, const gd_record *recs, int64_t n, int k_replic
   *n_medoids = m;
   return GD_OK;
 }
-
-}  // extern "C"
+}  // namespace
