@@ -18,7 +18,7 @@ from . import abi
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgeauxdock.so")
 
-__all__ = ["Engine", "GeauxDockError", "load_library", "run", "abi"]
+__all__ = ["Engine", "GeauxDockError", "load_library", "run", "summary", "abi"]
 
 
 class GeauxDockError(RuntimeError):
@@ -429,3 +429,23 @@ def run(cx, steps_total, steps_per_exchange=20, seed=0, steps_per_dump=2000, dev
                  ar=n_rec / float(cx.n_rep * max(steps_total, 1)),
                  mc_sweeps_per_s=steps_total * cx.n_rep / st["t_compute"] if st["t_compute"] else 0.0)
     return recs, mclog
+
+
+def summary(mclog, cx):
+    """The performance part of PrintSummary (src/util.C:1281-1364) as text:
+    sizes, the acceptance ratio, and replica-steps ("MC sweeps") per second
+    over compute time (t0) and wall time (t1)."""
+    bar = "=" * 80
+    n_rep = cx.n_lig * cx.n_prt * cx.n_tmp
+    lines = [bar, f"lig conformations\t\t{cx.n_lig}", f"prt conformations\t\t{cx.n_prt}",
+             f"temperatures\t\t\t{cx.n_tmp}", f"replica ensembles\t\t{n_rep}",
+             f"size_lig\t\t\t{cx.lna}", f"size_prt\t\t\t{cx.pnp}", f"size_pnk\t\t\t{cx.pnk}",
+             f"size_mcs\t\t\t{cx.pos}", f"AR of MC\t\t\t{mclog['ar']:.3f}", bar, "Performance", bar]
+    n = float(mclog["steps_total"]) * n_rep
+    for name, t in (("compute time", mclog["t0"]), ("wall time", mclog["t1"])):
+        rate = n / t if t > 0 else 0.0
+        lines += [f"{name}\t\t\t{t:.3f} seconds",
+                  f"time per MC sweep per replica\t{(1e6 / rate if rate else 0.0):.3f} * 1e-6 seconds",
+                  f"MC sweeps per second\t\t{rate:.3f}", f"speedup over 843.75\t\t{rate / 843.75:.3f} X"]
+    lines.append(bar)
+    return "\n".join(lines)

@@ -253,6 +253,35 @@ void RunT(const Ligand *lig, const Protein *prt, const Psp *psp, const Kde *kde,
   mclog->t1 += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_wall0).count();
 }
 
+// The performance part of PrintSummary (src/util.C:1281-1364): sizes, the
+// acceptance ratio and replica-steps per second over compute time (t0) and
+// wall time (t1), with the reference's "speedup over 843.75" lines.
+template <class McLog, class ComplexSize>
+void PrintSummary(const McLog *mclog, const ComplexSize &cs, FILE *out = stdout) {
+  const char *bar =
+      "================================================================================\n";
+  std::fprintf(out, "%s", bar);
+  std::fprintf(out, "lig conformations\t\t%d\n", cs.n_lig);
+  std::fprintf(out, "prt conformations\t\t%d\n", cs.n_prt);
+  std::fprintf(out, "temperatures\t\t\t%d\n", cs.n_tmp);
+  std::fprintf(out, "replica ensembles\t\t%d\n", cs.n_rep);
+  std::fprintf(out, "size_lig\t\t\t%d\nsize_prt\t\t\t%d\nsize_pnk\t\t\t%d\nsize_mcs\t\t\t%d\n",
+               cs.lna, cs.pnp, cs.pnk, cs.pos);
+  std::fprintf(out, "AR of MC\t\t\t%.3f\n%sPerformance\n%s", mclog->ar, bar, bar);
+  const double n = (double)mclog->steps_total * cs.n_rep;
+  const double times[2] = {mclog->t0, mclog->t1};
+  const char *names[2] = {"compute time", "wall time"};
+  for (int i = 0; i < 2; ++i) {
+    const double rate = times[i] > 0 ? n / times[i] : 0.0;
+    std::fprintf(out, "%s\t\t\t%.3f seconds\n", names[i], times[i]);
+    std::fprintf(out, "time per MC sweep per replica\t%.3f * 1e-6 seconds\n",
+                 rate > 0 ? 1e6 / rate : 0.0);
+    std::fprintf(out, "MC sweeps per second\t\t%.3f\n", rate);
+    std::fprintf(out, "speedup over 843.75\t\t%.3f X\n", rate / 843.75);
+  }
+  std::fprintf(out, "%s", bar);
+}
+
 // printStates (src/util.C:1504-1550) through gd_write_states_csv: one line per
 // record, "lig prt cms rmsd mv1..mv6 vdw ele pmf psp hdb hpc kde lhm edst ener",
 // 4 decimals.

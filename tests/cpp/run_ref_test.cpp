@@ -233,6 +233,7 @@ int main(int argc, char **argv) {
   fwrite(lg, sizeof lg, 1, o);
   fclose(o);
   if (!records.empty()) geauxdock::WriteStates(records.begin()->second, argv[3]);
+  geauxdock::PrintSummary(log.get(), cs, stderr);
   printf("records %lld ar %.4f t0 %.4f t1 %.4f steps %d\n", (long long)n, log->ar, log->t0, log->t1,
          log->steps_total);
   return 0;

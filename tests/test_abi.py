@@ -83,3 +83,14 @@ def test_batch_workload_sizes():
     cxs = synth.config_batch(6, seed=2)
     for c in cxs:
         assert 12 <= c.lna <= 64 and 800 <= c.pnp <= 4000 and c.n_rep == 120
+
+
+def test_summary_text_matches_reference_layout():
+    """PrintSummary's performance block (src/util.C:1347-1364): replica-steps
+    per second over compute and wall time and the speed-up over 843.75."""
+    text = gd.summary(dict(t0=1.5, t1=2.0, ar=0.42, steps_total=5000), synth.config0())
+    lines = text.split("\n")
+    assert "replica ensembles\t\t8" in lines and "AR of MC\t\t\t0.420" in lines
+    assert "MC sweeps per second\t\t26666.667" in lines  # 5000 * 8 / 1.5
+    assert "speedup over 843.75\t\t31.605 X" in lines
+    assert "MC sweeps per second\t\t20000.000" in lines  # wall time
