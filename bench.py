@@ -94,14 +94,14 @@ def copy_probe(mib, reps=5):
 def workload(args):
     from geaxdock_gpu_amd import synth
     if args.config == "single":
-        cxs = [synth.config0(seed=0)]
+        cxs = [synth.single_complex(seed=0)]
         desc = dict(workload="single complex: 1 lig x 1 prt x 8 temps, lna 32, pnp 1200")
     elif args.config == "batch":
-        cxs = synth.config_batch(args.batch, seed=2)
+        cxs = synth.uneven_batch(args.batch, seed=2)
         desc = dict(workload=f"uneven batch: {args.batch} complexes, lna U[12,64], pnp U[800,4000], "
                              "5 lig x 3 prt x 8 temps each")
     else:
-        cxs = [synth.config1(seed=1)]
+        cxs = [synth.ensemble(seed=1)]
         desc = dict(workload="ensemble: 20 lig x 11 prt x 16 temps = 3520 replicas, lna 48, "
                              "pnp 3000, pnk 200, mcs 8x40")
     return cxs, desc
@@ -115,7 +115,7 @@ def cpu_baseline(cxs, args):
     from geaxdock_gpu_amd import synth
     cx0 = cxs[0]
     if args.config == "ensemble":
-        sub = synth.config1(seed=1, n_lig=2, n_prt=1)
+        sub = synth.ensemble(seed=1, n_lig=2, n_prt=1)
     elif args.config == "single":
         sub = cx0
     else:

@@ -273,7 +273,7 @@ def make_complex(seed, lna=32, pnp=1200, pnk=100, pos=4, mcs_atoms=20, n_lig=1, 
 
 # ---- the benchmark workloads ------------------------------------------------
 
-def config0(seed=0, **kw):
+def single_complex(seed=0, **kw):
     """One complex, CPU-runnable: 1,200-point receptor in a 30 A cube, 32-atom
     ligand, 1 x 1 x 8 replicas, KDE 100, MCS 4 x 20."""
     args = dict(lna=32, pnp=1200, pnk=100, pos=4, mcs_atoms=20, n_lig=1, n_prt=1, n_tmp=8,
@@ -282,7 +282,7 @@ def config0(seed=0, **kw):
     return make_complex(seed, **args)
 
 
-def config1(seed=1, **kw):
+def ensemble(seed=1, **kw):
     """Ensemble: 3,000-point receptor in a 40 A cube with 11 conformers, 48-atom
     ligand with 20 conformers, 16 temperatures (3,520 replicas), KDE 200,
     MCS 8 x 40."""
@@ -292,7 +292,7 @@ def config1(seed=1, **kw):
     return make_complex(seed, **args)
 
 
-def config_batch(n_complex, seed=2, lna_range=(12, 64), pnp_range=(800, 4000), n_lig=5,
+def uneven_batch(n_complex, seed=2, lna_range=(12, 64), pnp_range=(800, 4000), n_lig=5,
                  n_prt=3, n_tmp=8, **kw):
     """Uneven batch: ligand atoms U[12,64], receptor points U[800,4000], each
     complex 5 ligand x 3 protein conformers x 8 temperatures."""

@@ -48,7 +48,7 @@ def _have_gpu():
 def test_no_device_or_bad_input_is_an_error_not_a_crash():
     if _have_gpu():
         with gd.Engine() as eng:
-            cx = synth.config0(seed=0)
+            cx = synth.single_complex(seed=0)
             cx.lig_t = cx.lig_t.copy()
             cx.lig_t[3] = 99  # type out of range
             with pytest.raises(gd.GeauxDockError) as ei:
@@ -61,12 +61,12 @@ def test_no_device_or_bad_input_is_an_error_not_a_crash():
 
 
 def test_synthetic_workloads_are_seeded():
-    a, b = synth.config0(seed=4), synth.config0(seed=4)
+    a, b = synth.single_complex(seed=4), synth.single_complex(seed=4)
     for k in ("prt_x", "lig_x", "prt_t", "kde_x", "mcs_x", "psp", "enepara"):
         assert np.array_equal(getattr(a, k), getattr(b, k)), k
-    c = synth.config0(seed=5)
+    c = synth.single_complex(seed=5)
     assert not np.array_equal(a.prt_x, c.prt_x)
-    cx = synth.config1(seed=1)
+    cx = synth.ensemble(seed=1)
     assert cx.n_rep == 3520 and cx.lna == 48 and cx.pnp == 3000 and cx.pnk == 200 and cx.pos == 8
     # receptor points keep out of the 10 A void; conformers differ by ~0.5 A
     x = np.stack([cx.prt_x, cx.prt_y, cx.prt_z]).reshape(3, cx.n_prt, cx.pnp)
@@ -80,7 +80,7 @@ def test_synthetic_workloads_are_seeded():
 
 
 def test_batch_workload_sizes():
-    cxs = synth.config_batch(6, seed=2)
+    cxs = synth.uneven_batch(6, seed=2)
     for c in cxs:
         assert 12 <= c.lna <= 64 and 800 <= c.pnp <= 4000 and c.n_rep == 120
 
@@ -88,7 +88,7 @@ def test_batch_workload_sizes():
 def test_summary_text_matches_reference_layout():
     """PrintSummary's performance block (src/util.C:1347-1364): replica-steps
     per second over compute and wall time and the speed-up over 843.75."""
-    text = gd.summary(dict(t0=1.5, t1=2.0, ar=0.42, steps_total=5000), synth.config0())
+    text = gd.summary(dict(t0=1.5, t1=2.0, ar=0.42, steps_total=5000), synth.single_complex())
     lines = text.split("\n")
     assert "replica ensembles\t\t8" in lines and "AR of MC\t\t\t0.420" in lines
     assert "MC sweeps per second\t\t26666.667" in lines  # 5000 * 8 / 1.5

@@ -39,7 +39,7 @@ def _engine(gd, cx, seed, params_seed=3, tab=False, **kw):
 
 def test_da_scale0_tracks_cpu_path(gd):
     from oracle import gdo
-    cx = synth.config0(seed=0)
+    cx = synth.single_complex(seed=0)
     n_steps, nc, k = 400, 20, 4
     recs = []
     eng, cid = _engine(gd, cx, 11)
@@ -70,7 +70,7 @@ def test_da_scale0_tracks_cpu_path(gd):
 
 @pytest.mark.parametrize("tab", [False, True])
 def test_da_states_carry_exact_energy(gd, tab):
-    cx = synth.config0(seed=1)
+    cx = synth.single_complex(seed=1)
     recs = []
     eng, cid = _engine(gd, cx, 5, tab=tab)
     eng.run_da(240, 20, 4, 3.0, sink=lambda c, r: recs.append(r))
@@ -90,7 +90,7 @@ def test_da_states_carry_exact_energy(gd, tab):
 
 @pytest.mark.parametrize("tab", [False, True])
 def test_da_deterministic_and_window_independent(gd, tab):
-    cx = synth.config0(seed=2)
+    cx = synth.single_complex(seed=2)
     out = []
     for dump in (40, 2000):
         eng, cid = _engine(gd, cx, 9, tab=tab, steps_per_dump=dump)
@@ -102,7 +102,7 @@ def test_da_deterministic_and_window_independent(gd, tab):
 
 
 def test_da_rejects_bad_arguments(gd):
-    cx = synth.config0(seed=2)
+    cx = synth.single_complex(seed=2)
     with gd.Engine(seed=1) as eng:
         eng.add_complex(cx)
         with pytest.raises(gd.GeauxDockError):
@@ -158,7 +158,7 @@ def test_surrogate_total_matches_oracle(gd, tab):
     sum|y| / lna + 1e-4 (1 + |cheap|)."""
     from oracle import gdo
     from oracle import surrogate as so
-    cx = synth.config0(seed=2, void_radius=7.0)
+    cx = synth.single_complex(seed=2, void_radius=7.0)
     spacing, half, scale = 0.8, 9.6, 7.0
     p = so.init_params(3)
     with gd.Engine(seed=1) as eng:

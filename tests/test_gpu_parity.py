@@ -61,7 +61,7 @@ def check_energy_parity(oracle, eng, cid, cx, mm, idx_lig, idx_prt, report):
 @pytest.mark.parametrize("lpr,apl,split", [(8, 4, 1), (8, 4, 2), (16, 2, 1), (16, 3, 1),
                                            (16, 3, 2), (32, 1, 2), (64, 1, 1)])
 def test_energy_terms_vs_fp64(oracle, eng_mod, lpr, apl, split):
-    cx = synth.config0(seed=3, identity_norm=True, void_radius=7.0)
+    cx = synth.single_complex(seed=3, identity_norm=True, void_radius=7.0)
     rng = np.random.default_rng(11)
     mm = random_poses(rng, 48)
     with eng_mod.Engine(seed=1, lanes_per_replica=lpr, atoms_per_lane=apl,
@@ -95,7 +95,7 @@ def test_energy_terms_weighted_multiconformer(oracle, eng_mod):
 def test_start_pose_mcc_is_one(oracle, eng_mod):
     """Replica 0 in its start pose reproduces the reference contact map exactly:
     cms = 1 whenever the map has contacts (src/kernel_cuda_l2_calcmcc.cu)."""
-    cx = synth.config0(seed=4, void_radius=5.5)
+    cx = synth.single_complex(seed=4, void_radius=5.5)
     d = cx.desc()
     ref = oracle.refmap_f64(d, cx)
     assert ref.sum() > 0
@@ -181,7 +181,7 @@ def test_mc_run_tracks_cpu_path(oracle, eng_mod, split):
     """Same seed, same random stream: the GPU engine and the FP32 CPU path make
     the same accept / exchange decisions, step for step, unless an energy falls
     within rounding of a threshold. Both receptor-split layouts."""
-    cx = synth.config0(seed=0)
+    cx = synth.single_complex(seed=0)
     n_steps, nc = 400, 20
     recs, state, st = _run_gpu(eng_mod, cx, 7, n_steps, nc, receptor_split=split)
     d = cx.desc()
@@ -206,7 +206,7 @@ def test_mc_run_tracks_cpu_path(oracle, eng_mod, split):
 
 
 def test_checkpoint_resume_bit_exact(eng_mod):
-    cx = synth.config0(seed=1)
+    cx = synth.single_complex(seed=1)
     with eng_mod.Engine(seed=5, record=False) as eng:
         cid = eng.add_complex(cx)
         eng.run(60, 20)
@@ -223,7 +223,7 @@ def test_checkpoint_resume_bit_exact(eng_mod):
 def test_run_deterministic_and_window_independent(eng_mod):
     """A run is a function of the seed alone: record windows of 50 or 2000
     steps give bit-identical states."""
-    cx = synth.config0(seed=2)
+    cx = synth.single_complex(seed=2)
     _, s1, _ = _run_gpu(eng_mod, cx, 9, 300, 20, steps_per_dump=50)
     _, s2, _ = _run_gpu(eng_mod, cx, 9, 300, 20, steps_per_dump=2000)
     for k in s1:
@@ -241,7 +241,7 @@ def _batch_states(eng_mod, cxs, seed, n_steps, nc, **kw):
 def test_launch_modes_bit_identical(eng_mod):
     """Graph-captured, eager, persistent and non-persistent launches of an
     uneven batch give bit-identical states (graph replays included)."""
-    cxs = synth.config_batch(3, seed=5, lna_range=(12, 40), pnp_range=(800, 1600))
+    cxs = synth.uneven_batch(3, seed=5, lna_range=(12, 40), pnp_range=(800, 1600))
     ref = None
     for launch in (0, abi.LAUNCH_NO_GRAPH, abi.LAUNCH_PERSISTENT,
                    abi.LAUNCH_PERSISTENT | abi.LAUNCH_NO_GRAPH):
@@ -258,7 +258,7 @@ def test_launch_modes_bit_identical(eng_mod):
 def test_batch_tracks_cpu_path(oracle, eng_mod, split, ladders):
     """An uneven batch in one engine: every complex follows the FP32 CPU path
     of that complex (its own Philox streams, keyed by complex id)."""
-    cxs = synth.config_batch(3, seed=6, lna_range=(12, 40), pnp_range=(800, 1600), n_lig=2,
+    cxs = synth.uneven_batch(3, seed=6, lna_range=(12, 40), pnp_range=(800, 1600), n_lig=2,
                              n_prt=2, n_tmp=4)
     n_steps, nc = 200, 20
     recs = {c: [] for c in range(len(cxs))}

@@ -35,7 +35,7 @@ def write_input(cx, path, steps, dump, nc, exchange, seed):
 def test_cpp_run_matches_c_api(tmp_path):
     import geaxdock_gpu_amd as gd
     assert os.path.exists(BIN), "build tests/cpp/run_ref_test first (__graft_entry__.build)"
-    cx = synth.config0(seed=6, move_kind=abi.MOVE_UNIFORM, trans_sigma=0.05,
+    cx = synth.single_complex(seed=6, move_kind=abi.MOVE_UNIFORM, trans_sigma=0.05,
                        rot_sigma_deg=2.0, n_lig=2, n_prt=2)
     steps, dump, nc, seed = 300, 120, 20, 0x1234_5678_9ABC
     inp, out, csv = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "states.csv"

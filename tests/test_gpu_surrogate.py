@@ -50,7 +50,7 @@ def poses(rng, n, sigma=2.0):
 @pytest.fixture(scope="module")
 def setup():
     import geaxdock_gpu_amd as gd
-    cx = synth.config0(seed=2, void_radius=7.0)
+    cx = synth.single_complex(seed=2, void_radius=7.0)
     eng = gd.Engine(seed=1)
     cid = eng.add_complex(cx)
     spacing, half = 0.8, 9.6
@@ -135,7 +135,7 @@ def test_dataset_add_multi_matches_per_complex_adds():
     """Poses of several complexes appended in one interleaved call give the
     same rows (hence predictions) as per-complex appends."""
     import geaxdock_gpu_amd as gd
-    cxs = synth.config_batch(2, seed=8, lna_range=(12, 30), pnp_range=(800, 1200))
+    cxs = synth.uneven_batch(2, seed=8, lna_range=(12, 30), pnp_range=(800, 1200))
     rng = np.random.default_rng(3)
     n = 40
     cid = rng.integers(0, 2, n).astype(np.int32)

@@ -341,17 +341,17 @@ def test_dst_rmsd_and_rigid_placement(oracle):
 def test_unopt_weights_reduce_to_vdw_plus_dst(oracle):
     """With WEI 1 0 ... 0 1, NOA 1, NOB 0 (data/parameters/unopt_paras) the
     total is vdw/lna + |t|."""
-    cx = synth.config0(seed=5, identity_norm=True, weights=[1.0] + [0.0] * 7 + [1.0, 0.0])
+    cx = synth.single_complex(seed=5, identity_norm=True, weights=[1.0] + [0.0] * 7 + [1.0, 0.0])
     t = np.float32([0.3, 0.2, -0.4, 0.1, 0.2, 0.3])
     o = e64(oracle, cx, t)
     assert o.e[9] == pytest.approx(o.raw[0] + np.linalg.norm(t[:3].astype(np.float64)), rel=1e-12)
 
 
 def test_mcc_start_pose_and_empty_map(oracle):
-    cx = synth.config0(seed=4, void_radius=5.5)
+    cx = synth.single_complex(seed=4, void_radius=5.5)
     o = e64(oracle, cx)
     assert o.tp > 0 and o.tn > 0 and o.fp == 0 and o.fn == 0 and o.cms == 1.0
-    cx = synth.config0(seed=4, void_radius=12.0)
+    cx = synth.single_complex(seed=4, void_radius=12.0)
     o = e64(oracle, cx)
     assert o.tp == 0 and o.cms == 0.0  # empty reference map: denominator 0 -> 0
 
@@ -405,7 +405,7 @@ def test_brute_force_tiny_asymmetric_tables(oracle):
 
 
 def test_fp32_cpu_path_vs_fp64(oracle):
-    cx = synth.config0(seed=3, identity_norm=True, void_radius=7.0)
+    cx = synth.single_complex(seed=3, identity_norm=True, void_radius=7.0)
     d = cx.desc()
     r64, r32 = oracle.refmap_f64(d, cx), oracle.refmap_f32(d, cx)
     rng = np.random.default_rng(1)
@@ -431,7 +431,7 @@ def test_temperature_ladder_closed_form():
 def test_cpu_mc_path_deterministic_and_consistent(oracle):
     """The CPU path is a function of the seed; records hold the energy of the
     recorded pose; counters match the record counts."""
-    cx = synth.config0(seed=1)
+    cx = synth.single_complex(seed=1)
     d = cx.desc()
     runs = []
     for _ in range(2):
