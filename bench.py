@@ -152,6 +152,11 @@ def kernel_names(eng):
     return ", ".join(f"gd::gd_mc_kernel<{a}, 0>" for a in apls)
 
 
+def progress(msg):
+    """One line on stderr when a phase of the run ends."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def da_compare(args, cxs, desc):
     """Plain vs delayed acceptance on the same complexes, seed and start state.
 
@@ -176,18 +181,22 @@ def da_compare(args, cxs, desc):
     for cx in cxs:
         eng.add_complex(cx)
     eng.run(args.warmup, args.nc)
+    progress(f"plain warm-up: {args.warmup} steps of {n_rep} replicas")
     t0 = time.perf_counter()
     # state snapshots every log_every steps over log_steps plain steps
     every = max(args.nc, args.log_steps // max(1, args.log_snapshots))
     recs, rcid = sg.log_states(eng, max(1, args.log_steps // every), every, args.nc)
     t_log = time.perf_counter() - t0
+    progress(f"logged {len(recs)} replica states")
     t0 = time.perf_counter()
     eng.sur_build_grids(args.grid_spacing, args.grid_half)
     ts = sg.build_training_set(eng, recs, rcid, args.train_poses, seed=1)
     m, sigma = sg.load_training_set(eng, ts)
     t_data = time.perf_counter() - t0
+    progress(f"labelled {len(ts['t'])} training poses")
     fit = sg.fit(eng, ts, sigma, n_train=int(0.9 * len(ts["t"])), epochs=args.epochs,
                  batch_poses=args.batch_poses, lr=args.lr)
+    progress(f"trained: held-out RMSE {fit['test_rmse_energy']:.4f}")
     params = eng.sur_get_params()
     blob = eng.checkpoint()
     eng.close()
@@ -227,6 +236,7 @@ def da_compare(args, cxs, desc):
              "accepted_per_step": st["accepted"] / st["replica_steps"],
              "kernel_ms_avg": st["kernel_ms_avg"], "seconds": dt}
         r.update(best_stats(tag))
+        progress(f"{tag}: {steps} steps, {r['replica_steps_per_s']:.4g} replica-steps/s")
         return r
 
     out = {"plain": one("plain", args.steps, "plain"), "da": one("da", args.steps, "da")}
@@ -278,7 +288,7 @@ def da_compare(args, cxs, desc):
 
 
 # HBM bytes of one 2,000-step plain MC launch on the ensemble workload without
-# records: rocprofv3 FETCH_SIZE 3681.1 KiB x 2 + WRITE_SIZE 786.8 KiB x 1 (the
+# records: rocprofv3 FETCH_SIZE 3690.1 KiB x 2 + WRITE_SIZE 786.8 KiB x 1 (the
 # factors a plain 256 MiB device copy gives), from
 # profiles/gd_mc_kernel_apl3_ensemble_3520rep_traffic.md
 TRAFFIC_ENSEMBLE_2000 = (3690.125 * 2 + 786.8125) * 1024
@@ -332,7 +342,7 @@ def post_bench(args, cxs, desc):
             eng.run(args.warmup, args.nc)
         eng.run(args.steps, args.nc, sink=lambda c, r: recs.append(r))
         allr = np.concatenate(recs)
-        eng.post_mc(allr[: min(len(allr), 200000)], 50, 500, seed=1)  # warm-up
+        eng.post_mc(allr, 50, 500, seed=1)  # warm-up
         tm = {}
         t0 = time.perf_counter()
         idx, sz = eng.post_mc(allr, 50, 500, seed=0, timing=tm)

@@ -28,9 +28,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <memory>
 #include <new>
+#include <thread>
 #include <vector>
 
 namespace gd {
@@ -237,13 +240,39 @@ inline void km_init_indices(int n, int k, int group, int level, uint64_t seed, s
 
 namespace {
 
-// Device k-means of n_groups groups of objects (features row-major n x 9 on
-// the host). medoid/size: [n_groups][k], group-local member index or -1.
-gd_status km_run(gd_ctx *ctx, int n_groups, const int64_t *off, const float *features, int k,
+// Host loops over millions of records run on up to 16 threads.
+template <class F>
+void par_for(int64_t n, F f) {
+  const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::min<int64_t>(std::min<int64_t>(hw, 16), std::max<int64_t>(1, n / 65536));
+  if (nt <= 1) {
+    f(0, (int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&, t] { f(t, n * t / nt, n * (t + 1) / nt); });
+  for (auto &x : th) x.join();
+}
+// Host arrays that every element of is written before use: no zero fill.
+template <class T>
+std::unique_ptr<T[]> uninit(size_t n) {
+  return std::unique_ptr<T[]>(new T[std::max<size_t>(n, 1)]);
+}
+
+inline int par_threads(int64_t n) {
+  const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+  return (int)std::min<int64_t>(std::min<int64_t>(hw, 16), std::max<int64_t>(1, n / 65536));
+}
+
+// Device k-means of n_groups groups of objects whose features are given as
+// SoA [9][n_total] on the host. medoid/size: [n_groups][k], group-local
+// member index or -1.
+gd_status km_run(gd_ctx *ctx, int n_groups, const int64_t *off, const float *soa_in, int k,
                  float threshold, int max_pass, uint64_t seed, int level, int *medoid, int *size,
                  int *passes, double *device_ms) {
   using namespace gd;
-  if (n_groups < 0 || !off || (!features && off[n_groups] > 0) || !medoid || !size)
+  if (n_groups < 0 || !off || (!soa_in && off[n_groups] > 0) || !medoid || !size)
     return fail(ctx, GD_ERR_INVALID, "kmeans: bad arguments");
   if (k < 1 || k > KM_MAX_K) return fail(ctx, GD_ERR_CAPACITY, "kmeans: k %d outside [1, %d]", k, KM_MAX_K);
   if (max_pass < 1) return fail(ctx, GD_ERR_INVALID, "kmeans: max_pass %d < 1", max_pass);
@@ -280,7 +309,7 @@ gd_status km_run(gd_ctx *ctx, int n_groups, const int64_t *off, const float *fea
     km_init_indices(n, k, g, level, seed, perm, init.data());
     for (int c = 0; c < k; ++c)
       for (int d = 0; d < KM_DIM; ++d)
-        cent[((size_t)g * k + c) * KM_DIM + d] = features[(off[g] + init[c]) * KM_DIM + d];
+        cent[((size_t)g * k + c) * KM_DIM + d] = soa_in[(size_t)d * n_total + off[g] + init[c]];
     for (int s = 0; s < n; s += KM_TILE) {
       tile_g.push_back(g);
       tile_s.push_back(s);
@@ -289,10 +318,6 @@ gd_status km_run(gd_ctx *ctx, int n_groups, const int64_t *off, const float *fea
   const int n_tiles = (int)tile_g.size();
   if (n_tiles == 0) return GD_OK;
   HIPCHK(hipSetDevice(ctx->device));
-  // features as SoA for coalesced per-term loads
-  std::vector<float> soa((size_t)KM_DIM * n_total);
-  for (int64_t i = 0; i < n_total; ++i)
-    for (int d = 0; d < KM_DIM; ++d) soa[(size_t)d * n_total + i] = features[i * KM_DIM + d];
   const size_t gk = (size_t)n_groups * k;
   std::vector<void *> bufs;
   auto alloc = [&](void **p, size_t bytes) -> hipError_t {
@@ -310,7 +335,7 @@ gd_status km_run(gd_ctx *ctx, int n_groups, const int64_t *off, const float *fea
   float *d_feat;
   int64_t *d_off;
   int *d_tg, *d_ts;
-  HIPCHK(alloc((void **)&d_feat, soa.size() * 4));
+  HIPCHK(alloc((void **)&d_feat, (size_t)KM_DIM * n_total * 4));
   HIPCHK(alloc((void **)&d_off, (size_t)(n_groups + 1) * 8));
   HIPCHK(alloc((void **)&d_tg, (size_t)n_tiles * 4));
   HIPCHK(alloc((void **)&d_ts, (size_t)n_tiles * 4));
@@ -325,7 +350,7 @@ gd_status km_run(gd_ctx *ctx, int n_groups, const int64_t *off, const float *fea
   HIPCHK(alloc((void **)&A.best, gk * 8));
   HIPCHK(alloc((void **)&A.size, gk * 4));
   hipStream_t st = ctx->stream;
-  HIPCHK(hipMemcpyAsync(d_feat, soa.data(), soa.size() * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_feat, soa_in, (size_t)KM_DIM * n_total * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(d_off, off, (size_t)(n_groups + 1) * 8, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(d_tg, tile_g.data(), (size_t)n_tiles * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(d_ts, tile_s.data(), (size_t)n_tiles * 4, hipMemcpyHostToDevice, st));
@@ -406,8 +431,20 @@ gd_status gd_kmeans_medoids(gd_ctx *ctx, int n_groups, const int64_t *group_off,
   if (!ctx) return GD_ERR_INVALID;
   if (device_ms) *device_ms = 0.0;
   try {
-    return km_run(ctx, n_groups, group_off, features, k, threshold, max_pass, seed, 0, medoid,
-                  size, passes, device_ms);
+    using namespace gd;
+    if (n_groups < 0 || !group_off || (n_groups > 0 && group_off[n_groups] > INT32_MAX))
+      return fail(ctx, GD_ERR_INVALID, "kmeans: bad group offsets");
+    const int64_t n_total = n_groups > 0 ? std::max<int64_t>(group_off[n_groups], 0) : 0;
+    if (n_total > 0 && !features) return fail(ctx, GD_ERR_INVALID, "kmeans: null features");
+    // row-major objects to SoA terms
+    auto soa_v = uninit<float>((size_t)KM_DIM * n_total);
+    float *soa = soa_v.get();
+    par_for(n_total, [&](int, int64_t a, int64_t b) {
+      for (int d = 0; d < KM_DIM; ++d)
+        for (int64_t i = a; i < b; ++i) soa[(size_t)d * n_total + i] = features[i * KM_DIM + d];
+    });
+    return km_run(ctx, n_groups, group_off, soa, k, threshold, max_pass, seed, 0, medoid, size,
+                  passes, device_ms);
   } catch (const std::bad_alloc &) {
     return fail(ctx, GD_ERR_NOMEM, "kmeans: out of host memory");
   }
@@ -450,33 +487,61 @@ gd_status post_mc_impl(gd_ctx *ctx, const gd_record *recs, int64_t n, int k_repl
                 KM_MAX_K);
   *n_medoids = 0;
   if (n == 0) return GD_OK;
-  // records grouped by replica, ascending replica, each in the given order
-  int max_rep = -1;
-  for (int64_t i = 0; i < n; ++i) {
-    if (recs[i].replica.idx_rep < 0)
-      return fail(ctx, GD_ERR_INVALID, "post_mc: record %lld has replica %d", (long long)i,
-                  recs[i].replica.idx_rep);
-    max_rep = std::max(max_rep, recs[i].replica.idx_rep);
+  // records grouped by replica (ascending replica, each in the given order):
+  // a stable counting sort, chunk-parallel
+  const int nt = par_threads(n);
+  std::vector<int> tmax(nt, -1), tbad(nt, -1);
+  par_for(n, [&](int t, int64_t a, int64_t b) {
+    int mx = -1;
+    for (int64_t i = a; i < b; ++i) {
+      const int r = recs[i].replica.idx_rep;
+      if (r < 0 && tbad[t] < 0) tbad[t] = (int)std::min<int64_t>(i, INT32_MAX);
+      mx = std::max(mx, r);
+    }
+    tmax[t] = mx;
+  });
+  for (int t = 0; t < nt; ++t)
+    if (tbad[t] >= 0)
+      return fail(ctx, GD_ERR_INVALID, "post_mc: record %d has a negative replica", tbad[t]);
+  const int max_rep = *std::max_element(tmax.begin(), tmax.end());
+  const size_t nr = (size_t)max_rep + 1;
+  // per-thread counts, then each thread's first slot per replica
+  std::vector<int64_t> tcnt((size_t)nt * nr, 0);
+  par_for(n, [&](int t, int64_t a, int64_t b) {
+    int64_t *c = tcnt.data() + (size_t)t * nr;
+    for (int64_t i = a; i < b; ++i) c[recs[i].replica.idx_rep] += 1;
+  });
+  std::vector<int64_t> cnt(nr + 1, 0);
+  for (size_t r = 0; r < nr; ++r) {
+    int64_t s0 = cnt[r];
+    for (int t = 0; t < nt; ++t) {
+      const int64_t c = tcnt[(size_t)t * nr + r];
+      tcnt[(size_t)t * nr + r] = s0;
+      s0 += c;
+    }
+    cnt[r + 1] = s0;
   }
-  std::vector<int64_t> cnt((size_t)max_rep + 2, 0);
-  for (int64_t i = 0; i < n; ++i) cnt[(size_t)recs[i].replica.idx_rep + 1] += 1;
-  for (size_t r = 1; r < cnt.size(); ++r) cnt[r] += cnt[r - 1];
-  std::vector<int64_t> order((size_t)n), pos(cnt.begin(), cnt.end() - 1);
-  for (int64_t i = 0; i < n; ++i) order[(size_t)pos[recs[i].replica.idx_rep]++] = i;
+  auto order = uninit<int64_t>((size_t)n);
+  par_for(n, [&](int t, int64_t a, int64_t b) {
+    int64_t *pos = tcnt.data() + (size_t)t * nr;
+    for (int64_t i = a; i < b; ++i) order[(size_t)pos[recs[i].replica.idx_rep]++] = i;
+  });
   // groups: replicas that have records
   std::vector<int64_t> off{0};
-  std::vector<int64_t> grp_first;
-  for (int r = 0; r <= max_rep; ++r)
-    if (cnt[r + 1] > cnt[r]) {
-      grp_first.push_back(cnt[r]);
-      off.push_back(cnt[r + 1]);
+  for (size_t r = 0; r < nr; ++r)
+    if (cnt[r + 1] > cnt[r]) off.push_back(cnt[r + 1]);
+  const int n_groups = (int)off.size() - 1;
+  // features of the grouped records as SoA terms
+  auto feat_v = uninit<float>((size_t)KM_DIM * n);
+  float *feat = feat_v.get();
+  par_for(n, [&](int, int64_t a, int64_t b) {
+    for (int64_t i = a; i < b; ++i) {
+      const float *e = recs[order[i]].energy.e;
+      for (int d = 0; d < KM_DIM; ++d) feat[(size_t)d * n + i] = e[d];
     }
-  const int n_groups = (int)grp_first.size();
-  std::vector<float> feat((size_t)n * KM_DIM);
-  for (int64_t i = 0; i < n; ++i)
-    for (int d = 0; d < KM_DIM; ++d) feat[(size_t)i * KM_DIM + d] = recs[order[i]].energy.e[d];
+  });
   std::vector<int> med((size_t)n_groups * k_replica), sz((size_t)n_groups * k_replica);
-  TRY(km_run(ctx, n_groups, off.data(), feat.data(), k_replica, 0.001f, 501, seed, 1, med.data(),
+  TRY(km_run(ctx, n_groups, off.data(), feat, k_replica, 0.001f, 501, seed, 1, med.data(),
              sz.data(), nullptr, device_ms));
   // level 2 over the level-1 medoids, in replica and cluster order
   std::vector<int64_t> l1;
@@ -485,9 +550,10 @@ gd_status post_mc_impl(gd_ctx *ctx, const gd_record *recs, int64_t n, int k_repl
       const int m = med[(size_t)g * k_replica + c];
       if (m >= 0) l1.push_back(order[off[g] + m]);
     }
-  std::vector<float> f2(l1.size() * KM_DIM);
-  for (size_t i = 0; i < l1.size(); ++i)
-    for (int d = 0; d < KM_DIM; ++d) f2[i * KM_DIM + d] = recs[l1[i]].energy.e[d];
+  const size_t n1 = l1.size();
+  std::vector<float> f2((size_t)KM_DIM * std::max<size_t>(n1, 1));
+  for (size_t i = 0; i < n1; ++i)
+    for (int d = 0; d < KM_DIM; ++d) f2[(size_t)d * n1 + i] = recs[l1[i]].energy.e[d];
   const int64_t off2[2] = {0, (int64_t)l1.size()};
   std::vector<int> med2(k_final), sz2(k_final);
   TRY(km_run(ctx, 1, off2, f2.data(), k_final, 0.001f, 501, seed, 2, med2.data(), sz2.data(),
