@@ -207,6 +207,46 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, cons
   }
 }
 
+// Sum over MCS templates of tcc_m * sqrt(mean squared distance of the
+// template's atoms), src/kernel_cuda_l2_calcenergy.cu:274-313. Templates go
+// four at a time: their loads are issued together and their lane sums share
+// each butterfly step, so the per-template latencies overlap. Per template
+// the additions are the same as one template at a time.
+template <int APL>
+__device__ __forceinline__ float mcs_sum(const DevComplex &C, int lpr, const float (&x)[APL],
+                                         const float (&y)[APL], const float (&z)[APL],
+                                         const int (&li)[APL], const bool (&val)[APL]) {
+  float elhm = 0.0f;
+  const int pos = C.pos;
+  for (int m0 = 0; m0 < pos; m0 += 4) {
+    float s[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      s[q] = 0.0f;
+      const int m = m0 + q;
+      if (m >= pos) continue;
+#pragma unroll
+      for (int k = 0; k < APL; ++k) {
+        if (!val[k]) continue;
+        const float4 M = C.mcs[(size_t)m * C.lna + li[k]];
+        if (M.w != 0.0f) s[q] += pair_d2(x[k], y[k], z[k], M.x, M.y, M.z);
+      }
+    }
+    for (int b = 1; b < lpr; b <<= 1) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) s[q] += __shfl_xor(s[q], b);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int m = m0 + q;
+      if (m >= pos) break;
+      const float n = C.mcs_cnt[m];
+      if (n > 0.0f) elhm += C.mcs_tcc[m] * sqrtf(s[q] / n);
+    }
+  }
+  return elhm;
+}
+
 // Energy, RMSD and MCC of one pose held by `lpr` lanes (CalcEnergy_d +
 // CombineEnergy_d + CalcRmsd_d + CalcMcc_d). Every lane of the replica ends
 // with the same bit pattern in E.
@@ -380,20 +420,8 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
   }
 
   // MCS restraints, src/kernel_cuda_l2_calcenergy.cu:274-313
-  float elhm = 0.0f;
   const int pos = C.pos;
-  for (int m = 0; m < pos; ++m) {
-    float s = 0.0f;
-#pragma unroll
-    for (int k = 0; k < APL; ++k) {
-      if (!val[k]) continue;
-      const float4 M = C.mcs[(size_t)m * C.lna + li[k]];
-      if (M.w != 0.0f) s += pair_d2(x[k], y[k], z[k], M.x, M.y, M.z);
-    }
-    s = seg_sum(s, lpr);
-    const float n = C.mcs_cnt[m];
-    if (n > 0.0f) elhm += C.mcs_tcc[m] * sqrtf(s / n);
-  }
+  const float elhm = mcs_sum<APL>(C, lpr, x, y, z, li, val);
 
   t_hpc = seg_sum(t_hpc, lpr);
   t_kde = seg_sum(t_kde, lpr);
@@ -455,20 +483,8 @@ __device__ __forceinline__ float cheap_terms(const DevComplex &C, const Lds &S, 
     }
     if (kr.y > kr.x) t_kde += ks / (float)(kr.y - kr.x);
   }
-  float elhm = 0.0f;
   const int pos = C.pos;
-  for (int m = 0; m < pos; ++m) {
-    float s = 0.0f;
-#pragma unroll
-    for (int k = 0; k < APL; ++k) {
-      if (!val[k]) continue;
-      const float4 M = C.mcs[(size_t)m * C.lna + li[k]];
-      if (M.w != 0.0f) s += pair_d2(x[k], y[k], z[k], M.x, M.y, M.z);
-    }
-    s = seg_sum(s, lpr);
-    const float n = C.mcs_cnt[m];
-    if (n > 0.0f) elhm += C.mcs_tcc[m] * sqrtf(s / n);
-  }
+  const float elhm = mcs_sum<APL>(C, lpr, x, y, z, li, val);
   t_kde = seg_sum(t_kde, lpr);
   const float lna = (float)C.lna;
   const float e6 = (t_kde / C.kde3) / lna;
