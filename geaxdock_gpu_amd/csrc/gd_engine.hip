@@ -402,6 +402,48 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
       mcnt[m] += ok ? 1.0f : 0.0f;
     }
   }
+  // MCS moments per (ligand conformer, template) for the closed-form
+  // restraint of the surrogate path (double sums, stored as float)
+  std::vector<float4> mcsq((size_t)std::max(d->n_lig * pos, 1) * 4, make_float4(0, 0, 0, 0));
+  for (int c = 0; c < d->n_lig; ++c)
+    for (int m = 0; m < pos; ++m) {
+      double so[3] = {0, 0, 0}, sm[3] = {0, 0, 0};
+      int n = 0;
+      for (int l = 0; l < lna; ++l) {
+        const float4 f = mcs[(size_t)m * lna + l];
+        if (f.w == 0.0f) continue;
+        const size_t o = (size_t)c * 3 * lna + l;
+        so[0] += orig[o];
+        so[1] += orig[o + lna];
+        so[2] += orig[o + 2 * lna];
+        sm[0] += f.x;
+        sm[1] += f.y;
+        sm[2] += f.z;
+        n += 1;
+      }
+      if (n == 0) continue;
+      double ob[3], mb[3], A = 0.0, M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int i = 0; i < 3; ++i) {
+        ob[i] = so[i] / n;
+        mb[i] = sm[i] / n;
+      }
+      for (int l = 0; l < lna; ++l) {
+        const float4 f = mcs[(size_t)m * lna + l];
+        if (f.w == 0.0f) continue;
+        const size_t o = (size_t)c * 3 * lna + l;
+        const double op[3] = {orig[o] - ob[0], orig[o + lna] - ob[1], orig[o + 2 * lna] - ob[2]};
+        const double mp[3] = {f.x - mb[0], f.y - mb[1], f.z - mb[2]};
+        for (int i = 0; i < 3; ++i) {
+          A += op[i] * op[i] + mp[i] * mp[i];
+          for (int j = 0; j < 3; ++j) M[i][j] += mp[i] * op[j];
+        }
+      }
+      float4 *q = &mcsq[((size_t)c * pos + m) * 4];
+      q[0] = make_float4((float)A, (float)ob[0], (float)ob[1], (float)ob[2]);
+      q[1] = make_float4((float)mb[0], (float)mb[1], (float)mb[2], (float)M[0][0]);
+      q[2] = make_float4((float)M[0][1], (float)M[0][2], (float)M[1][0], (float)M[1][1]);
+      q[3] = make_float4((float)M[1][2], (float)M[2][0], (float)M[2][1], (float)M[2][2]);
+    }
   // KDE sorted by type with per-type ranges
   std::vector<int> kperm(pnk);
   for (int k = 0; k < pnk; ++k) kperm[k] = k;
@@ -481,6 +523,7 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
   TRY(dev_upload(ctx, h, (float4 **)&D.mcs, mcs.data(), mcs.size()));
   TRY(dev_upload(ctx, h, (float **)&D.mcs_tcc, tcc.data(), tcc.size()));
   TRY(dev_upload(ctx, h, (float **)&D.mcs_cnt, mcnt.data(), mcnt.size()));
+  TRY(dev_upload(ctx, h, (float4 **)&D.mcsq, mcsq.data(), mcsq.size()));
   TRY(dev_upload(ctx, h, (float4 **)&D.kde, kde.data(), kde.size()));
   TRY(dev_upload(ctx, h, (int2 **)&D.kde_range, krange.data(), krange.size()));
   TRY(dev_upload(ctx, h, (float **)&D.psp, psp_src, psp_n));

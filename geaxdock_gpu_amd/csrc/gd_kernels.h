@@ -46,6 +46,11 @@ struct DevComplex {
   const float4 *mcs;       // [pos * lna] x, y, z, valid
   const float *mcs_tcc;    // [pos]
   const float *mcs_cnt;    // [pos] valid atoms per template
+  // per (ligand conformer, template), over the template's valid atoms:
+  // {A, obar}, {mbar, M00}, {M01, M02, M10, M11}, {M12, M20, M21, M22} with
+  // A = sum |o - obar|^2 + sum |m - mbar|^2 and M = sum (m - mbar)(o - obar)^T,
+  // so that sum |R o + t - m|^2 = A + n |R obar + t - mbar|^2 - 2 <R, M>
+  const float4 *mcsq;      // [n_lig * pos * 4]
   const float4 *kde;       // [pnk] sorted by type
   const int2 *kde_range;   // [NLT] begin, end
   const float *psp;        // [NLT * n_res]

@@ -468,8 +468,8 @@ template <int APL>
 __device__ __forceinline__ float cheap_terms(const DevComplex &C, const Lds &S, int lpr,
                                              const float (&x)[APL], const float (&y)[APL],
                                              const float (&z)[APL], const int (&lt)[APL],
-                                             const int (&li)[APL], const bool (&val)[APL],
-                                             const float mm[6]) {
+                                             const bool (&val)[APL], const float mm[6],
+                                             const Rot &R, int a, int ilig) {
   float t_kde = 0.0f;
   const float kde2l = C.kde2l;
 #pragma unroll
@@ -483,8 +483,28 @@ __device__ __forceinline__ float cheap_terms(const DevComplex &C, const Lds &S, 
     }
     if (kr.y > kr.x) t_kde += ks / (float)(kr.y - kr.x);
   }
+  // MCS restraints in closed form from the conformer's template moments:
+  // sum |R o + t - m|^2 = A + n |R obar + t - mbar|^2 - 2 <R, M>; the
+  // replica's lanes take the templates in turn
   const int pos = C.pos;
-  const float elhm = mcs_sum<APL>(C, lpr, x, y, z, li, val);
+  float elhm = 0.0f;
+  {
+    const float4 *qm = C.mcsq + (size_t)ilig * pos * 4;
+    const float tx = mm[0] + C.pocket[0], ty = mm[1] + C.pocket[1], tz = mm[2] + C.pocket[2];
+    for (int m = a; m < pos; m += lpr) {
+      const float n = C.mcs_cnt[m];
+      if (!(n > 0.0f)) continue;
+      const float4 q0 = qm[4 * m], q1 = qm[4 * m + 1], q2 = qm[4 * m + 2], q3 = qm[4 * m + 3];
+      const float ux = R.r00 * q0.y + R.r01 * q0.z + R.r02 * q0.w + tx - q1.x;
+      const float uy = R.r10 * q0.y + R.r11 * q0.z + R.r12 * q0.w + ty - q1.y;
+      const float uz = R.r20 * q0.y + R.r21 * q0.z + R.r22 * q0.w + tz - q1.z;
+      const float rm = R.r00 * q1.w + R.r01 * q2.x + R.r02 * q2.y + R.r10 * q2.z +
+                       R.r11 * q2.w + R.r12 * q3.x + R.r20 * q3.y + R.r21 * q3.z + R.r22 * q3.w;
+      const float sq = fmaxf(q0.x + n * (ux * ux + uy * uy + uz * uz) - 2.0f * rm, 0.0f);
+      elhm += C.mcs_tcc[m] * sqrtf(sq / n);
+    }
+  }
+  elhm = seg_sum(elhm, lpr);
   t_kde = seg_sum(t_kde, lpr);
   const float lna = (float)C.lna;
   const float e6 = (t_kde / C.kde3) / lna;
@@ -507,8 +527,8 @@ __device__ __forceinline__ float sur_total(const KParams &P, const DevComplex &C
                                            unsigned char *da, int lpr, int prt,
                                            const float (&x)[APL], const float (&y)[APL],
                                            const float (&z)[APL], const int (&lt)[APL],
-                                           const int (&li)[APL], const bool (&val)[APL],
-                                           const float mm[6]) {
+                                           const bool (&val)[APL], const float mm[6],
+                                           const Rot &R, int a, int ilig) {
   if constexpr (DAM == 2) {
     // tabulated network: trilinear interpolation of the atom type's output
     // grid (same grid geometry and clamping as the feature grid)
@@ -537,7 +557,7 @@ __device__ __forceinline__ float sur_total(const KParams &P, const DevComplex &C
     }
     ys = seg_sum(ys, lpr);
     return P.sur_scale * (ys / (float)C.lna) +
-           cheap_terms<APL>(C, S, lpr, x, y, z, lt, li, val, mm);
+           cheap_terms<APL>(C, S, lpr, x, y, z, lt, val, mm, R, a, ilig);
   }
   const int l = threadIdx.x & 63;
   unsigned short *Xs = reinterpret_cast<unsigned short *>(da);
@@ -592,7 +612,7 @@ __device__ __forceinline__ float sur_total(const KParams &P, const DevComplex &C
   ysum = seg_sum(ysum, lpr);
   // the network sums per-atom pair energies; the terms are per-atom means
   return P.sur_scale * (ysum / (float)C.lna) +
-         cheap_terms<APL>(C, S, lpr, x, y, z, lt, li, val, mm);
+         cheap_terms<APL>(C, S, lpr, x, y, z, lt, val, mm, R, a, ilig);
 }
 
 // ExchangeReplicas_d for one ladder (src/kernel_cuda_l1_exchangereplicas.cu:33-78):
@@ -684,8 +704,10 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   }
 
   float x[APL], y[APL], z[APL];
+  Rot Rp{};  // rotation of the last placed pose (the surrogate's MCS moments use it)
   auto place = [&](const float m[6]) {
     const Rot R = euler(m[3], m[4], m[5]);
+    Rp = R;
 #pragma unroll
     for (int k = 0; k < APL; ++k) {
       x[k] = R.r00 * ox[k] + R.r01 * oy[k] + R.r02 * oz[k] + m[0] + C.pocket[0];
@@ -699,7 +721,8 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
       extern __shared__ __attribute__((aligned(16))) unsigned char lds_se[];
       unsigned char *dse = lds_se + P.off_da + (threadIdx.x >> 6) * 6208;
       place(mo);
-      const float sv_s = sur_total<APL, DA>(P, C, S, dse, lpr, pk.prt, x, y, z, lt, li, val, mo);
+      const float sv_s = sur_total<APL, DA>(P, C, S, dse, lpr, pk.prt, x, y, z, lt, val, mo, Rp, a,
+                                                ilig);
       if (active && lead) P.ev_sur[sv] = sv_s;
       return;
     }
@@ -781,7 +804,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
     da = lds_da + P.off_da + (threadIdx.x >> 6) * 6208;
     if (wave_active) {
       place(mo);
-      s_old = sur_total<APL, DA>(P, C, S, da, lpr, pk.prt, x, y, z, lt, li, val, mo);
+      s_old = sur_total<APL, DA>(P, C, S, da, lpr, pk.prt, x, y, z, lt, val, mo, Rp, a, ilig);
     }
   }
 
@@ -819,7 +842,8 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
 #pragma unroll
         for (int i = 0; i < 6; ++i) mn[i] = yv[i] + dm[i];
         place(mn);
-        const float s_n = sur_total<APL, DA>(P, C, S, da, lpr, pk.prt, x, y, z, lt, li, val, mn);
+        const float s_n =
+            sur_total<APL, DA>(P, C, S, da, lpr, pk.prt, x, y, z, lt, val, mn, Rp, a, ilig);
         if (metropolis(s_n, s_y, mb, u[6])) {
 #pragma unroll
           for (int i = 0; i < 6; ++i) yv[i] = mn[i];

@@ -228,3 +228,31 @@ def test_da_equals_plain_when_surrogate_is_exact(gd, tab):
           f"({same_mm.sum()} same pose); plain acceptance {p['acs_mc'].mean() / 400:.3f}")
     assert same.sum() >= len(same) - 1, same
     assert np.array_equal(p["replica"][:, 2], q["replica"][:, 2]) or same.sum() < len(same)
+
+
+@pytest.mark.parametrize("tab", [False, True])
+def test_surrogate_cheap_terms_equal_exact_terms(gd, tab):
+    """With scale 0 the surrogate is its cheap part: the weighted KDE, MCS and
+    pocket-distance terms. Its MCS restraint comes in closed form from the
+    conformer's template moments (A + n|R obar + t - mbar|^2 - 2<R, M>); it
+    must equal the exact per-atom terms of gd_calc_energy to FP32 rounding,
+    over many conformers, both receptor models and poses far from and near
+    the templates."""
+    cx = synth.make_complex(seed=21, lna=40, pnp=600, pnk=120, pos=7, mcs_atoms=30, n_lig=5,
+                            n_prt=2, n_tmp=2)
+    rng = np.random.default_rng(4)
+    n = 300
+    mm = np.zeros((n, 6), np.float32)
+    mm[:, :3] = rng.normal(0, 2.0, (n, 3)) * (rng.random((n, 1)) < 0.7)
+    mm[:, 3:] = rng.uniform(-np.pi, np.pi, (n, 3)) * (rng.random((n, 1)) < 0.7)
+    il = rng.integers(0, cx.n_lig, n).astype(np.int32)
+    ip = rng.integers(0, cx.n_prt, n).astype(np.int32)
+    eng, cid = _engine(gd, cx, 3, tab=tab)
+    s = eng.sur_surrogate(cid, il, ip, mm, 0.0).astype(np.float64)
+    e = eng.calc_energy(cid, il, ip, mm)["e"].astype(np.float64)
+    eng.close()
+    w = np.array(cx.enepara_struct.w[:], np.float64)
+    want = (e[:, 6:9] * w[6:9]).sum(axis=1)
+    err = np.abs(s - want)
+    print(f"\ntab={tab}: worst |S - exact cheap terms| {err.max():.2e} (scale {np.abs(want).max():.2f})")
+    assert np.all(err <= 2e-5 * np.maximum(1.0, np.abs(want))), err.max()
