@@ -256,3 +256,54 @@ def test_surrogate_cheap_terms_equal_exact_terms(gd, tab):
     err = np.abs(s - want)
     print(f"\ntab={tab}: worst |S - exact cheap terms| {err.max():.2e} (scale {np.abs(want).max():.2f})")
     assert np.all(err <= 2e-5 * np.maximum(1.0, np.abs(want))), err.max()
+
+
+def test_trained_surrogate_gains_over_no_surrogate(gd):
+    """What the learned part buys. Delayed acceptance pays, at its exact
+    stage, for the part of a composite move's energy change the screen did
+    not see: exp(-beta [(E_y - E_x) - (S_y - S_x)]). Without the network
+    (scale 0) the screen sees only the cheap terms, so the residual is the
+    whole receptor-pair change; the trained network must explain a good part
+    of it on held-out composite moves (k = 4 proposals from logged states),
+    while an untrained network only adds noise. (Trained on too few poses
+    the network is worse than none: its error on small differences then
+    exceeds the differences.)"""
+    cx = synth.make_complex(seed=31, lna=32, pnp=1200, n_lig=4, n_prt=2, n_tmp=8)
+    k, nc = 4, 20
+    eng = gd.Engine(seed=7, record=False)
+    cid = eng.add_complex(cx)
+    eng.run(200, nc)
+    recs, rcid = sg.log_states(eng, 20, 20, nc)
+    eng.sur_build_grids(0.6, 12.0)
+    ts = sg.build_training_set(eng, recs, rcid, 200000, seed=1)
+    _, sigma = sg.load_training_set(eng, ts)
+    fit = sg.fit(eng, ts, sigma, n_train=int(0.9 * len(ts["t"])), epochs=20)
+    trained = eng.sur_get_params()
+    # held-out composite moves: fresh states of a continued run, k steps each
+    more, _ = sg.log_states(eng, 4, 20, nc)
+    rng = np.random.default_rng(9)
+    x = more["movematrix"].astype(np.float32)
+    scale = np.array(cx.move_scale, np.float32)
+    y = x + (rng.normal(size=(len(x), k, 6)) * scale).sum(axis=1).astype(np.float32)
+    il = more["replica"][:, 3].astype(np.int32)
+    ip = more["replica"][:, 1].astype(np.int32)
+    dE = (eng.calc_energy(cid, il, ip, y)["e"][:, 9] -
+          eng.calc_energy(cid, il, ip, x)["e"][:, 9]).astype(np.float64)
+
+    def residual_var(params, s):
+        eng.sur_set_params(params)
+        eng.sur_tabulate()
+        dS = (eng.sur_surrogate(cid, il, ip, y, s) -
+              eng.sur_surrogate(cid, il, ip, x, s)).astype(np.float64)
+        return float(np.var(dE - dS))
+
+    v_none = residual_var(trained, 0.0)
+    v_trained = residual_var(trained, sigma)
+    v_random = residual_var(sg.pack_params(sg.init_params(5)), sigma)
+    eng.close()
+    print(f"\nheld-out RMSE {fit['test_rmse_energy']:.3f} (std {fit['test_energy_std']:.3f}); "
+          f"{len(x)} composite moves: residual variance without network {v_none:.4g}, "
+          f"trained {v_trained:.4g} ({1 - v_trained / v_none:.0%} explained), "
+          f"untrained {v_random:.4g}")
+    assert v_trained < 0.6 * v_none
+    assert v_random > v_none
