@@ -11,7 +11,8 @@ training on the GPU, then from one checkpoint `steps` plain steps and `steps`
 delayed-acceptance steps (k surrogate steps per exact evaluation), both with
 all inputs resident in HBM and no records. value = the DA run's replica-steps
 per second of wall time; the plain figure, the speed-up and the best total
-energy each run reached are reported beside it. --mode plain times the plain
+energy each run reached are reported beside it, with a DA run of equal wall
+time and a DA run with the network removed (scale 0). --mode plain times the plain
 engine alone (records written to HBM as the reference's kernel does).
 
 roofline: algorithmic FP32 work of the plain MC kernel, 36 flops per
@@ -221,14 +222,14 @@ def da_compare(args, cxs, desc):
         return {"best_energy": float(b.min()), "best_p01": float(np.percentile(b, 1)),
                 "replica_best_mean": float(b.mean())}
 
-    def one(mode, steps, tag):
+    def one(mode, steps, tag, scale=None):
         run.restore(blob)
         run.reset_best()
         t0 = time.perf_counter()
         if mode == "plain":
             st = run.run(steps, args.nc)
         else:
-            st = run.run_da(steps, args.nc, k, sigma)
+            st = run.run_da(steps, args.nc, k, sigma if scale is None else scale)
         dt = time.perf_counter() - t0
         r = {"steps": steps, "replica_steps_per_s": n_rep * steps / dt,
              "kernel_replica_steps_per_s": n_rep * steps / st["t_compute"],
@@ -245,6 +246,9 @@ def da_compare(args, cxs, desc):
     step = math.lcm(k, args.nc) if args.nc > 0 else k
     eq = max(step, eq // step * step)
     out["da_equal_time"] = one("da", eq, "da_eq")
+    # the same delayed acceptance with the network removed (scale 0: the
+    # screen keeps only the exact cheap terms), for what the network adds
+    out["da_no_network"] = one("da", args.steps, "da_nonet", scale=0.0)
     kname = kernel_names(run)
     run.close()
     speed = out["da"]["replica_steps_per_s"] / out["plain"]["replica_steps_per_s"]
@@ -254,7 +258,8 @@ def da_compare(args, cxs, desc):
     best_cmp = {"complexes": len(cxs), "delta_mean": float(dcx.mean()),
                 "delta_max": float(dcx.max()), "delta_min": float(dcx.min()),
                 "within_0p1": float(np.mean(np.abs(dcx) <= 0.1)),
-                "delta_equal_time_mean": float((per_cx["da_eq"] - per_cx["plain"]).mean())}
+                "delta_equal_time_mean": float((per_cx["da_eq"] - per_cx["plain"]).mean()),
+                "delta_no_network_mean": float((per_cx["da_nonet"] - per_cx["plain"]).mean())}
     pairs_per_step = sum(c.n_rep * c.lna * c.pnp for c in cxs)
     roof = roofline(pairs_per_step, min(args.steps, 2000), out["plain"]["kernel_ms_avg"], kname,
                     desc)
@@ -275,6 +280,7 @@ def da_compare(args, cxs, desc):
         "roofline_da_exact_part": roof_da,
         "launch_options": opts, "autotune": tuned,
         "plain": out["plain"], "da": out["da"], "da_equal_time": out["da_equal_time"],
+        "da_no_network": out["da_no_network"],
         "da_speedup": speed, "best_energy_delta": d_best, "best_energy_per_complex": best_cmp,
         "target_met": bool(speed >= 3.0 and abs(d_best) <= 0.1),
         "surrogate": {"log_seconds": t_log, "label_seconds": t_data, "sigma": sigma,
