@@ -89,7 +89,8 @@ struct ChunkPar {
 // rounds to 1 there, so PMF is pmf1 bit for bit); contacts, the near
 // electrostatic form and PSP run only when some lane is within contact range.
 template <int APL, int KIND, int NP>
-__device__ __forceinline__ int pair_group(const Lds &S, int p, const PairConst &K,
+__device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)[NP],
+                                           const PairConst &K,
                                            const float (&x)[APL], const float (&y)[APL],
                                            const float (&z)[APL], const ChunkPar (&cp)[APL],
                                            const float *const (&psprow)[APL],
@@ -99,12 +100,10 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const PairConst &
                                            int (&cnt)[APL], float (&vdwf)[APL]) {
   const float pa = -0.5f * LOG2E;
   int nfar = 0;
-  float4 P[NP];
   float hp[NP];
   int ps[NP];
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
-    P[j] = S.xyze[p + j];
     hp[j] = (KIND >= KIND_HPC) ? S.hpp[p + j] : 0.0f;
     ps[j] = (KIND == KIND_PSP) ? S.psp[p + j] : 0;
   }
@@ -193,12 +192,28 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, cons
   float vdwf[APL];
 #pragma unroll
   for (int k = 0; k < APL; ++k) vdwf[k] = 0.0f;
-  for (; p + NP <= end; p += NP)
-    nfar += pair_group<APL, KIND, NP>(S, p, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc,
-                                      cnt, vdwf);
-  for (; p < end; ++p)
-    nfar += pair_group<APL, KIND, 1>(S, p, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc,
-                                     cnt, vdwf);
+  // the next group's points are read from LDS one group ahead, so the read
+  // latency overlaps the current group's arithmetic (indices clamped into
+  // the chunk: the last prefetch re-reads its final point)
+  if (p + NP <= end) {
+    float4 P[NP];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) P[j] = S.xyze[p + j];
+    for (; p + NP <= end; p += NP) {
+      float4 Pn[NP];
+#pragma unroll
+      for (int j = 0; j < NP; ++j) Pn[j] = S.xyze[min(p + NP + j, end - 1)];
+      nfar += pair_group<APL, KIND, NP>(S, p, P, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
+                                        hpc, cnt, vdwf);
+#pragma unroll
+      for (int j = 0; j < NP; ++j) P[j] = Pn[j];
+    }
+  }
+  for (; p < end; ++p) {
+    const float4 P1[1] = {S.xyze[p]};
+    nfar += pair_group<APL, KIND, 1>(S, p, P1, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
+                                     hpc, cnt, vdwf);
+  }
   const float nf = (float)nfar;
 #pragma unroll
   for (int k = 0; k < APL; ++k) {
