@@ -108,6 +108,20 @@ def workload(args):
     return cxs, desc
 
 
+def host_cpu():
+    """CPU model and logical core count of this host (for the CPU baseline)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count()}
+
+
 def cpu_baseline(cxs, args):
     """FP32 CPU path on a bounded sample: two ligand conformers x one protein
     conformer x all temperatures of the first complex."""
@@ -142,7 +156,7 @@ def cpu_baseline(cxs, args):
     w_sub = sub.lna * sub.pnp
     w_job = np.mean([c.lna * c.pnp for c in cxs])
     return {"value": rs / dt * (w_sub / w_job), "unit": "replica-steps/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": host_cpu(),
             "sample": f"{sub.n_rep} replicas x {steps} steps (lna {sub.lna}, pnp {sub.pnp}); "
                       f"{dt:.1f} s on {threads} threads; scaled by pair count to the workload"}
 
@@ -413,6 +427,7 @@ def post_cpu_baseline(allr, args):
             break
         n_r = min(len(reps), int(n_r * max(2.0, args.cpu_seconds / max(dt, 1e-3))))
     return {"value": len(sub) / dt, "unit": "records/s", "cores": 1, "kind": "port",
+            "host": host_cpu(),
             "sample": f"{len(sub)} records of {n_r} replicas, {dt:.1f} s"}
 
 
