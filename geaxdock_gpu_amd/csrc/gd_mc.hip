@@ -195,18 +195,26 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, cons
   // the next group's points are read from LDS one group ahead, so the read
   // latency overlaps the current group's arithmetic (indices clamped into
   // the chunk: the last prefetch re-reads its final point)
+  // two register sets in turn (no copies): while one group is evaluated the
+  // other is being read
   if (p + NP <= end) {
-    float4 P[NP];
+    float4 Pa[NP], Pb[NP];
 #pragma unroll
-    for (int j = 0; j < NP; ++j) P[j] = S.xyze[p + j];
-    for (; p + NP <= end; p += NP) {
-      float4 Pn[NP];
+    for (int j = 0; j < NP; ++j) Pa[j] = S.xyze[p + j];
+    for (; p + 2 * NP <= end; p += 2 * NP) {
 #pragma unroll
-      for (int j = 0; j < NP; ++j) Pn[j] = S.xyze[min(p + NP + j, end - 1)];
-      nfar += pair_group<APL, KIND, NP>(S, p, P, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
+      for (int j = 0; j < NP; ++j) Pb[j] = S.xyze[p + NP + j];
+      nfar += pair_group<APL, KIND, NP>(S, p, Pa, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
                                         hpc, cnt, vdwf);
 #pragma unroll
-      for (int j = 0; j < NP; ++j) P[j] = Pn[j];
+      for (int j = 0; j < NP; ++j) Pa[j] = S.xyze[min(p + 2 * NP + j, end - 1)];
+      nfar += pair_group<APL, KIND, NP>(S, p + NP, Pb, K, x, y, z, cp, psprow, vdw, ele, pmf, psp,
+                                        hdb, hpc, cnt, vdwf);
+    }
+    if (p + NP <= end) {
+      nfar += pair_group<APL, KIND, NP>(S, p, Pa, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
+                                        hpc, cnt, vdwf);
+      p += NP;
     }
   }
   for (; p < end; ++p) {
