@@ -158,7 +158,34 @@ def cpu_baseline(cxs, args):
     return {"value": rs / dt * (w_sub / w_job), "unit": "replica-steps/s", "cores": threads,
             "kind": "port", "host": host_cpu(),
             "sample": f"{sub.n_rep} replicas x {steps} steps (lna {sub.lna}, pnp {sub.pnp}); "
-                      f"{dt:.1f} s on {threads} threads; scaled by pair count to the workload"}
+                      f"{dt:.1f} s on {threads} threads; scaled by pair count to the workload",
+            "one_core_single_complex": one_core_rate(args)}
+
+
+def one_core_rate(args, seconds=3.0):
+    """The same CPU path on one thread over the single-complex workload (1 lig x
+    1 prt x 8 temperatures, lna 32, pnp 1200), for comparison with the
+    reference's printed single-core figure of 843.75 MC sweeps per second
+    (src/util.C:1357)."""
+    from oracle import gdo
+    from geaxdock_gpu_amd import synth
+    cx = synth.single_complex(seed=0)
+    d = cx.desc()
+    st = gdo.McState(cx, rec_cap=0)
+    gdo.mc_run(d, st, seed=0, init=True, step0=0, n_steps=0, nc=args.nc, threads=1)
+    chunk = max(args.nc, 20)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        gdo.mc_run(d, st, seed=0, init=False, step0=steps, n_steps=chunk, nc=args.nc, threads=1)
+        steps += chunk
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    rate = cx.n_rep * steps / dt
+    return {"value": rate, "unit": "replica-steps/s", "cores": 1,
+            "sample": f"{cx.n_rep} replicas x {steps} steps, {dt:.1f} s",
+            "reference_single_core": 843.75, "ratio_to_reference_single_core": rate / 843.75}
 
 
 def kernel_names(eng):
