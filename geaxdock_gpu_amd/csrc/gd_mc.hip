@@ -51,12 +51,13 @@ __device__ __forceinline__ int seg_sum_i(int v, int lpr) {
   return v;
 }
 
-// HPC switching polynomial in d^2 (src/kernel_cuda_l2_calcenergy.cu:116-121):
-// 1 - (3.5 x - 4.5 x^2 + 2.5 x^3 - 0.5 x^4), x = d^2 / 81.
-constexpr float H1 = -3.5f / 81.0f;
-constexpr float H2 = 4.5f / (81.0f * 81.0f);
-constexpr float H3 = -2.5f / (81.0f * 81.0f * 81.0f);
-constexpr float H4 = 0.5f / (81.0f * 81.0f * 81.0f * 81.0f);
+// HPC switching polynomial (src/kernel_cuda_l2_calcenergy.cu:116-121):
+// 1 - (3.5 x - 4.5 x^2 + 2.5 x^3 - 0.5 x^4), x = d^2 / 81, which factors as
+// (1 - x)^3 (1 - x/2) = u^3 (1 + u) / 2 with u = 1 - x. Clamping u at 0 makes
+// it exactly 0 beyond d = 9 (the reference's d^2 <= 81 test), and the factored
+// form has no cancellation near the cut-off. The 1/2 is folded into the
+// receptor's hpp values when they are staged into LDS (an exact scaling).
+constexpr float INV81 = 1.0f / 81.0f;
 constexpr float LOG2E = 1.44269504088896340736f;
 
 struct PairConst {
@@ -81,7 +82,7 @@ struct ChunkPar {
 //   pmf  pmf1 / (1 + exp((6 - d/2)(d - pmf0))), exponent in base 2 as
 //        A d^2 + B d + C; vdW and PMF share one reciprocal 1/(den (1 + e));
 //   hdb  -hdb1/sqrt(2 pi) exp(-((d - hdb0) hdb1)^2 / 2) (zero record if hdb0 <= 0.1);
-//   hpc  hpp * switch(d^2) for d^2 <= 81 (KIND >= 1);
+//   hpc  hpp * switch(d^2) for d^2 <= 81 (KIND >= 1; S.hpp holds hpp / 2);
 //   psp  psp[lig_t][seq3r] when class 2 and d <= pmf0 (KIND == 2);
 //   contacts  d <= pmf0.
 // Two wave-uniform branches skip work no lane of the wave needs: when every
@@ -142,8 +143,9 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
         const float h = fmaf(d, cp[k].ha, cp[k].hb);
         hdb[k] = fmaf(cp[k].hc, fast_exp2(-(h * h)), hdb[k]);
         if (KIND >= KIND_HPC) {
-          const float poly = fmaf(q2, fmaf(q2, fmaf(q2, fmaf(q2, H4, H3), H2), H1), 1.0f);
-          hpc[k] += (q2 <= 81.0f) ? hp[j] * poly : 0.0f;
+          const float u = fmaxf(fmaf(q2, -INV81, 1.0f), 0.0f);
+          const float u3 = u * u * u;
+          hpc[k] = fmaf(hp[j], fmaf(u3, u, u3), hpc[k]);
         }
       }
   } else {
@@ -960,7 +962,7 @@ __device__ __forceinline__ void mc_kernel_body(const KParams &P, int idx, int st
     const float4 *src = C.prt_xyze + (size_t)pk.prt * pnp;
     for (int i = threadIdx.x; i < pnp; i += nt) {
       xyze[i] = src[i];
-      hpp[i] = C.prt_hpp[i];
+      hpp[i] = 0.5f * C.prt_hpp[i];
       psp[i] = C.prt_psp[i];
     }
     for (int i = threadIdx.x; i < PAR_FLOATS; i += nt) par[i] = C.params[i];
