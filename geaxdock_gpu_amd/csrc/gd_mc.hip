@@ -67,6 +67,7 @@ struct PairConst {
 // Per-lane parameters of one chunk (one receptor type) for one ligand atom.
 struct ChunkPar {
   float p1a, p2a, pmf0, pmf1, pb, pc, ha, hb, hc;
+  float a0, a1;  // p1a lj0, p1a lj0 lj1: den = a0 + a1 d^2 + d^9
   float nr2;  // contacts and the near electrostatic form lie within sqrt(nr2)
   float rf2;  // beyond sqrt(rf2) PMF equals pmf1 exactly and H-bond < 1e-18
 };
@@ -81,6 +82,7 @@ struct ChunkPar {
 //        ele_p/el1); where el1 d < 1 the near form replaces it;
 //   pmf  pmf1 / (1 + exp((6 - d/2)(d - pmf0))), exponent in base 2 as
 //        A d^2 + B d + C; vdW and PMF share one reciprocal 1/(den (1 + e));
+//        `pmf` here sums 1/(1 + e) over the chunk (pmf1 is applied per chunk);
 //   hdb  -hdb1/sqrt(2 pi) exp(-((d - hdb0) hdb1)^2 / 2) (zero record if hdb0 <= 0.1);
 //   hpc  hpp * switch(d^2) for d^2 <= 81 (KIND >= 1; S.hpp holds hpp / 2);
 //   psp  psp[lig_t][seq3r] when class 2 and d <= pmf0 (KIND == 2);
@@ -124,9 +126,8 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
       near |= q2 <= cp[k].nr2;
       const float d3 = q2 * d;
       const float d6 = d3 * d3;
-      const float d9 = d6 * d3;
       num[j][k] = fmaf(-cp[k].p2a, d3, cp[k].p1a);
-      den[j][k] = fmaf(cp[k].p1a, fmaf(K.lj01, q2, K.lj0), d9);
+      den[j][k] = fmaf(d6, d3, fmaf(cp[k].a1, q2, cp[k].a0));
       ele[k] = fmaf(P[j].w, r, ele[k]);
     }
   if (__ballot(mid) != 0) {
@@ -139,7 +140,7 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
         const float e1 = 1.0f + fast_exp2(wv);
         const float r = fast_rcp(den[j][k] * e1);
         vdw[k] = fmaf(num[j][k] * e1, r, vdw[k]);
-        pmf[k] = fmaf(cp[k].pmf1 * den[j][k], r, pmf[k]);
+        pmf[k] = fmaf(den[j][k], r, pmf[k]);
         const float h = fmaf(d, cp[k].ha, cp[k].hb);
         hdb[k] = fmaf(cp[k].hc, fast_exp2(-(h * h)), hdb[k]);
         if (KIND >= KIND_HPC) {
@@ -190,10 +191,11 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, cons
   constexpr int NP = APL >= 3 ? 1 : (APL == 2 ? 2 : 4);
   int p = beg, nfar = 0;
   // far-field vdW has its own accumulators so that no accumulator is
-  // updated on both sides of the far/mid branch (keeps them in place)
-  float vdwf[APL];
+  // updated on both sides of the far/mid branch (keeps them in place); the
+  // chunk's PMF sum is taken without pmf1, which multiplies it once at the end
+  float vdwf[APL], pmfc[APL];
 #pragma unroll
-  for (int k = 0; k < APL; ++k) vdwf[k] = 0.0f;
+  for (int k = 0; k < APL; ++k) vdwf[k] = pmfc[k] = 0.0f;
   // the next group's points are read from LDS one group ahead, so the read
   // latency overlaps the current group's arithmetic (indices clamped into
   // the chunk: the last prefetch re-reads its final point)
@@ -206,28 +208,28 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, cons
     for (; p + 2 * NP <= end; p += 2 * NP) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) Pb[j] = S.xyze[p + NP + j];
-      nfar += pair_group<APL, KIND, NP>(S, p, Pa, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
+      nfar += pair_group<APL, KIND, NP>(S, p, Pa, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
                                         hpc, cnt, vdwf);
 #pragma unroll
       for (int j = 0; j < NP; ++j) Pa[j] = S.xyze[min(p + 2 * NP + j, end - 1)];
-      nfar += pair_group<APL, KIND, NP>(S, p + NP, Pb, K, x, y, z, cp, psprow, vdw, ele, pmf, psp,
+      nfar += pair_group<APL, KIND, NP>(S, p + NP, Pb, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp,
                                         hdb, hpc, cnt, vdwf);
     }
     if (p + NP <= end) {
-      nfar += pair_group<APL, KIND, NP>(S, p, Pa, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
+      nfar += pair_group<APL, KIND, NP>(S, p, Pa, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
                                         hpc, cnt, vdwf);
       p += NP;
     }
   }
   for (; p < end; ++p) {
     const float4 P1[1] = {S.xyze[p]};
-    nfar += pair_group<APL, KIND, 1>(S, p, P1, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
+    nfar += pair_group<APL, KIND, 1>(S, p, P1, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
                                      hpc, cnt, vdwf);
   }
   const float nf = (float)nfar;
 #pragma unroll
   for (int k = 0; k < APL; ++k) {
-    pmf[k] = fmaf(cp[k].pmf1, nf, pmf[k]);
+    pmf[k] = fmaf(cp[k].pmf1, pmfc[k] + nf, pmf[k]);
     vdw[k] += vdwf[k];
   }
 }
@@ -329,6 +331,8 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
       cp[k].ha = b.x;
       cp[k].hb = b.y;
       cp[k].hc = b.z;
+      cp[k].a0 = a.x * K.lj0;
+      cp[k].a1 = a.x * K.lj01;
       // (6 - d/2)(d - pmf0) log2(e) = A d^2 + B d + C
       cp[k].pb = LOG2E * fmaf(0.5f, a.z, 6.0f);
       cp[k].pc = -6.0f * LOG2E * a.z;
