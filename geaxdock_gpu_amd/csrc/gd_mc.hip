@@ -12,6 +12,7 @@
 // CalcRmsd_d, CalcMcc_d, CalcEnergy_d (+CombineEnergy_d), Accept_d,
 // RecordLigand_d; ExchangeReplicas_d between segments
 // (src/kernel_cuda_lancher_siglegpu.cu:25-31).
+#include <type_traits>
 #include "gd_device.h"
 #include "gd_kernels.h"
 
@@ -91,7 +92,7 @@ struct ChunkPar {
 // lane is beyond its far radius the exponentials are not evaluated (1 + e
 // rounds to 1 there, so PMF is pmf1 bit for bit); contacts, the near
 // electrostatic form and PSP run only when some lane is within contact range.
-template <int APL, int KIND, int NP>
+template <int APL, int KIND, int NP, bool CLAMP>
 __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)[NP],
                                            const PairConst &K,
                                            const float (&x)[APL], const float (&y)[APL],
@@ -136,7 +137,10 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
 #pragma unroll
       for (int k = 0; k < APL; ++k) {
         const float q2 = d2[j][k], d = dd[j][k];
-        const float wv = fminf(fmaf(pa, q2, fmaf(cp[k].pb, d, cp[k].pc)), 64.0f);
+        // the exponent is clamped at 64 (exp2 must stay finite) only in chunks
+        // whose parameters let it reach 60; elsewhere the clamp is a no-op
+        const float wv0 = fmaf(pa, q2, fmaf(cp[k].pb, d, cp[k].pc));
+        const float wv = CLAMP ? fminf(wv0, 64.0f) : wv0;
         const float e1 = 1.0f + fast_exp2(wv);
         const float r = fast_rcp(den[j][k] * e1);
         vdw[k] = fmaf(num[j][k] * e1, r, vdw[k]);
@@ -178,7 +182,7 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
   return nfar;
 }
 
-template <int APL, int KIND>
+template <int APL, int KIND, bool CLAMP>
 __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, const PairConst &K,
                                             const float (&x)[APL], const float (&y)[APL],
                                             const float (&z)[APL], const ChunkPar (&cp)[APL],
@@ -208,22 +212,22 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, cons
     for (; p + 2 * NP <= end; p += 2 * NP) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) Pb[j] = S.xyze[p + NP + j];
-      nfar += pair_group<APL, KIND, NP>(S, p, Pa, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
+      nfar += pair_group<APL, KIND, NP, CLAMP>(S, p, Pa, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
                                         hpc, cnt, vdwf);
 #pragma unroll
       for (int j = 0; j < NP; ++j) Pa[j] = S.xyze[min(p + 2 * NP + j, end - 1)];
-      nfar += pair_group<APL, KIND, NP>(S, p + NP, Pb, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp,
+      nfar += pair_group<APL, KIND, NP, CLAMP>(S, p + NP, Pb, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp,
                                         hdb, hpc, cnt, vdwf);
     }
     if (p + NP <= end) {
-      nfar += pair_group<APL, KIND, NP>(S, p, Pa, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
+      nfar += pair_group<APL, KIND, NP, CLAMP>(S, p, Pa, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
                                         hpc, cnt, vdwf);
       p += NP;
     }
   }
   for (; p < end; ++p) {
     const float4 P1[1] = {S.xyze[p]};
-    nfar += pair_group<APL, KIND, 1>(S, p, P1, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
+    nfar += pair_group<APL, KIND, 1, CLAMP>(S, p, P1, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
                                      hpc, cnt, vdwf);
   }
   const float nf = (float)nfar;
@@ -348,15 +352,31 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
       rf *= 1.0001f;
       cp[k].rf2 = rf * rf;
     }
-    if (kind == KIND_PLAIN)
-      chunk_pairs<APL, KIND_PLAIN>(S, beg, end, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
-                                   hpc, cnt);
-    else if (kind == KIND_HPC)
-      chunk_pairs<APL, KIND_HPC>(S, beg, end, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
-                                 hpc, cnt);
-    else
-      chunk_pairs<APL, KIND_PSP>(S, beg, end, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb,
-                                 hpc, cnt);
+    // the PMF exponent (6 - d/2)(d - pmf0) log2(e) peaks at (pmf0 - 12)^2 / 8
+    // log2(e); only a chunk where some lane's peak passes 60 needs the clamp
+    bool wide = false;
+#pragma unroll
+    for (int k = 0; k < APL; ++k) {
+      const float t = cp[k].pmf0 - 12.0f;
+      wide |= !(0.125f * LOG2E * t * t <= 60.0f);
+    }
+    const bool clamp = __ballot(wide) != 0;
+    auto pairs = [&](auto kind_c, auto clamp_c) {
+      chunk_pairs<APL, decltype(kind_c)::value, decltype(clamp_c)::value>(
+          S, beg, end, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc, cnt);
+    };
+    using Yes = std::true_type;
+    using No = std::false_type;
+    if (kind == KIND_PLAIN) {
+      if (clamp) pairs(std::integral_constant<int, KIND_PLAIN>{}, Yes{});
+      else pairs(std::integral_constant<int, KIND_PLAIN>{}, No{});
+    } else if (kind == KIND_HPC) {
+      if (clamp) pairs(std::integral_constant<int, KIND_HPC>{}, Yes{});
+      else pairs(std::integral_constant<int, KIND_HPC>{}, No{});
+    } else {
+      if (clamp) pairs(std::integral_constant<int, KIND_PSP>{}, Yes{});
+      else pairs(std::integral_constant<int, KIND_PSP>{}, No{});
+    }
   }
 
   // pair sums that are linear in the receptor points
