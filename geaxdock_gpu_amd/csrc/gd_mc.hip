@@ -70,7 +70,7 @@ struct ChunkPar {
   float p1a, p2a, pmf0, pmf1, pb, pc, ha, hb, hc;
   float a0, a1;  // p1a lj0, p1a lj0 lj1: den = a0 + a1 d^2 + d^9
   float nr2;  // contacts and the near electrostatic form lie within sqrt(nr2)
-  float rf2;  // beyond sqrt(rf2) PMF equals pmf1 exactly and H-bond < 1e-18
+  float rf2;  // beyond sqrt(rf2) PMF is pmf1 within 2^-17 and H-bond < 2^-40 of its peak
 };
 
 // Receptor points [p, p + NP) of one chunk against the lane's APL atoms.
@@ -89,9 +89,11 @@ struct ChunkPar {
 //   psp  psp[lig_t][seq3r] when class 2 and d <= pmf0 (KIND == 2);
 //   contacts  d <= pmf0.
 // Two wave-uniform branches skip work no lane of the wave needs: when every
-// lane is beyond its far radius the exponentials are not evaluated (1 + e
-// rounds to 1 there, so PMF is pmf1 bit for bit); contacts, the near
-// electrostatic form and PSP run only when some lane is within contact range.
+// lane is beyond its far radius the exponentials are not evaluated (there
+// the PMF sigmoid differs from pmf1 by at most 2^-17 of pmf1 and the H-bond
+// term is below 2^-40 of its peak, so PMF counts as pmf1 and H-bond as 0);
+// contacts, the near electrostatic form and PSP run only when some lane is
+// within contact range.
 template <int APL, int KIND, int NP, bool CLAMP>
 __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)[NP],
                                            const PairConst &K,
@@ -344,11 +346,12 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
       // lie within this radius; the margin covers the rounding of d
       const float rn = fmaxf(a.z, C.inv_el1) * 1.0001f;
       cp[k].nr2 = rn * rn;
-      // far radius: (d/2 - 6)(d - pmf0) >= 25 ln 2 makes exp2 of the PMF
-      // exponent <= 2^-25, so 1 + e == 1; the H-bond exponent -h^2 <= -60
+      // far radius: (d/2 - 6)(d - pmf0) >= 17 ln 2 makes exp2 of the PMF
+      // exponent <= 2^-17, so pmf1 / (1 + e) is pmf1 within 2^-17 of pmf1;
+      // the H-bond exponent -h^2 <= -40
       const float u = 0.5f * a.z - 6.0f;
-      float rf = (6.0f + 0.5f * a.z) + sqrtf(u * u + 2.0f * 25.0f * 0.69314718f);
-      if (b.z != 0.0f) rf = fmaxf(rf, (7.7459667f - b.y) / b.x);
+      float rf = (6.0f + 0.5f * a.z) + sqrtf(u * u + 2.0f * 17.0f * 0.69314718f);
+      if (b.z != 0.0f) rf = fmaxf(rf, (6.3245553f - b.y) / b.x);
       rf *= 1.0001f;
       cp[k].rf2 = rf * rf;
     }
