@@ -92,6 +92,32 @@ def test_energy_terms_weighted_multiconformer(oracle, eng_mod):
     print("\nworst (rel, cond):", {k: (f"{v[0]:.2e}", f"{v[1]:.2e}") for k, v in rep.items()})
 
 
+def test_energy_terms_extreme_pmf_distances(oracle, eng_mod):
+    """PMF distances far outside the parameter file's range (pmf0 = 45 for
+    receptor type 0, -20 for types 1-7): the PMF exponent (6 - d/2)(d - pmf0)
+    reaches 120-136 nats, so exp2 would overflow without its clamp. Those
+    chunks run the pair loop with the clamp; the rest run it without (both
+    variants against the FP64 oracle). pmf0 = 45 also widens the contact
+    radius of type 0 to 45 A, which the contact map and MCC follow (one type
+    only, so no ligand atom passes the engine's 256 reference contacts)."""
+    cx = synth.single_complex(seed=3, identity_norm=True, void_radius=7.0)
+    ep = cx.enepara_struct
+    for lt in range(abi.MAXTP2):
+        ep.pmf0[lt][0] = 45.0
+        for pt in range(1, 8):
+            ep.pmf0[lt][pt] = -20.0
+    cx.enepara = np.frombuffer(bytes(ep), dtype=np.uint8).copy()
+    rng = np.random.default_rng(13)
+    mm = random_poses(rng, 32)
+    with eng_mod.Engine(seed=1) as eng:
+        cid = eng.add_complex(cx)
+        rep = {}
+        out = check_energy_parity(oracle, eng, cid, cx, mm, np.zeros(32, np.int32),
+                                  np.zeros(32, np.int32), rep)
+    assert np.isfinite(out["e"]).all()
+    print("\nworst (rel, cond):", {k: (f"{v[0]:.2e}", f"{v[1]:.2e}") for k, v in rep.items()})
+
+
 def test_start_pose_mcc_is_one(oracle, eng_mod):
     """Replica 0 in its start pose reproduces the reference contact map exactly:
     cms = 1 whenever the map has contacts (src/kernel_cuda_l2_calcmcc.cu)."""
