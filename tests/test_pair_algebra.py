@@ -1,0 +1,90 @@
+"""Closed-form checks of the rewrites the MC kernel's pair loop makes of the
+reference's pair terms (src/kernel_cuda_l2_calcenergy.cu:105-184). Each test
+states the reference's form and the kernel's form and checks that they agree
+(polynomial identities exactly, bounds on a dense grid in FP64)."""
+import math
+
+import numpy as np
+from numpy.polynomial import polynomial as P
+
+LN2 = math.log(2.0)
+LOG2E = 1.0 / LN2
+
+
+def test_hpc_switch_factors_exactly():
+    """1 - (3.5x - 4.5x^2 + 2.5x^3 - 0.5x^4) (calcenergy.cu:116-121) equals
+    (1 - x)^3 (1 - x/2), i.e. u^3 (1 + u) / 2 with u = 1 - x, and is zero at
+    the cut-off x = 1 (d = 9 A) with zero slope."""
+    ref = np.array([1.0, -3.5, 4.5, -2.5, 0.5])
+    fac = P.polymul(P.polypow([1.0, -1.0], 3), [1.0, -0.5])
+    np.testing.assert_array_equal(ref, fac)
+    x = np.linspace(0.0, 1.0, 1001)
+    u = 1.0 - x
+    np.testing.assert_allclose(P.polyval(x, ref), u ** 3 * (1.0 + u) / 2.0, rtol=0, atol=1e-15)
+    assert P.polyval(1.0, ref) == 0.0
+    assert P.polyval(1.0, P.polyder(ref)) == 0.0
+    # beyond the cut-off the reference adds nothing; max(u, 0) gives 0 there
+    assert max(1.0 - 1.5, 0.0) ** 3 * (1.0 + max(1.0 - 1.5, 0.0)) == 0.0
+
+
+def test_lj_denominator_rewrite():
+    """p1 lj0 (1 + lj1 d^2) + 1 with p1 = p1a / d^9 (calcenergy.cu:130-135),
+    multiplied through by d^9, is p1a lj0 + p1a lj0 lj1 d^2 + d^6 d^3: the
+    kernel's fma(d^6, d^3, a1 d^2 + a0) with a0 = p1a lj0, a1 = p1a lj0 lj1."""
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        p1a, p2a = rng.uniform(1e2, 1e7), rng.uniform(1e1, 1e5)
+        lj0, lj1 = rng.uniform(0.5, 1.0), rng.uniform(0.5, 1.0)
+        d = rng.uniform(0.5, 30.0)
+        p1, p2 = p1a / d ** 9, p2a / d ** 6
+        vdw_ref = (p1 - p2) / (p1 * lj0 * (1.0 + lj1 * d * d) + 1.0)
+        a0, a1 = p1a * lj0, p1a * lj0 * lj1
+        d3 = d * d * d
+        vdw_ker = (p1a - p2a * d3) / ((d3 * d3) * d3 + (a1 * d * d + a0))
+        assert abs(vdw_ker - vdw_ref) <= 1e-12 * max(1.0, abs(vdw_ref))
+
+
+def test_pmf_exponent_peak():
+    """(6 - d/2)(d - pmf0) (calcenergy.cu:157-160) peaks at d = 6 + pmf0/2
+    with value (pmf0 - 12)^2 / 8; its log2 form A d^2 + B d + C uses
+    A = -log2(e)/2, B = log2(e)(6 + pmf0/2), C = -6 log2(e) pmf0."""
+    d = np.linspace(0.0, 60.0, 600001)
+    for pmf0 in (-20.0, -6.0, 2.69, 4.0, 7.38, 30.0, 45.0):
+        w = (6.0 - d / 2.0) * (d - pmf0)
+        peak = (pmf0 - 12.0) ** 2 / 8.0
+        if 6.0 + pmf0 / 2.0 >= 0.0:
+            assert abs(w.max() - peak) <= 1e-6 * max(1.0, peak)
+        assert w.max() <= peak + 1e-9
+        A, B, C = -LOG2E / 2.0, LOG2E * (6.0 + pmf0 / 2.0), -6.0 * LOG2E * pmf0
+        np.testing.assert_allclose(A * d * d + B * d + C, w * LOG2E, rtol=1e-12, atol=1e-9)
+    # the parameter file's range keeps the exponent far below exp2's range:
+    # the kernel drops its clamp where the peak is at most 60 (in log2 units)
+    assert LOG2E * (2.69 - 12.0) ** 2 / 8.0 < 16.0
+    assert LOG2E * (7.38 - 12.0) ** 2 / 8.0 < 16.0
+
+
+def test_far_radius_bounds():
+    """Beyond the kernel's far radius the PMF sigmoid pmf1 / (1 + e^w) is pmf1
+    within 2^-17 of pmf1, and the H-bond Gaussian
+    exp(-((d - hdb0) hdb1)^2 / 2) (calcenergy.cu:179-184) is below 2^-40."""
+    for pmf0 in (-20.0, 2.69, 4.0, 7.38, 30.0):
+        u = 0.5 * pmf0 - 6.0
+        rf = (6.0 + 0.5 * pmf0) + math.sqrt(u * u + 2.0 * 17.0 * LN2)
+        d = np.linspace(rf, rf + 60.0, 20001)
+        w = (6.0 - d / 2.0) * (d - pmf0)
+        assert abs(w[0] + 17.0 * LN2) < 1e-9
+        rel = 1.0 - 1.0 / (1.0 + np.exp(w))
+        assert rel.max() <= 2.0 ** -17 * (1.0 + 1e-9)
+    rng = np.random.default_rng(1)
+    for _ in range(100):
+        hdb0, sig = rng.uniform(3.29, 4.55), rng.uniform(0.6, 1.48)
+        hdb1 = 1.0 / sig
+        # exp2(-(d HA + HB)^2) == exp(-((d - hdb0) hdb1)^2 / 2)
+        ha = hdb1 * math.sqrt(LOG2E / 2.0)
+        hb = -hdb0 * ha
+        rh = (math.sqrt(40.0) - hb) / ha
+        d = np.linspace(rh, rh + 30.0, 3001)
+        g_ref = np.exp(-((d - hdb0) * hdb1) ** 2 / 2.0)
+        g_ker = np.exp2(-(d * ha + hb) ** 2)
+        np.testing.assert_allclose(g_ker, g_ref, rtol=1e-9, atol=0)
+        assert g_ref.max() <= 2.0 ** -40 * (1.0 + 1e-9)
