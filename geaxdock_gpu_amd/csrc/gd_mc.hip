@@ -96,7 +96,7 @@ struct ChunkPar {
 // within contact range.
 template <int APL, int KIND, int NP, bool CLAMP>
 __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)[NP],
-                                           const PairConst &K,
+                                           const PairConst &K, float nr2max,
                                            const float (&x)[APL], const float (&y)[APL],
                                            const float (&z)[APL], const ChunkPar (&cp)[APL],
                                            const float *const (&psprow)[APL],
@@ -116,7 +116,11 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
   float d2[NP][APL], rd[NP][APL], dd[NP][APL], num[NP][APL], den[NP][APL];
   bool mid = false, near = false;
 #pragma unroll
-  for (int j = 0; j < NP; ++j)
+  for (int j = 0; j < NP; ++j) {
+    // contact range: the lane's nearest atom against the largest of its
+    // atoms' near radii (a superset of the pairs the branch needs; the terms
+    // inside test each pair exactly)
+    float qmin = 0.0f;
 #pragma unroll
     for (int k = 0; k < APL; ++k) {
       const float q2 = pair_d2(x[k], y[k], z[k], P[j].x, P[j].y, P[j].z);
@@ -126,13 +130,15 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
       rd[j][k] = r;
       dd[j][k] = d;
       mid |= q2 < cp[k].rf2;
-      near |= q2 <= cp[k].nr2;
+      qmin = k == 0 ? q2 : fminf(qmin, q2);
       const float d3 = q2 * d;
       const float d6 = d3 * d3;
       num[j][k] = fmaf(-cp[k].p2a, d3, cp[k].p1a);
       den[j][k] = fmaf(d6, d3, fmaf(cp[k].a1, q2, cp[k].a0));
       ele[k] = fmaf(P[j].w, r, ele[k]);
     }
+    near |= qmin <= nr2max;
+  }
   if (__ballot(mid) != 0) {
 #pragma unroll
     for (int j = 0; j < NP; ++j)
@@ -186,6 +192,7 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
 
 template <int APL, int KIND, bool CLAMP>
 __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, const PairConst &K,
+                                            float nr2max,
                                             const float (&x)[APL], const float (&y)[APL],
                                             const float (&z)[APL], const ChunkPar (&cp)[APL],
                                             const float *const (&psprow)[APL],
@@ -214,22 +221,22 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, cons
     for (; p + 2 * NP <= end; p += 2 * NP) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) Pb[j] = S.xyze[p + NP + j];
-      nfar += pair_group<APL, KIND, NP, CLAMP>(S, p, Pa, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
+      nfar += pair_group<APL, KIND, NP, CLAMP>(S, p, Pa, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
                                         hpc, cnt, vdwf);
 #pragma unroll
       for (int j = 0; j < NP; ++j) Pa[j] = S.xyze[min(p + 2 * NP + j, end - 1)];
-      nfar += pair_group<APL, KIND, NP, CLAMP>(S, p + NP, Pb, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp,
+      nfar += pair_group<APL, KIND, NP, CLAMP>(S, p + NP, Pb, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp,
                                         hdb, hpc, cnt, vdwf);
     }
     if (p + NP <= end) {
-      nfar += pair_group<APL, KIND, NP, CLAMP>(S, p, Pa, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
+      nfar += pair_group<APL, KIND, NP, CLAMP>(S, p, Pa, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
                                         hpc, cnt, vdwf);
       p += NP;
     }
   }
   for (; p < end; ++p) {
     const float4 P1[1] = {S.xyze[p]};
-    nfar += pair_group<APL, KIND, 1, CLAMP>(S, p, P1, K, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
+    nfar += pair_group<APL, KIND, 1, CLAMP>(S, p, P1, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
                                      hpc, cnt, vdwf);
   }
   const float nf = (float)nfar;
@@ -358,15 +365,17 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
     // the PMF exponent (6 - d/2)(d - pmf0) log2(e) peaks at (pmf0 - 12)^2 / 8
     // log2(e); only a chunk where some lane's peak passes 60 needs the clamp
     bool wide = false;
+    float nr2max = cp[0].nr2;
 #pragma unroll
     for (int k = 0; k < APL; ++k) {
       const float t = cp[k].pmf0 - 12.0f;
       wide |= !(0.125f * LOG2E * t * t <= 60.0f);
+      nr2max = fmaxf(nr2max, cp[k].nr2);
     }
     const bool clamp = __ballot(wide) != 0;
     auto pairs = [&](auto kind_c, auto clamp_c) {
       chunk_pairs<APL, decltype(kind_c)::value, decltype(clamp_c)::value>(
-          S, beg, end, K, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc, cnt);
+          S, beg, end, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc, cnt);
     };
     using Yes = std::true_type;
     using No = std::false_type;
