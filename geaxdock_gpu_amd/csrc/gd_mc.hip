@@ -94,7 +94,7 @@ struct ChunkPar {
 // term is below 2^-40 of its peak, so PMF counts as pmf1 and H-bond as 0);
 // contacts, the near electrostatic form and PSP run only when some lane is
 // within contact range.
-template <int APL, int KIND, int NP, bool CLAMP>
+template <int APL, int KIND, int NP, bool CLAMP, bool HB>
 __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)[NP],
                                            const PairConst &K, float nr2max,
                                            const float (&x)[APL], const float (&y)[APL],
@@ -153,8 +153,12 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
         const float r = fast_rcp(den[j][k] * e1);
         vdw[k] = fmaf(num[j][k] * e1, r, vdw[k]);
         pmf[k] = fmaf(den[j][k], r, pmf[k]);
-        const float h = fmaf(d, cp[k].ha, cp[k].hb);
-        hdb[k] = fmaf(cp[k].hc, fast_exp2(-(h * h)), hdb[k]);
+        // H-bond: only in chunks where some lane's atom has an H-bond record
+        // with this receptor type (elsewhere every term would be 0 * exp2)
+        if (HB) {
+          const float h = fmaf(d, cp[k].ha, cp[k].hb);
+          hdb[k] = fmaf(cp[k].hc, fast_exp2(-(h * h)), hdb[k]);
+        }
         if (KIND >= KIND_HPC) {
           const float u = fmaxf(fmaf(q2, -INV81, 1.0f), 0.0f);
           const float u3 = u * u * u;
@@ -190,7 +194,7 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
   return nfar;
 }
 
-template <int APL, int KIND, bool CLAMP>
+template <int APL, int KIND, bool CLAMP, bool HB>
 __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, const PairConst &K,
                                             float nr2max,
                                             const float (&x)[APL], const float (&y)[APL],
@@ -221,22 +225,22 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, cons
     for (; p + 2 * NP <= end; p += 2 * NP) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) Pb[j] = S.xyze[p + NP + j];
-      nfar += pair_group<APL, KIND, NP, CLAMP>(S, p, Pa, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
+      nfar += pair_group<APL, KIND, NP, CLAMP, HB>(S, p, Pa, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
                                         hpc, cnt, vdwf);
 #pragma unroll
       for (int j = 0; j < NP; ++j) Pa[j] = S.xyze[min(p + 2 * NP + j, end - 1)];
-      nfar += pair_group<APL, KIND, NP, CLAMP>(S, p + NP, Pb, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp,
+      nfar += pair_group<APL, KIND, NP, CLAMP, HB>(S, p + NP, Pb, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp,
                                         hdb, hpc, cnt, vdwf);
     }
     if (p + NP <= end) {
-      nfar += pair_group<APL, KIND, NP, CLAMP>(S, p, Pa, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
+      nfar += pair_group<APL, KIND, NP, CLAMP, HB>(S, p, Pa, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
                                         hpc, cnt, vdwf);
       p += NP;
     }
   }
   for (; p < end; ++p) {
     const float4 P1[1] = {S.xyze[p]};
-    nfar += pair_group<APL, KIND, 1, CLAMP>(S, p, P1, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
+    nfar += pair_group<APL, KIND, 1, CLAMP, HB>(S, p, P1, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
                                      hpc, cnt, vdwf);
   }
   const float nf = (float)nfar;
@@ -364,31 +368,33 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
     }
     // the PMF exponent (6 - d/2)(d - pmf0) log2(e) peaks at (pmf0 - 12)^2 / 8
     // log2(e); only a chunk where some lane's peak passes 60 needs the clamp
-    bool wide = false;
+    bool wide = false, hbond = false;
     float nr2max = cp[0].nr2;
 #pragma unroll
     for (int k = 0; k < APL; ++k) {
       const float t = cp[k].pmf0 - 12.0f;
       wide |= !(0.125f * LOG2E * t * t <= 60.0f);
+      hbond |= cp[k].hc != 0.0f;
       nr2max = fmaxf(nr2max, cp[k].nr2);
     }
     const bool clamp = __ballot(wide) != 0;
-    auto pairs = [&](auto kind_c, auto clamp_c) {
-      chunk_pairs<APL, decltype(kind_c)::value, decltype(clamp_c)::value>(
-          S, beg, end, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmf, psp, hdb, hpc, cnt);
+    const bool hb = __ballot(hbond) != 0;
+    auto pairs = [&](auto kind_c, auto clamp_c, auto hb_c) {
+      chunk_pairs<APL, decltype(kind_c)::value, decltype(clamp_c)::value,
+                  decltype(hb_c)::value>(S, beg, end, K, nr2max, x, y, z, cp, psprow, vdw, ele,
+                                         pmf, psp, hdb, hpc, cnt);
     };
-    using Yes = std::true_type;
-    using No = std::false_type;
-    if (kind == KIND_PLAIN) {
-      if (clamp) pairs(std::integral_constant<int, KIND_PLAIN>{}, Yes{});
-      else pairs(std::integral_constant<int, KIND_PLAIN>{}, No{});
-    } else if (kind == KIND_HPC) {
-      if (clamp) pairs(std::integral_constant<int, KIND_HPC>{}, Yes{});
-      else pairs(std::integral_constant<int, KIND_HPC>{}, No{});
-    } else {
-      if (clamp) pairs(std::integral_constant<int, KIND_PSP>{}, Yes{});
-      else pairs(std::integral_constant<int, KIND_PSP>{}, No{});
-    }
+    auto by_hb = [&](auto kind_c, auto clamp_c) {
+      if (hb) pairs(kind_c, clamp_c, std::true_type{});
+      else pairs(kind_c, clamp_c, std::false_type{});
+    };
+    auto by_clamp = [&](auto kind_c) {
+      if (clamp) by_hb(kind_c, std::true_type{});
+      else by_hb(kind_c, std::false_type{});
+    };
+    if (kind == KIND_PLAIN) by_clamp(std::integral_constant<int, KIND_PLAIN>{});
+    else if (kind == KIND_HPC) by_clamp(std::integral_constant<int, KIND_HPC>{});
+    else by_clamp(std::integral_constant<int, KIND_PSP>{});
   }
 
   // pair sums that are linear in the receptor points
