@@ -335,10 +335,10 @@ def da_compare(args, cxs, desc):
 
 
 # HBM bytes of one 2,000-step plain MC launch on the ensemble workload without
-# records: rocprofv3 FETCH_SIZE 3775.0 KiB x 2 + WRITE_SIZE 786.8 KiB x 1 (the
+# records: rocprofv3 FETCH_SIZE 3845.0 KiB x 2 + WRITE_SIZE 786.8 KiB x 1 (the
 # factors a plain 256 MiB device copy gives), from
 # profiles/gd_mc_kernel_apl3_ensemble_3520rep_traffic.md
-TRAFFIC_ENSEMBLE_2000 = (3775.0 * 2 + 786.8125) * 1024
+TRAFFIC_ENSEMBLE_2000 = (3845.0 * 2 + 786.8125) * 1024
 
 
 def roofline(pairs_per_step, steps_per_launch, kernel_ms, kname, desc):
