@@ -254,11 +254,22 @@ def da_compare(args, cxs, desc):
         run.sur_tabulate()
     t_tab = time.perf_counter() - t0
 
-    per_cx = {}
+    per_cx, per_ladder = {}, {}
+
+    def ladder_best(c, b):
+        # a temperature ladder is one (protein conformer, ligand conformer)
+        # pair; its best is the minimum over its replicas
+        rep = run.state(c)["replica"]
+        key = rep[:, 1].astype(np.int64) * cxs[c].n_lig + rep[:, 3]
+        keys, inv = np.unique(key, return_inverse=True)
+        lb = np.full(len(keys), np.inf)
+        np.minimum.at(lb, inv, b)
+        return lb
 
     def best_stats(tag):
         bs = [run.best(c)[0].astype(np.float64) for c in range(len(cxs))]
         per_cx[tag] = np.array([b.min() for b in bs])
+        per_ladder[tag] = np.concatenate([ladder_best(c, b) for c, b in enumerate(bs)])
         b = np.concatenate(bs)
         return {"best_energy": float(b.min()), "best_p01": float(np.percentile(b, 1)),
                 "replica_best_mean": float(b.mean())}
@@ -301,6 +312,15 @@ def da_compare(args, cxs, desc):
                 "within_0p1": float(np.mean(np.abs(dcx) <= 0.1)),
                 "delta_equal_time_mean": float((per_cx["da_eq"] - per_cx["plain"]).mean()),
                 "delta_no_network_mean": float((per_cx["da_nonet"] - per_cx["plain"]).mean())}
+    # the same comparison per temperature ladder: many more samples than the
+    # one minimum per complex, so a steadier view of what DA gives up
+    dl = per_ladder["da"] - per_ladder["plain"]
+    best_cmp.update({"ladders": int(len(dl)), "ladder_delta_mean": float(dl.mean()),
+                     "ladders_within_0p1": float(np.mean(np.abs(dl) <= 0.1)),
+                     "ladder_delta_equal_time_mean":
+                         float((per_ladder["da_eq"] - per_ladder["plain"]).mean()),
+                     "ladder_delta_no_network_mean":
+                         float((per_ladder["da_nonet"] - per_ladder["plain"]).mean())})
     pairs_per_step = sum(c.n_rep * c.lna * c.pnp for c in cxs)
     roof = roofline(pairs_per_step, min(args.steps, 2000), out["plain"]["kernel_ms_avg"], kname,
                     desc)
