@@ -77,7 +77,8 @@ struct ChunkPar {
 // All terms of src/kernel_cuda_l2_calcenergy.cu:105-184 plus the contact
 // count of CalcMcc_d:
 //   vdW  (p1 - p2) / (p1 lj0 (1 + lj1 d^2) + 1), p1 = p1a/d^9, p2 = p2a/d^6,
-//        evaluated multiplied through by d^9: num/den = (p1a - p2a d^3)/(p1a t + d^9);
+//        evaluated multiplied through by d^9:
+//        num/den = (p1a - p2a d^3) / (a0 + a1 d^2 + d^6 d^3);
 //   ele  ele_p * g(el1 d) (the ligand charge is applied per atom later);
 //        g = 1/(el1 d) is accumulated for every pair (the point stores
 //        ele_p/el1); where el1 d < 1 the near form replaces it;
@@ -93,7 +94,10 @@ struct ChunkPar {
 // the PMF sigmoid differs from pmf1 by at most 2^-17 of pmf1 and the H-bond
 // term is below 2^-40 of its peak, so PMF counts as pmf1 and H-bond as 0);
 // contacts, the near electrostatic form and PSP run only when some lane is
-// within contact range.
+// within contact range. Two chunk-uniform choices are template parameters:
+// CLAMP keeps the PMF exponent finite where the chunk's pmf0 lets it grow
+// past 60, HB evaluates the H-bond term where some lane's atom has an H-bond
+// record with the chunk's receptor type.
 template <int APL, int KIND, int NP, bool CLAMP, bool HB>
 __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)[NP],
                                            const PairConst &K, float nr2max,
