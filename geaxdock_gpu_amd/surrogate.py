@@ -134,14 +134,16 @@ class PoseLog:
         return self.buf[:self.fill], self.cid[:self.fill]
 
 
-def log_states(eng, n_snapshots, every, nc=20, da=None):
+def log_states(eng, n_snapshots, every, nc=20, with_time=False):
     """Accepted states of a plain run: after every `every` steps, each
     replica's current pose and its exact energy (a record array with the
     gd_record layout, and the complex of each entry). Memory is bounded by
-    n_snapshots x replicas, whatever the run length."""
+    n_snapshots x replicas, whatever the run length. With `with_time`, the
+    run's summed kernel seconds are returned as a third value."""
     out, cids = [], []
+    t_kernel = 0.0
     for _ in range(n_snapshots):
-        eng.run(every, nc)
+        t_kernel += eng.run(every, nc)["t_compute"]
         for c in range(len(eng.complexes)):
             st = eng.state(c)
             r = np.zeros(len(st["movematrix"]), abi.RECORD_DTYPE)
@@ -150,6 +152,8 @@ def log_states(eng, n_snapshots, every, nc=20, da=None):
             r["movematrix"] = st["movematrix"]
             out.append(r)
             cids.append(np.full(len(r), c, np.int32))
+    if with_time:
+        return np.concatenate(out), np.concatenate(cids), t_kernel
     return np.concatenate(out), np.concatenate(cids)
 
 

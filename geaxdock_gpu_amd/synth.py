@@ -30,6 +30,7 @@ Choices the workload descriptions leave open (all fixed by the seed):
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -297,12 +298,30 @@ def uneven_batch(n_complex, seed=2, lna_range=(12, 64), pnp_range=(800, 4000), n
     """Uneven batch: ligand atoms U[12,64], receptor points U[800,4000], each
     complex 5 ligand x 3 protein conformers x 8 temperatures."""
     rng = np.random.default_rng([seed, 0xBA7C])
-    out = []
+    jobs = []
     for i in range(n_complex):
         lna = int(rng.integers(lna_range[0], lna_range[1] + 1))
         pnp = int(rng.integers(pnp_range[0], pnp_range[1] + 1))
         args = dict(lna=lna, pnp=pnp, pnk=100, pos=4, mcs_atoms=min(20, lna), n_lig=n_lig,
                     n_prt=n_prt, n_tmp=n_tmp, enepara_seed=seed)
         args.update(kw)
-        out.append(make_complex(seed * 100003 + i, **args))
+        jobs.append((seed * 100003 + i, args))
+    return _build_all(jobs)
+
+
+def _build_one(job):
+    return make_complex(job[0], **job[1])
+
+
+def _build_all(jobs, min_parallel=64):
+    """make_complex over (seed, kwargs) jobs, in order. Every complex depends
+    only on its own seed, so large batches are built by a process pool (fresh
+    interpreters: the caller may already hold a GPU context)."""
+    if len(jobs) < min_parallel:
+        return [_build_one(j) for j in jobs]
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    workers = max(1, min(16, (os.cpu_count() or 1)))
+    with cf.ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
+        out = list(ex.map(_build_one, jobs, chunksize=max(1, len(jobs) // (4 * workers))))
     return out

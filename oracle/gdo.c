@@ -129,6 +129,11 @@ int gdo_accept(float e_new, float e_old, float minus_beta, float u) {
  * src/kernel_cuda_l1_exchangereplicas.cu:33-78. */
 void gdo_exchange_group(int n_tmp, int mode_t, const float *etot, const float *mb,
                         const float *u, int32_t *idx_tmp, int32_t *acs) {
+  gdo_exchange_group_log(n_tmp, mode_t, etot, mb, u, idx_tmp, acs, NULL);
+}
+
+void gdo_exchange_group_log(int n_tmp, int mode_t, const float *etot, const float *mb,
+                            const float *u, int32_t *idx_tmp, int32_t *acs, float *margin) {
   int temps[GD_MAX_NTMP], order[GD_MAX_NTMP];
   float en[GD_MAX_NTMP];
   for (int t = 0; t < n_tmp; ++t) {
@@ -150,6 +155,7 @@ void gdo_exchange_group(int n_tmp, int mode_t, const float *etot, const float *m
       const float d2 = e2 - e1;
       const float delta = d1 * d2;
       const float prob = gdo_expf(delta);
+      if (margin) margin[pair] = (float)((double)delta - log((double)u[pair]));
       if (prob > u[pair]) {
         temps[o1] = temps[o2];
         temps[o2] = tt;
@@ -588,6 +594,21 @@ static void put_record(gdo_mc_io *io, int flat, const gd_replica *rp, const gd_e
   io->rec_cnt[flat] = c + 1;
 }
 
+/* Decision-log slot of (replica, step), or NULL when not logged. */
+static float *log_slot(const gdo_mc_io *io, float *log, int flat, int step) {
+  if (!log) return NULL;
+  const int i = step - io->log_step0;
+  if (i < 0 || i >= io->log_steps) return NULL;
+  return log + (size_t)flat * io->log_steps + i;
+}
+
+/* Margin of the Metropolis test u < exp((e_new - e_old) * minus_beta), with
+ * the FP32 operands the test uses: (e_new - e_old) * minus_beta - ln u. */
+static float margin_of(float e_new, float e_old, float minus_beta, float u) {
+  const float prod = (e_new - e_old) * minus_beta;
+  return (float)((double)prod - log((double)u));
+}
+
 /* MonteCarlo_Init_d + MonteCarlo_d + ExchangeReplicas_d on the CPU. */
 /* Replica exchange after segment seg for every (prt, lig) ladder. */
 static void exchange_all(const gd_complex_desc *cx, uint64_t seed, int cid, int seg,
@@ -608,7 +629,12 @@ static void exchange_all(const gd_complex_desc *cx, uint64_t seed, int cid, int 
         ac[t] = io->acs_x[f];
       }
       for (int k = 0; k < n_tmp / 2; ++k) u[k] = gdo_exchange_uniform(seed, cid, g, seg, k);
-      gdo_exchange_group(n_tmp, mode_t, et, mb, u, it, ac);
+      float *xm = NULL;
+      if (io->xlog && seg >= 0 && seg < io->log_segs) {
+        xm = io->xlog + ((size_t)seg * n_prt * n_lig + g) * (GD_MAX_NTMP / 2);
+        for (int k = 0; k < GD_MAX_NTMP / 2; ++k) xm[k] = NAN;
+      }
+      gdo_exchange_group_log(n_tmp, mode_t, et, mb, u, it, ac, xm);
       for (int t = 0; t < n_tmp; ++t) {
         const int f = n_tmp * n_lig * p + n_lig * t + l;
         io->idx_tmp[f] = it[t];
@@ -665,6 +691,8 @@ int gdo_mc_run(const gd_complex_desc *cx, uint64_t seed, int cid, int init, int 
         gd_energy en;
         gdo_energy_f32(cx, ref, p, l, mn, &en);
         const float mb = cx->temp[io->idx_tmp[flat]].minus_beta;
+        float *ml = log_slot(io, io->mlog, flat, st);
+        if (ml) *ml = margin_of(en.e[9], io->e_old[flat].e[9], mb, u[6]);
         if (gdo_accept(en.e[9], io->e_old[flat].e[9], mb, u[6])) {
           for (int i = 0; i < 6; ++i) mm[i] = mn[i];
           io->e_old[flat] = en;
@@ -738,6 +766,10 @@ int gdo_mc_run_da(const gd_complex_desc *cx, uint64_t seed, int cid, int step0, 
           gd_energy en;
           gdo_energy_f32(cx, ref, p, l, mn, &en);
           const float s_n = cheap_total(cx, &en);
+          float *ml = log_slot(io, io->mlog, flat, st + j);
+          if (ml) *ml = margin_of(s_n, s_y, mb, u[6]);
+          float *cl = log_slot(io, io->clog, flat, st + j);
+          if (cl) *cl = NAN;
           if (gdo_accept(s_n, s_y, mb, u[6])) {
             for (int i = 0; i < 6; ++i) yv[i] = mn[i];
             s_y = s_n;
@@ -751,6 +783,8 @@ int gdo_mc_run_da(const gd_complex_desc *cx, uint64_t seed, int cid, int step0, 
         const float d1 = E.e[9] - io->e_old[flat].e[9];
         const float d2 = s_y - s_old;
         const float delta = d1 - d2;
+        float *cl = log_slot(io, io->clog, flat, st + k_sub - 1);
+        if (cl) *cl = (float)((double)(delta * mb) - log((double)u7));
         if (u7 < gdo_expf(delta * mb)) {
           for (int i = 0; i < 6; ++i) mm[i] = yv[i];
           io->e_old[flat] = E;

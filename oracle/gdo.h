@@ -44,6 +44,10 @@ int gdo_accept(float e_new, float e_old, float minus_beta, float u);
  * temperature, u by pair. */
 void gdo_exchange_group(int n_tmp, int mode_t, const float *etot, const float *mb,
                         const float *u, int32_t *idx_tmp, int32_t *acs);
+/* The same sweep, writing each attempted pair's margin delta - ln U to
+ * margin[pair] when margin is not NULL. */
+void gdo_exchange_group_log(int n_tmp, int mode_t, const float *etot, const float *mb,
+                            const float *u, int32_t *idx_tmp, int32_t *acs, float *margin);
 
 /* ---- FP64 energy oracle ------------------------------------------------- */
 typedef struct gdo_energy64 {
@@ -78,6 +82,18 @@ typedef struct gdo_mc_io {
   gd_record *rec;     /* [n_rep*rec_cap] or NULL */
   int64_t rec_cap;    /* records per replica */
   int64_t *rec_cnt;   /* [n_rep] */
+  /* Decision margins, optional (NULL = not logged), in FP64 of the FP32
+   * operands: positive accepts. For steps s in [log_step0, log_step0+log_steps):
+   *   mlog[flat*log_steps + s-log_step0]  Metropolis test of step s,
+   *     minus_beta*(E_new-E_old) - ln u6 (delayed acceptance: the surrogate
+   *     sub-step test of step s);
+   *   clog[same index]  delayed acceptance only: the composite test of the
+   *     group ending at step s, minus_beta*(dE-dS) - ln u7 (NaN elsewhere);
+   *   xlog[(seg*n_ladders + g)*(GD_MAX_NTMP/2) + pair]  exchange pair `pair`
+   *     of ladder g (p*n_lig + l) after segment seg < log_segs,
+   *     (mb1-mb2)(E2-E1) - ln U (NaN when the pair is not attempted). */
+  float *mlog, *clog, *xlog;
+  int log_step0, log_steps, log_segs;
 } gdo_mc_io;
 
 /* Runs steps [step0, step0+n_steps). init=1 first evaluates the start poses

@@ -75,7 +75,10 @@ class McIO(C.Structure):
     _fields_ = [("mm", C.POINTER(C.c_float)), ("e_old", C.POINTER(abi.Energy)),
                 ("idx_tmp", C.POINTER(C.c_int32)), ("acs_mc", C.POINTER(C.c_int32)),
                 ("acs_x", C.POINTER(C.c_int32)), ("rec", C.POINTER(abi.Record)),
-                ("rec_cap", C.c_int64), ("rec_cnt", C.POINTER(C.c_int64))]
+                ("rec_cap", C.c_int64), ("rec_cnt", C.POINTER(C.c_int64)),
+                ("mlog", C.POINTER(C.c_float)), ("clog", C.POINTER(C.c_float)),
+                ("xlog", C.POINTER(C.c_float)), ("log_step0", C.c_int),
+                ("log_steps", C.c_int), ("log_segs", C.c_int)]
 
 
 _lib = None
@@ -153,6 +156,21 @@ class McState:
         self.rec_cap = rec_cap
         self.rec = np.zeros(max(1, n * rec_cap), abi.RECORD_DTYPE)
         self.rec_cnt = np.zeros(n, np.int64)
+        self.n_ladders = cx.n_prt * cx.n_lig
+        self.mlog = self.clog = self.xlog = None
+
+    def enable_log(self, step0, n_steps, n_segs):
+        """Log the decision margins (FP64 of the FP32 operands, positive
+        accepts) of steps [step0, step0 + n_steps) and of the exchanges after
+        segments 0 .. n_segs - 1: mlog[replica, step - step0] (Metropolis, or
+        a delayed-acceptance sub-step), clog[replica, step - step0] (composite
+        test of the DA group ending at that step) and xlog[seg, ladder, pair]."""
+        n = len(self.mm)
+        self.log_step0, self.log_steps, self.log_segs = int(step0), int(n_steps), int(n_segs)
+        self.mlog = np.full((n, max(1, n_steps)), np.nan, np.float32)
+        self.clog = np.full((n, max(1, n_steps)), np.nan, np.float32)
+        self.xlog = np.full((max(1, n_segs), self.n_ladders, abi.MAX_NTMP // 2), np.nan,
+                            np.float32)
 
     def io(self):
         io = McIO()
@@ -164,6 +182,12 @@ class McState:
         io.rec = C.cast(self.rec.ctypes.data, C.POINTER(abi.Record)) if self.rec_cap else None
         io.rec_cap = self.rec_cap
         io.rec_cnt = self.rec_cnt.ctypes.data_as(C.POINTER(C.c_int64))
+        if self.mlog is not None:
+            io.mlog = abi.fptr(self.mlog)
+            io.clog = abi.fptr(self.clog)
+            io.xlog = abi.fptr(self.xlog)
+            io.log_step0, io.log_steps, io.log_segs = (self.log_step0, self.log_steps,
+                                                       self.log_segs)
         return io
 
     def records(self, flat):
