@@ -16,7 +16,7 @@
 #include <vector>
 
 #include "gd_sur.hip"
-#include "gd_mc.hip"
+#include "gd_util.hip"
 
 using namespace gd;
 
@@ -849,21 +849,12 @@ KParams base_params(gd_ctx *ctx) {
 
 // Kernel instance for atoms per lane `apl` and mode `da`: 0 plain,
 // 1 delayed acceptance with the network per atom, 2 with its tables.
-template <int A>
-const void *mc_kernel_da(int da) {
-  switch (da) {
-    case 1: return (const void *)gd_mc_kernel<A, 1>;
-    case 2: return (const void *)gd_mc_kernel<A, 2>;
-    default: return (const void *)gd_mc_kernel<A, 0>;
-  }
-}
-
 const void *mc_kernel_fn(int apl, int da) {
   switch (apl) {
-    case 1: return mc_kernel_da<1>(da);
-    case 2: return mc_kernel_da<2>(da);
-    case 3: return mc_kernel_da<3>(da);
-    default: return mc_kernel_da<4>(da);
+    case 1: return mc_kernel_handle<1>(da);
+    case 2: return mc_kernel_handle<2>(da);
+    case 3: return mc_kernel_handle<3>(da);
+    default: return mc_kernel_handle<4>(da);
   }
 }
 

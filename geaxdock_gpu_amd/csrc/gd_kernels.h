@@ -120,4 +120,14 @@ struct KParams {
   const float *sb2, *sw3, *sb3;
 };
 
+// Host handle of the MC kernel instance for `APL` ligand atoms per lane and
+// mode `da` (0 plain, 1 delayed acceptance with the network per atom, 2 with
+// its tabulated outputs); defined in gd_mc_inst.hip, one object per APL.
+template <int APL>
+const void *mc_kernel_handle(int da);
+template <> const void *mc_kernel_handle<1>(int da);
+template <> const void *mc_kernel_handle<2>(int da);
+template <> const void *mc_kernel_handle<3>(int da);
+template <> const void *mc_kernel_handle<4>(int da);
+
 }  // namespace gd

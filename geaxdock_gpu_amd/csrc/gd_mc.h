@@ -1,4 +1,5 @@
-// gd_mc.hip -- the MC step kernels for CDNA4 (gfx950).
+// gd_mc.h -- the MC step kernel for CDNA4 (gfx950): device code and the
+// gd_mc_kernel template, instantiated per atoms-per-lane in gd_mc_inst.hip.
 //
 // One workgroup owns a pack: whole temperature ladders of one complex that
 // share one protein conformer. The receptor conformer, the type-pair
@@ -12,9 +13,13 @@
 // CalcRmsd_d, CalcMcc_d, CalcEnergy_d (+CombineEnergy_d), Accept_d,
 // RecordLigand_d; ExchangeReplicas_d between segments
 // (src/kernel_cuda_lancher_siglegpu.cu:25-31).
+#pragma once
+
 #include <type_traits>
+
 #include "gd_device.h"
 #include "gd_kernels.h"
+#include "gd_sur.h"
 
 namespace gd {
 
@@ -70,7 +75,6 @@ struct ChunkPar {
   float p1a, p2a, pmf0, pmf1, pb, pc, ha, hb, hc;
   float a0, a1;  // p1a lj0, p1a lj0 lj1: den = a0 + a1 d^2 + d^9
   float nr2;  // contacts and the near electrostatic form lie within sqrt(nr2)
-  float rf2;  // beyond sqrt(rf2) PMF is pmf1 within 2^-17 and H-bond < 2^-40 of its peak
 };
 
 // Receptor points [p, p + NP) of one chunk against the lane's APL atoms.
@@ -89,17 +93,16 @@ struct ChunkPar {
 //   hpc  hpp * switch(d^2) for d^2 <= 81 (KIND >= 1; S.hpp holds hpp / 2);
 //   psp  psp[lig_t][seq3r] when class 2 and d <= pmf0 (KIND == 2);
 //   contacts  d <= pmf0.
-// Two wave-uniform branches skip work no lane of the wave needs: when every
-// lane is beyond its far radius the exponentials are not evaluated (there
-// the PMF sigmoid differs from pmf1 by at most 2^-17 of pmf1 and the H-bond
-// term is below 2^-40 of its peak, so PMF counts as pmf1 and H-bond as 0);
-// contacts, the near electrostatic form and PSP run only when some lane is
-// within contact range. Two chunk-uniform choices are template parameters:
+// Every pair takes the full form of every term, so a pair's contribution is
+// a function of that pair alone. One wave-uniform branch skips work no lane
+// needs: contacts, the near electrostatic form and PSP run only when some lane
+// is within contact range (the terms inside test each pair exactly, so pairs
+// outside add exact zeros). Two chunk-uniform choices are template parameters:
 // CLAMP keeps the PMF exponent finite where the chunk's pmf0 lets it grow
 // past 60, HB evaluates the H-bond term where some lane's atom has an H-bond
 // record with the chunk's receptor type.
 template <int APL, int KIND, int NP, bool CLAMP, bool HB>
-__device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)[NP],
+__device__ __forceinline__ void pair_group(const Lds &S, int p, const float4 (&P)[NP],
                                            const PairConst &K, float nr2max,
                                            const float (&x)[APL], const float (&y)[APL],
                                            const float (&z)[APL], const ChunkPar (&cp)[APL],
@@ -107,9 +110,8 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
                                            float (&vdw)[APL], float (&ele)[APL],
                                            float (&pmf)[APL], float (&psp)[APL],
                                            float (&hdb)[APL], float (&hpc)[APL],
-                                           int (&cnt)[APL], float (&vdwf)[APL]) {
+                                           int (&cnt)[APL]) {
   const float pa = -0.5f * LOG2E;
-  int nfar = 0;
   float hp[NP];
   int ps[NP];
 #pragma unroll
@@ -118,7 +120,7 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
     ps[j] = (KIND == KIND_PSP) ? S.psp[p + j] : 0;
   }
   float d2[NP][APL], rd[NP][APL], dd[NP][APL], num[NP][APL], den[NP][APL];
-  bool mid = false, near = false;
+  bool near = false;
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     // contact range: the lane's nearest atom against the largest of its
@@ -133,7 +135,6 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
       d2[j][k] = q2;
       rd[j][k] = r;
       dd[j][k] = d;
-      mid |= q2 < cp[k].rf2;
       qmin = k == 0 ? q2 : fminf(qmin, q2);
       const float d3 = q2 * d;
       const float d6 = d3 * d3;
@@ -143,40 +144,31 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
     }
     near |= qmin <= nr2max;
   }
-  if (__ballot(mid) != 0) {
 #pragma unroll
-    for (int j = 0; j < NP; ++j)
+  for (int j = 0; j < NP; ++j)
 #pragma unroll
-      for (int k = 0; k < APL; ++k) {
-        const float q2 = d2[j][k], d = dd[j][k];
-        // the exponent is clamped at 64 (exp2 must stay finite) only in chunks
-        // whose parameters let it reach 60; elsewhere the clamp is a no-op
-        const float wv0 = fmaf(pa, q2, fmaf(cp[k].pb, d, cp[k].pc));
-        const float wv = CLAMP ? fminf(wv0, 64.0f) : wv0;
-        const float e1 = 1.0f + fast_exp2(wv);
-        const float r = fast_rcp(den[j][k] * e1);
-        vdw[k] = fmaf(num[j][k] * e1, r, vdw[k]);
-        pmf[k] = fmaf(den[j][k], r, pmf[k]);
-        // H-bond: only in chunks where some lane's atom has an H-bond record
-        // with this receptor type (elsewhere every term would be 0 * exp2)
-        if (HB) {
-          const float h = fmaf(d, cp[k].ha, cp[k].hb);
-          hdb[k] = fmaf(cp[k].hc, fast_exp2(-(h * h)), hdb[k]);
-        }
-        if (KIND >= KIND_HPC) {
-          const float u = fmaxf(fmaf(q2, -INV81, 1.0f), 0.0f);
-          const float u3 = u * u * u;
-          hpc[k] = fmaf(hp[j], fmaf(u3, u, u3), hpc[k]);
-        }
+    for (int k = 0; k < APL; ++k) {
+      const float q2 = d2[j][k], d = dd[j][k];
+      // the exponent is clamped at 64 (exp2 must stay finite) only in chunks
+      // whose parameters let it reach 60; elsewhere the clamp is a no-op
+      const float wv0 = fmaf(pa, q2, fmaf(cp[k].pb, d, cp[k].pc));
+      const float wv = CLAMP ? fminf(wv0, 64.0f) : wv0;
+      const float e1 = 1.0f + fast_exp2(wv);
+      const float r = fast_rcp(den[j][k] * e1);
+      vdw[k] = fmaf(num[j][k] * e1, r, vdw[k]);
+      pmf[k] = fmaf(den[j][k], r, pmf[k]);
+      // H-bond: only in chunks where some lane's atom has an H-bond record
+      // with this receptor type (elsewhere every term would be 0 * exp2)
+      if (HB) {
+        const float h = fmaf(d, cp[k].ha, cp[k].hb);
+        hdb[k] = fmaf(cp[k].hc, fast_exp2(-(h * h)), hdb[k]);
       }
-  } else {
-    // far field: PMF contributes pmf1 per point, added once per chunk
-#pragma unroll
-    for (int j = 0; j < NP; ++j)
-#pragma unroll
-      for (int k = 0; k < APL; ++k) vdwf[k] = fmaf(num[j][k], fast_rcp(den[j][k]), vdwf[k]);
-    nfar = NP;
-  }
+      if (KIND >= KIND_HPC) {
+        const float u = fmaxf(fmaf(q2, -INV81, 1.0f), 0.0f);
+        const float u3 = u * u * u;
+        hpc[k] = fmaf(hp[j], fmaf(u3, u, u3), hpc[k]);
+      }
+    }
   if (__ballot(near) != 0) {
 #pragma unroll
     for (int j = 0; j < NP; ++j)
@@ -195,7 +187,6 @@ __device__ __forceinline__ int pair_group(const Lds &S, int p, const float4 (&P)
         }
       }
   }
-  return nfar;
 }
 
 template <int APL, int KIND, bool CLAMP, bool HB>
@@ -210,13 +201,12 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, cons
                                             int (&cnt)[APL]) {
   // points per group: enough independent pairs per lane to cover latency
   constexpr int NP = APL >= 3 ? 1 : (APL == 2 ? 2 : 4);
-  int p = beg, nfar = 0;
-  // far-field vdW has its own accumulators so that no accumulator is
-  // updated on both sides of the far/mid branch (keeps them in place); the
-  // chunk's PMF sum is taken without pmf1, which multiplies it once at the end
-  float vdwf[APL], pmfc[APL];
+  int p = beg;
+  // the chunk's PMF sum is taken without pmf1, which multiplies it once at
+  // the end
+  float pmfc[APL];
 #pragma unroll
-  for (int k = 0; k < APL; ++k) vdwf[k] = pmfc[k] = 0.0f;
+  for (int k = 0; k < APL; ++k) pmfc[k] = 0.0f;
   // the next group's points are read from LDS one group ahead, so the read
   // latency overlaps the current group's arithmetic (indices clamped into
   // the chunk: the last prefetch re-reads its final point)
@@ -229,30 +219,26 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, cons
     for (; p + 2 * NP <= end; p += 2 * NP) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) Pb[j] = S.xyze[p + NP + j];
-      nfar += pair_group<APL, KIND, NP, CLAMP, HB>(S, p, Pa, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
-                                        hpc, cnt, vdwf);
+      pair_group<APL, KIND, NP, CLAMP, HB>(S, p, Pa, K, nr2max, x, y, z, cp, psprow, vdw, ele,
+                                           pmfc, psp, hdb, hpc, cnt);
 #pragma unroll
       for (int j = 0; j < NP; ++j) Pa[j] = S.xyze[min(p + 2 * NP + j, end - 1)];
-      nfar += pair_group<APL, KIND, NP, CLAMP, HB>(S, p + NP, Pb, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp,
-                                        hdb, hpc, cnt, vdwf);
+      pair_group<APL, KIND, NP, CLAMP, HB>(S, p + NP, Pb, K, nr2max, x, y, z, cp, psprow, vdw,
+                                           ele, pmfc, psp, hdb, hpc, cnt);
     }
     if (p + NP <= end) {
-      nfar += pair_group<APL, KIND, NP, CLAMP, HB>(S, p, Pa, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
-                                        hpc, cnt, vdwf);
+      pair_group<APL, KIND, NP, CLAMP, HB>(S, p, Pa, K, nr2max, x, y, z, cp, psprow, vdw, ele,
+                                           pmfc, psp, hdb, hpc, cnt);
       p += NP;
     }
   }
   for (; p < end; ++p) {
     const float4 P1[1] = {S.xyze[p]};
-    nfar += pair_group<APL, KIND, 1, CLAMP, HB>(S, p, P1, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc, psp, hdb,
-                                     hpc, cnt, vdwf);
+    pair_group<APL, KIND, 1, CLAMP, HB>(S, p, P1, K, nr2max, x, y, z, cp, psprow, vdw, ele, pmfc,
+                                        psp, hdb, hpc, cnt);
   }
-  const float nf = (float)nfar;
 #pragma unroll
-  for (int k = 0; k < APL; ++k) {
-    pmf[k] = fmaf(cp[k].pmf1, pmfc[k] + nf, pmf[k]);
-    vdw[k] += vdwf[k];
-  }
+  for (int k = 0; k < APL; ++k) pmf[k] = fmaf(cp[k].pmf1, pmfc[k], pmf[k]);
 }
 
 // Sum over MCS templates of tcc_m * sqrt(mean squared distance of the
@@ -361,14 +347,6 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
       // lie within this radius; the margin covers the rounding of d
       const float rn = fmaxf(a.z, C.inv_el1) * 1.0001f;
       cp[k].nr2 = rn * rn;
-      // far radius: (d/2 - 6)(d - pmf0) >= 17 ln 2 makes exp2 of the PMF
-      // exponent <= 2^-17, so pmf1 / (1 + e) is pmf1 within 2^-17 of pmf1;
-      // the H-bond exponent -h^2 <= -40
-      const float u = 0.5f * a.z - 6.0f;
-      float rf = (6.0f + 0.5f * a.z) + sqrtf(u * u + 2.0f * 17.0f * 0.69314718f);
-      if (b.z != 0.0f) rf = fmaxf(rf, (6.3245553f - b.y) / b.x);
-      rf *= 1.0001f;
-      cp[k].rf2 = rf * rf;
     }
     // the PMF exponent (6 - d/2)(d - pmf0) log2(e) peaks at (pmf0 - 12)^2 / 8
     // log2(e); only a chunk where some lane's peak passes 60 needs the clamp
@@ -1052,101 +1030,6 @@ __global__ void __launch_bounds__(512) gd_mc_kernel(KParams P) {
     mc_kernel_body<APL, DA>(P, idx, step0);
     if (!persistent) break;
   }
-}
-
-// Zero the per-class pack counters of persistent launches.
-extern "C" __global__ void gd_queue_reset_kernel(int *queue, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) queue[i] = 0;
-}
-
-// Best-so-far := current state, for every replica.
-extern "C" __global__ void gd_reset_best_kernel(int n, const gd_energy *e_old, const float *mm,
-                                                float *best_e, float *best_mm) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  best_e[i] = e_old[i].e[GD_MAXWEI - 1];
-  for (int k = 0; k < 6; ++k) best_mm[(size_t)i * 6 + k] = mm[(size_t)i * 6 + k];
-}
-
-// Reference contact map of a complex (InitRefMatrix_d,
-// src/kernel_cuda_l2_calcmcc.cu:57-83): contacts of replica 0 (ligand
-// conformer 0, protein conformer 0) in its start pose, as per-atom lists of
-// sorted receptor indices. One thread per ligand atom; run before any MCC.
-extern "C" __global__ void gd_ref_kernel(DevComplex *cxs, int cxi, int *overflow) {
-  DevComplex &C = cxs[cxi];
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= C.lna) return;
-  const int lna = C.lna;
-  const float m[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  const Rot R = euler(m[3], m[4], m[5]);
-  const float ox = C.lig_orig[l], oy = C.lig_orig[lna + l], oz = C.lig_orig[2 * lna + l];
-  const float x = R.r00 * ox + R.r01 * oy + R.r02 * oz + m[0] + C.pocket[0];
-  const float y = R.r10 * ox + R.r11 * oy + R.r12 * oz + m[1] + C.pocket[1];
-  const float z = R.r20 * ox + R.r21 * oy + R.r22 * oz + m[2] + C.pocket[2];
-  const int lt = C.lig_t[l];
-  int n = 0;
-  for (int ci = 0; ci < C.n_chunks; ++ci) {
-    const int4 ch = C.chunks[ci];
-    const float pmf0 = C.params[(ch.z * NLT + lt) * NPAR + P_PMF0];
-    for (int p = ch.x; p < ch.x + ch.y; ++p) {
-      const float4 P = C.prt_xyze[p];
-      const float d2 = pair_d2(x, y, z, P.x, P.y, P.z);
-      const float d = d2 * fast_rsq(d2);
-      if (d <= pmf0) {
-        if (n < KMAX) C.ref[(size_t)l * KMAX + n] = make_int2(p, __float_as_int(pmf0));
-        ++n;
-      }
-    }
-  }
-  if (n > KMAX) {
-    atomicAdd(overflow, 1);
-    n = KMAX;
-  }
-  C.ref_cnt[l] = n;
-}
-
-extern "C" __global__ void gd_accept_kernel(int n, const float *e_new, const float *e_old,
-                                            const float *mb, const float *u, int *acc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) acc[i] = metropolis(e_new[i], e_old[i], mb[i], u[i]) ? 1 : 0;
-}
-
-extern "C" __global__ void gd_exchange_kernel(int n_groups, int n_tmp, int mode_t,
-                                              const float *etot, const float *mb,
-                                              const float *u, int *idx_tmp, int *acs) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n_groups) return;
-  int order[GD_MAX_NTMP];
-  const float *ug = u + (size_t)g * (n_tmp / 2);
-  exchange_ladder(n_tmp, mode_t, etot + (size_t)g * n_tmp, mb, idx_tmp + (size_t)g * n_tmp,
-                  acs + (size_t)g * n_tmp, order, [&](int pair) { return ug[pair]; });
-}
-
-extern "C" __global__ void gd_philox_kernel(int n, const uint32_t *ctr, const uint32_t *key,
-                                            uint32_t *out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const u32x4 r = philox4x32_10({ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3]},
-                                key[2 * i], key[2 * i + 1]);
-  out[4 * i] = r.x;
-  out[4 * i + 1] = r.y;
-  out[4 * i + 2] = r.z;
-  out[4 * i + 3] = r.w;
-}
-
-// Gathers each replica's window of records into one dense buffer
-// (replica order, then step), ahead of one device-to-host copy.
-extern "C" __global__ void gd_compact_kernel(int n_rep, const gd_record *rec, const int *cnt,
-                                             const int64_t *off, int cap, gd_record *out) {
-  const int r = blockIdx.x;
-  if (r >= n_rep) return;
-  const int n = cnt[r];
-  const int64_t o = off[r];
-  const int *src = reinterpret_cast<const int *>(rec + (size_t)r * cap);
-  int *dst = reinterpret_cast<int *>(out + o);
-  const int words = n * (int)(sizeof(gd_record) / 4);
-  for (int i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
 }
 
 }  // namespace gd

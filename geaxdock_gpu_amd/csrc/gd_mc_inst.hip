@@ -1,0 +1,20 @@
+// gd_mc_inst.hip -- the MC kernel instances of one atoms-per-lane count,
+// GD_MC_APL (set by the build: this file is compiled once per count, so the
+// instances build in parallel). Each instance keeps only the registers its
+// lane shape needs.
+#include "gd_mc.h"
+
+namespace gd {
+
+// Host handle of gd_mc_kernel<GD_MC_APL, da>: 0 plain MC and pose evaluation,
+// 1 delayed acceptance with the network per atom, 2 with its tabulated outputs.
+template <>
+const void *mc_kernel_handle<GD_MC_APL>(int da) {
+  switch (da) {
+    case 1: return (const void *)gd_mc_kernel<GD_MC_APL, 1>;
+    case 2: return (const void *)gd_mc_kernel<GD_MC_APL, 2>;
+    default: return (const void *)gd_mc_kernel<GD_MC_APL, 0>;
+  }
+}
+
+}  // namespace gd

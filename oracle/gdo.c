@@ -403,6 +403,17 @@ void gdo_energy_f64(const gd_complex_desc *cx, const uint8_t *ref, int iprt, int
   free(x);
 }
 
+void gdo_energy_f64_batch(const gd_complex_desc *cx, const uint8_t *ref, int n, const int *iprt,
+                          const int *ilig, const float *mm, gdo_energy64 *out, int n_threads) {
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+  (void)n_threads;
+#endif
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < n; ++i) gdo_energy_f64(cx, ref, iprt[i], ilig[i], mm + 6 * (size_t)i, out + i);
+}
+
 /* ------------------------------------------------------------------------ */
 /* FP32 CPU path: the reference's arithmetic in single precision.           */
 /* ------------------------------------------------------------------------ */

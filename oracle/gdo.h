@@ -64,6 +64,10 @@ typedef struct gdo_energy64 {
 void gdo_refmap_f64(const gd_complex_desc *cx, uint8_t *ref);
 void gdo_energy_f64(const gd_complex_desc *cx, const uint8_t *ref, int iprt, int ilig,
                     const float mm[6], gdo_energy64 *out);
+/* gdo_energy_f64 of n poses (pose i: protein conformer iprt[i], ligand
+ * conformer ilig[i], movematrix mm[6i..6i+5]), OpenMP over poses. */
+void gdo_energy_f64_batch(const gd_complex_desc *cx, const uint8_t *ref, int n, const int *iprt,
+                          const int *ilig, const float *mm, gdo_energy64 *out, int n_threads);
 /* Ligand coordinates of a pose in FP64 (Move_d, src/kernel_cuda_l2_move.cu:21-80). */
 void gdo_place_f64(const gd_complex_desc *cx, int ilig, const float mm[6], double *x,
                    double *y, double *z);

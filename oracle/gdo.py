@@ -44,6 +44,9 @@ def load():
                                    fp, C.POINTER(Energy64)]
     lib.gdo_energy_f32.argtypes = [C.POINTER(abi.ComplexDesc), C.c_void_p, C.c_int, C.c_int,
                                    fp, C.POINTER(abi.Energy)]
+    lib.gdo_energy_f64_batch.argtypes = [C.POINTER(abi.ComplexDesc), C.c_void_p, C.c_int,
+                                         C.POINTER(C.c_int), C.POINTER(C.c_int), fp, C.c_void_p,
+                                         C.c_int]
     lib.gdo_place_f64.argtypes = [C.POINTER(abi.ComplexDesc), C.c_int, fp,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
@@ -122,6 +125,25 @@ def energy_f64(desc, ref, iprt, ilig, mm):
     mm = np.ascontiguousarray(mm, np.float32)
     lib().gdo_energy_f64(C.byref(desc), None if ref is None else ref.ctypes.data, iprt, ilig,
                          abi.fptr(mm), C.byref(out))
+    return out
+
+
+ENERGY64_DTYPE = np.dtype([("e", "<f8", 10), ("raw", "<f8", 10), ("mag", "<f8", 10),
+                           ("cms", "<f8"), ("rmsd", "<f8"), ("tp", "<i8"), ("fn", "<i8"),
+                           ("fp", "<i8"), ("tn", "<i8"), ("min_margin", "<f8")])
+
+
+def energy_f64_batch(desc, ref, iprt, ilig, mm, threads=0):
+    """energy_f64 of many poses (OpenMP over poses), as an ENERGY64_DTYPE array."""
+    iprt = np.ascontiguousarray(iprt, np.int32)
+    ilig = np.ascontiguousarray(ilig, np.int32)
+    mm = np.ascontiguousarray(mm, np.float32).reshape(-1, 6)
+    out = np.zeros(len(mm), ENERGY64_DTYPE)
+    assert ENERGY64_DTYPE.itemsize == C.sizeof(Energy64)
+    ip = C.POINTER(C.c_int)
+    lib().gdo_energy_f64_batch(C.byref(desc), None if ref is None else ref.ctypes.data, len(mm),
+                               iprt.ctypes.data_as(ip), ilig.ctypes.data_as(ip), abi.fptr(mm),
+                               out.ctypes.data, threads)
     return out
 
 

@@ -63,28 +63,18 @@ def test_pmf_exponent_peak():
     assert LOG2E * (7.38 - 12.0) ** 2 / 8.0 < 16.0
 
 
-def test_far_radius_bounds():
-    """Beyond the kernel's far radius the PMF sigmoid pmf1 / (1 + e^w) is pmf1
-    within 2^-17 of pmf1, and the H-bond Gaussian
-    exp(-((d - hdb0) hdb1)^2 / 2) (calcenergy.cu:179-184) is below 2^-40."""
-    for pmf0 in (-20.0, 2.69, 4.0, 7.38, 30.0):
-        u = 0.5 * pmf0 - 6.0
-        rf = (6.0 + 0.5 * pmf0) + math.sqrt(u * u + 2.0 * 17.0 * LN2)
-        d = np.linspace(rf, rf + 60.0, 20001)
-        w = (6.0 - d / 2.0) * (d - pmf0)
-        assert abs(w[0] + 17.0 * LN2) < 1e-9
-        rel = 1.0 - 1.0 / (1.0 + np.exp(w))
-        assert rel.max() <= 2.0 ** -17 * (1.0 + 1e-9)
+def test_hbond_gaussian_in_base_2():
+    """The kernel's H-bond Gaussian HC exp2(-(d HA + HB)^2), HA = hdb1
+    sqrt(log2(e) / 2), HB = -hdb0 HA, is the reference's
+    exp(-((d - hdb0) hdb1)^2 / 2) (src/kernel_cuda_l2_calcenergy.cu:179-184)
+    at every distance."""
     rng = np.random.default_rng(1)
+    d = np.linspace(0.0, 25.0, 5001)
     for _ in range(100):
         hdb0, sig = rng.uniform(3.29, 4.55), rng.uniform(0.6, 1.48)
         hdb1 = 1.0 / sig
-        # exp2(-(d HA + HB)^2) == exp(-((d - hdb0) hdb1)^2 / 2)
         ha = hdb1 * math.sqrt(LOG2E / 2.0)
         hb = -hdb0 * ha
-        rh = (math.sqrt(40.0) - hb) / ha
-        d = np.linspace(rh, rh + 30.0, 3001)
         g_ref = np.exp(-((d - hdb0) * hdb1) ** 2 / 2.0)
         g_ker = np.exp2(-(d * ha + hb) ** 2)
-        np.testing.assert_allclose(g_ker, g_ref, rtol=1e-9, atol=0)
-        assert g_ref.max() <= 2.0 ** -40 * (1.0 + 1e-9)
+        np.testing.assert_allclose(g_ker, g_ref, rtol=1e-9, atol=1e-300)
