@@ -51,6 +51,9 @@ def load_library():
     lib.gd_checkpoint.argtypes = [vp, vp, P(C.c_size_t)]
     lib.gd_restore.argtypes = [vp, vp, C.c_size_t]
     lib.gd_calc_energy.argtypes = [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_int), P(C.c_float), vp]
+    lib.gd_calc_energy_noalloc.argtypes = [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_int),
+                                           P(C.c_float), vp]
+    lib.gd_calc_energy_noalloc.restype = C.c_int
     lib.gd_accept.argtypes = [vp, C.c_int, P(C.c_float), P(C.c_float), P(C.c_float),
                               P(C.c_float), P(C.c_int32)]
     lib.gd_exchange.argtypes = [vp, C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_float),
@@ -255,6 +258,19 @@ class Engine:
         out = np.zeros(n, abi.ENERGY_DTYPE)
         self._check(self.lib.gd_calc_energy(self._ctx, cid, n, _i(idx_lig), _i(idx_prt), _f(mm),
                                             out.ctypes.data))
+        return out
+
+    def calc_energy_noalloc(self, cid, idx_lig, idx_prt, movematrix):
+        """calc_energy through gd_calc_energy_noalloc: no device allocation,
+        at most abi.NOALLOC_MAX_POSES poses; needs one calc_energy or run
+        call after the last add_complex."""
+        idx_lig = np.ascontiguousarray(idx_lig, np.int32)
+        idx_prt = np.ascontiguousarray(idx_prt, np.int32)
+        mm = np.ascontiguousarray(movematrix, np.float32).reshape(-1, 6)
+        n = mm.shape[0]
+        out = np.zeros(n, abi.ENERGY_DTYPE)
+        self._check(self.lib.gd_calc_energy_noalloc(self._ctx, cid, n, _i(idx_lig), _i(idx_prt),
+                                                    _f(mm), out.ctypes.data))
         return out
 
     def accept(self, e_new, e_old, minus_beta, u):

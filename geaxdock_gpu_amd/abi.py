@@ -91,6 +91,9 @@ class Opts(C.Structure):
                 ("launch", C.c_int)]
 
 
+NOALLOC_SCRATCH_BYTES = 4 << 20  # GD_NOALLOC_SCRATCH_BYTES
+NOALLOC_MAX_POSES = 32768        # GD_NOALLOC_MAX_POSES
+
 LAUNCH_NO_GRAPH = 1
 LAUNCH_PERSISTENT = 2
 

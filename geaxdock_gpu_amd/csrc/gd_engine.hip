@@ -20,6 +20,13 @@
 
 using namespace gd;
 
+// Device scratch of gd_calc_energy_noalloc: the pose tables and the energies
+// of one call (packs, slots, conformer indices, movematrices, outputs), so
+// that entry point allocates nothing. One copy per GPU, held until the
+// process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (gd_ctx::scratch).
+static __device__ __align__(256) unsigned char scratch[GD_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 // One kernel launch of a pack set: packs [begin, end) share the atoms-per-lane
@@ -55,6 +62,7 @@ struct gd_ctx {
   std::string err;
   int n_cu = 0;
   size_t max_lds = 65536;
+  unsigned char *scratch = nullptr;  // this device's copy of the file-scope scratch array
   std::vector<HostComplex> cx;
   DevComplex *d_cx = nullptr;
   bool cx_dirty = true;
@@ -220,6 +228,7 @@ gd_status gd_create(gd_ctx **out, const gd_opts *opts) {
     HIPCHK(hipEventCreateWithFlags(&ctx->ev_join[i], hipEventDisableTiming));
   }
   HIPCHK(hipMalloc(&ctx->d_overflow, sizeof(int)));
+  HIPCHK(hipGetSymbolAddress((void **)&ctx->scratch, HIP_SYMBOL(scratch)));
   return GD_OK;
 }
 
@@ -1258,7 +1267,23 @@ gd_status gd_restore(gd_ctx *ctx, const void *buf, size_t size) {
 
 static gd_status eval_poses(gd_ctx *ctx, int cid, int n, const int *idx_lig, const int *idx_prt,
                             const float *movematrix, gd_energy *out, float *out_sur,
-                            float sur_scale);
+                            float sur_scale, bool no_alloc = false);
+
+gd_status gd_calc_energy_noalloc(gd_ctx *ctx, int cid, int n, const int *idx_lig,
+                                 const int *idx_prt, const float *movematrix, gd_energy *out) {
+  if (!ctx || cid < 0 || cid >= (int)ctx->cx.size() || n < 0) return GD_ERR_INVALID;
+  if (n == 0) return GD_OK;
+  if (!idx_lig || !idx_prt || !movematrix || !out) return GD_ERR_INVALID;
+  if (n > GD_NOALLOC_MAX_POSES)
+    return fail(ctx, GD_ERR_CAPACITY, "gd_calc_energy_noalloc: %d poses > %d per call", n,
+                GD_NOALLOC_MAX_POSES);
+  if (!ctx->scratch) return fail(ctx, GD_ERR_STATE, "no device scratch address");
+  if (ctx->cx_dirty || !ctx->cx[cid].ref_built)
+    return fail(ctx, GD_ERR_STATE,
+                "gd_calc_energy_noalloc: the complex table and contact map are not on the device "
+                "yet; call gd_calc_energy or gd_run once after the last gd_add_complex");
+  return eval_poses(ctx, cid, n, idx_lig, idx_prt, movematrix, out, nullptr, 0.0f, true);
+}
 
 gd_status gd_calc_energy(gd_ctx *ctx, int cid, int n, const int *idx_lig, const int *idx_prt,
                          const float *movematrix, gd_energy *out) {
@@ -1278,15 +1303,19 @@ gd_status gd_sur_surrogate(gd_ctx *ctx, int cid, int n, const int *idx_lig, cons
   return eval_poses(ctx, cid, n, idx_lig, idx_prt, movematrix, nullptr, out, sur_scale);
 }
 
+// Energies (or surrogate totals) of n poses of one complex. With no_alloc the
+// pose tables and outputs live in the device scratch array (the caller has
+// checked n and that the complex table and contact map are on the device);
+// otherwise they are allocated for the call.
 static gd_status eval_poses(gd_ctx *ctx, int cid, int n, const int *idx_lig, const int *idx_prt,
                             const float *movematrix, gd_energy *out, float *out_sur,
-                            float sur_scale) {
+                            float sur_scale, bool no_alloc) {
   HostComplex &h = ctx->cx[cid];
   for (int i = 0; i < n; ++i)
     if (idx_lig[i] < 0 || idx_lig[i] >= h.d.n_lig || idx_prt[i] < 0 || idx_prt[i] >= h.d.n_prt)
       return fail(ctx, GD_ERR_INVALID, "pose %d: conformer index out of range", i);
   HIPCHK(hipSetDevice(ctx->device));
-  TRY(ensure_ref(ctx));
+  if (!no_alloc) TRY(ensure_ref(ctx));
   PackSet ps;
   const int per = std::max(1, 256 / h.lpr);
   for (int p = 0; p < h.d.n_prt; ++p) {
@@ -1327,12 +1356,30 @@ static gd_status eval_poses(gd_ctx *ctx, int cid, int n, const int *idx_lig, con
   float *d_mm = nullptr;
   gd_energy *d_out = nullptr;
   float *d_sur = nullptr;
-  HIPCHK(hipMalloc(&d_packs, ps.packs.size() * sizeof(Pack)));
-  HIPCHK(hipMalloc(&d_slots, ps.slots.size() * sizeof(int)));
-  HIPCHK(hipMalloc(&d_lig, n * sizeof(int)));
-  HIPCHK(hipMalloc(&d_mm, (size_t)n * 6 * sizeof(float)));
-  HIPCHK(hipMalloc(&d_out, (size_t)n * sizeof(gd_energy)));
-  HIPCHK(hipMalloc(&d_sur, (size_t)n * sizeof(float)));
+  if (no_alloc) {
+    // regions of the scratch array, each 256-byte aligned
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t o_slots = al(ps.packs.size() * sizeof(Pack));
+    const size_t o_lig = o_slots + al((size_t)n * sizeof(int));
+    const size_t o_mm = o_lig + al((size_t)n * sizeof(int));
+    const size_t o_out = o_mm + al((size_t)n * 6 * sizeof(float));
+    const size_t end = o_out + al((size_t)n * sizeof(gd_energy));
+    if (end > GD_NOALLOC_SCRATCH_BYTES)
+      return fail(ctx, GD_ERR_CAPACITY, "gd_calc_energy_noalloc: %zu scratch bytes needed, %d held",
+                  end, GD_NOALLOC_SCRATCH_BYTES);
+    d_packs = reinterpret_cast<Pack *>(ctx->scratch);
+    d_slots = reinterpret_cast<int *>(ctx->scratch + o_slots);
+    d_lig = reinterpret_cast<int *>(ctx->scratch + o_lig);
+    d_mm = reinterpret_cast<float *>(ctx->scratch + o_mm);
+    d_out = reinterpret_cast<gd_energy *>(ctx->scratch + o_out);
+  } else {
+    HIPCHK(hipMalloc(&d_packs, ps.packs.size() * sizeof(Pack)));
+    HIPCHK(hipMalloc(&d_slots, ps.slots.size() * sizeof(int)));
+    HIPCHK(hipMalloc(&d_lig, n * sizeof(int)));
+    HIPCHK(hipMalloc(&d_mm, (size_t)n * 6 * sizeof(float)));
+    HIPCHK(hipMalloc(&d_out, (size_t)n * sizeof(gd_energy)));
+    HIPCHK(hipMalloc(&d_sur, (size_t)n * sizeof(float)));
+  }
   HIPCHK(hipMemcpyAsync(d_packs, ps.packs.data(), ps.packs.size() * sizeof(Pack), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(d_slots, ps.slots.data(), ps.slots.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(d_lig, idx_lig, n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
@@ -1352,12 +1399,14 @@ static gd_status eval_poses(gd_ctx *ctx, int cid, int n, const int *idx_lig, con
   if (le == hipSuccess && sur)
     le = hipMemcpyAsync(out_sur, d_sur, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
   if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
-  (void)hipFree(d_sur);
-  (void)hipFree(d_packs);
-  (void)hipFree(d_slots);
-  (void)hipFree(d_lig);
-  (void)hipFree(d_mm);
-  (void)hipFree(d_out);
+  if (!no_alloc) {
+    (void)hipFree(d_sur);
+    (void)hipFree(d_packs);
+    (void)hipFree(d_slots);
+    (void)hipFree(d_lig);
+    (void)hipFree(d_mm);
+    (void)hipFree(d_out);
+  }
   if (le != hipSuccess) return fail(ctx, GD_ERR_HIP, "energy evaluation: %s", hipGetErrorString(le));
   return GD_OK;
 }

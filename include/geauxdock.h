@@ -234,6 +234,21 @@ gd_status gd_restore(gd_ctx *ctx, const void *buf, size_t size);
 gd_status gd_calc_energy(gd_ctx *ctx, int complex_id, int n, const int *idx_lig,
                          const int *idx_prt, const float *movematrix, gd_energy *out);
 
+/* gd_calc_energy without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. Same inputs, outputs
+ * and kernels as gd_calc_energy; the pose tables and energies of the call are
+ * staged in one file-scope device array of GD_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends). At most GD_NOALLOC_MAX_POSES poses per call
+ * (GD_ERR_CAPACITY beyond). The complex table and contact map must already be
+ * on the device: one gd_calc_energy or gd_run call after the last
+ * gd_add_complex (GD_ERR_STATE otherwise). Every call on a GPU shares that
+ * GPU's array, whatever the context: calls that overlap on one GPU race. */
+#define GD_NOALLOC_SCRATCH_BYTES (4 << 20)
+#define GD_NOALLOC_MAX_POSES 32768
+gd_status gd_calc_energy_noalloc(gd_ctx *ctx, int complex_id, int n, const int *idx_lig,
+                                 const int *idx_prt, const float *movematrix, gd_energy *out);
+
 /* Parity hook: Metropolis decisions accept[i] = u[i] < exp((e_new[i]-e_old[i]) *
  * minus_beta[i]) on the device (Accept_d, src/kernel_cuda_l2_accept.cu:16-42). */
 gd_status gd_accept(gd_ctx *ctx, int n, const float *e_new, const float *e_old,
