@@ -74,6 +74,8 @@ def parse(argv=None):
     ap.add_argument("--ladders", type=int, default=-1,
                     help="temperature ladders per workgroup (0 = engine's choice)")
     ap.add_argument("--launch", type=int, default=-1, help="GD_LAUNCH_* flags")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="receptor points per LDS tile (0: resident receptors where they fit)")
     ap.add_argument("--steps-per-dump", type=int, default=1000,
                     help="MC steps per launch window")
     ap.add_argument("--verbose", action="store_true")
@@ -259,7 +261,7 @@ def launch_options(args, cxs):
         opts = dict(receptor_split=2, ladders_per_pack=3, launch=abi.LAUNCH_PERSISTENT)
     else:
         opts = dict(receptor_split=0, ladders_per_pack=0, launch=0)
-    opts.update(lanes_per_replica=args.lpr, atoms_per_lane=args.apl)
+    opts.update(lanes_per_replica=args.lpr, atoms_per_lane=args.apl, receptor_tile=args.tile)
     if args.split >= 0:
         opts["receptor_split"] = args.split
     if args.ladders >= 0:

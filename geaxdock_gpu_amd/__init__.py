@@ -127,7 +127,7 @@ class Engine:
 
     def __init__(self, seed=0, device=0, steps_per_dump=2000, record=True, lanes_per_replica=0,
                  atoms_per_lane=0, verbose=False, receptor_split=0, ladders_per_pack=0,
-                 launch=0):
+                 launch=0, receptor_tile=0):
         self.lib = load_library()
         o = abi.Opts()
         o.device, o.seed, o.steps_per_dump = device, seed, steps_per_dump
@@ -136,6 +136,7 @@ class Engine:
         o.receptor_split = receptor_split
         o.ladders_per_pack = ladders_per_pack
         o.launch = launch
+        o.receptor_tile = receptor_tile
         self._ctx = C.c_void_p()
         st = self.lib.gd_create(C.byref(self._ctx), C.byref(o))
         if st != 0:

@@ -88,7 +88,7 @@ class Opts(C.Structure):
                 ("record", C.c_int), ("lanes_per_replica", C.c_int),
                 ("atoms_per_lane", C.c_int), ("verbose", C.c_int),
                 ("receptor_split", C.c_int), ("ladders_per_pack", C.c_int),
-                ("launch", C.c_int)]
+                ("launch", C.c_int), ("receptor_tile", C.c_int)]
 
 
 NOALLOC_SCRATCH_BYTES = 4 << 20  # GD_NOALLOC_SCRATCH_BYTES

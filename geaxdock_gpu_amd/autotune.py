@@ -10,11 +10,16 @@ mix of ligand and receptor sizes:
                     workgroup = ladders x n_tmp);
   launch            persistent workgroups or one workgroup per pack;
   lanes_per_replica lanes per replica (8..64; 0 lets the engine choose per
-                    complex from its atom count).
+                    complex from its atom count);
+  receptor_tile     receptor points per LDS tile (0: each receptor resident in
+                    LDS when it fits, tiles of 256 otherwise; 64..512: every
+                    receptor streams through LDS tiles of that length).
 
 `tune` times each candidate on the real complexes for a few MC steps (a
 fresh engine per candidate, same seed, no records) and returns the fastest
-options. Results are cached in a JSON file keyed by the workload's shape.
+options: first the shape knobs with the default tiling, then the tile
+lengths with the best shape. Results are cached in a JSON file keyed by the
+workload's shape.
 """
 import hashlib
 import json
@@ -32,20 +37,28 @@ def signature(cxs, nc):
     return h.hexdigest()[:16]
 
 
+TILES = (64, 128, 256, 512)
+
+
 def candidates(cxs, full=False):
-    """Option sets to try. The default grid covers the knobs that matter on
-    uneven batches; `full` adds the lane-shape axis."""
+    """Shape option sets to try (default tiling). The default grid covers the
+    knobs that matter on uneven batches; `full` adds the lane-shape axis."""
     out = []
     for split in (1, 2):
         for ladders in (0, 1, 2, 3, 4):
             for launch in (0, abi.LAUNCH_PERSISTENT):
                 out.append(dict(receptor_split=split, ladders_per_pack=ladders, launch=launch,
-                                lanes_per_replica=0))
+                                lanes_per_replica=0, receptor_tile=0))
     if full:
         for lpr in (8, 16, 32):
             out.append(dict(receptor_split=0, ladders_per_pack=0, launch=0,
-                            lanes_per_replica=lpr))
+                            lanes_per_replica=lpr, receptor_tile=0))
     return out
+
+
+def tile_candidates(best, tiles=TILES):
+    """The best shape with each receptor tile length."""
+    return [dict(best, receptor_tile=t) for t in tiles]
 
 
 def time_candidate(cxs, opts, steps=40, warmup=8, nc=20, seed=0):
@@ -63,8 +76,11 @@ def time_candidate(cxs, opts, steps=40, warmup=8, nc=20, seed=0):
         return None
 
 
-def tune(cxs, nc=20, steps=40, warmup=8, cache=None, full=False, budget_s=120.0, log=None):
-    """Fastest option set for these complexes: (options, table of trials)."""
+def tune(cxs, nc=20, steps=40, warmup=8, cache=None, full=False, budget_s=120.0, log=None,
+         tiles=TILES, shapes=None):
+    """Fastest option set for these complexes: (options, table of trials).
+    `shapes` replaces the shape grid of the first stage; `tiles` are the
+    tile lengths of the second."""
     key = signature(cxs, nc)
     if cache and os.path.exists(cache):
         with open(cache) as f:
@@ -73,17 +89,25 @@ def tune(cxs, nc=20, steps=40, warmup=8, cache=None, full=False, budget_s=120.0,
             return db[key]["best"], db[key]["trials"]
     t_end = time.perf_counter() + budget_s
     trials = []
-    for opts in candidates(cxs, full):
-        if time.perf_counter() > t_end:
-            break
-        s = time_candidate(cxs, opts, steps, warmup, nc)
-        trials.append({"opts": opts, "s_per_step": s})
-        if log:
-            log(f"autotune {opts} -> {s}")
-    ok = [t for t in trials if t["s_per_step"] is not None]
-    if not ok:
-        raise RuntimeError("autotune: no candidate launch shape was accepted")
-    best = min(ok, key=lambda t: t["s_per_step"])["opts"]
+
+    def run(cands):
+        for opts in cands:
+            if time.perf_counter() > t_end:
+                break
+            s = time_candidate(cxs, opts, steps, warmup, nc)
+            trials.append({"opts": opts, "s_per_step": s})
+            if log:
+                log(f"autotune {opts} -> {s}")
+
+    def fastest():
+        ok = [t for t in trials if t["s_per_step"] is not None]
+        if not ok:
+            raise RuntimeError("autotune: no candidate launch shape was accepted")
+        return min(ok, key=lambda t: t["s_per_step"])["opts"]
+
+    run(shapes if shapes is not None else candidates(cxs, full))
+    run(tile_candidates(fastest(), tiles))
+    best = fastest()
     if cache:
         db = {}
         if os.path.exists(cache):

@@ -33,6 +33,7 @@ namespace {
 // kernel instance, the LDS layout and the block shape.
 struct LaunchClass {
   int apl = 1, begin = 0, end = 0;
+  int tile = 0;    // receptor tile length (0: receptor resident in LDS)
   int block = 64;  // lanes: n_split groups of `half`
   size_t lds = 0;  // dynamic LDS bytes
   KParams lay{};   // LDS offsets and receptor-split fields
@@ -42,6 +43,7 @@ struct HostComplex {
   gd_complex_desc d;  // sizes, scalars (pointers are not kept)
   int n_rep = 0, rep_base = 0;
   int lpr = 0, apl = 0, n_chunks = 0;
+  bool tiled = false;  // receptor streamed through LDS tiles
   bool ref_built = false;
   std::vector<int> lig_t;  // ligand atom types (host copy)
   DevComplex dev{};
@@ -180,12 +182,22 @@ void choose_shape(int lna, int n_tmp, int force_lpr, int force_apl, int *lpr, in
   *apl = best_a;
 }
 
-size_t lds_need(const HostComplex &h, int slots) {
+// Default tile length of receptors that do not fit in LDS.
+constexpr int DEFAULT_TILE = 256;
+
+// LDS bytes of a complex's pack with `slots` replicas: the resident layout
+// (tile = 0) holds the receptor conformer and KDE points, the tiled layout two
+// receptor tiles.
+size_t lds_need(const HostComplex &h, int slots, int tile) {
   const gd_complex_desc &d = h.d;
-  size_t b = (size_t)d.pnp * 16 + (size_t)d.pnp * 8 + PAR_FLOATS * 4 + (size_t)d.pnk * 16 +
-             NLT * 8 + (size_t)h.n_chunks * 16 + (size_t)slots * 16 + 256;
-  return b;
+  const size_t pts = tile > 0 ? (size_t)2 * tile : (size_t)d.pnp;
+  const size_t kde = tile > 0 ? 0 : (size_t)d.pnk * 16;
+  return pts * 24 + PAR_FLOATS * 4 + kde + NLT * 8 + (size_t)h.n_chunks * 16 +
+         (size_t)slots * 16 + 256;
 }
+
+// Tile length of a complex's packs, 0 when its receptor stays resident.
+int complex_tile(const gd_ctx *ctx, const HostComplex &h);
 
 }  // namespace
 
@@ -547,10 +559,14 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
     D.ref_cnt = (int *)p;
   }
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  h.lds_bytes = lds_need(h, 512 / h.lpr);
-  if (h.lds_bytes > ctx->max_lds)
-    return fail(ctx, GD_ERR_CAPACITY, "complex needs %zu bytes of LDS, device has %zu",
-                h.lds_bytes, ctx->max_lds);
+  h.lds_bytes = lds_need(h, 512 / h.lpr, 0);
+  h.tiled = ctx->opts.receptor_tile > 0 || h.lds_bytes > ctx->max_lds;
+  if (h.tiled) {
+    h.lds_bytes = lds_need(h, 512 / h.lpr, complex_tile(ctx, h));
+    if (h.lds_bytes > ctx->max_lds)
+      return fail(ctx, GD_ERR_CAPACITY, "complex needs %zu bytes of LDS with receptor tiles, "
+                  "device has %zu", h.lds_bytes, ctx->max_lds);
+  }
   ctx->cx_dirty = true;
   if (complex_id) *complex_id = cid;
   return GD_OK;
@@ -606,10 +622,21 @@ gd_status ensure_ref(gd_ctx *ctx) {
   return GD_OK;
 }
 
-// LDS layout for a launch covering packs with these maxima.
+int complex_tile(const gd_ctx *ctx, const HostComplex &h) {
+  if (!h.tiled) return 0;
+  return ctx->opts.receptor_tile > 0 ? ctx->opts.receptor_tile : DEFAULT_TILE;
+}
+
+// LDS layout for a launch covering packs with these maxima. With tile > 0
+// the receptor arrays are two tiles of `tile` points and KDE stays in HBM.
 void layout(KParams &P, size_t *bytes, int pnp_max, int pnk_max, int nch_max, int slots_max,
-            int n_split, int half) {
+            int n_split, int half, int tile = 0) {
   auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
+  P.tile = tile;
+  if (tile > 0) {
+    pnp_max = 2 * tile;
+    pnk_max = 0;
+  }
   size_t o = al((size_t)pnp_max * 16);
   P.off_hpp = (int)o;
   o = al(o + (size_t)pnp_max * 4);
@@ -667,6 +694,7 @@ void apply_layout(KParams &P, const KParams &L) {
   P.xp_floats = L.xp_floats;
   P.xp_slots = L.xp_slots;
   P.off_q = L.off_q;
+  P.tile = L.tile;
 }
 
 // Group packs into launch classes by (atoms per lane, workgroups per CU that
@@ -675,17 +703,20 @@ void apply_layout(KParams &P, const KParams &L) {
 // among its packs, so small complexes keep several workgroups per CU.
 void pack_finish(const gd_ctx *ctx, PackSet &ps, int ns) {
   const size_t n_pk = ps.packs.size();
-  std::vector<int> res(n_pk), half(n_pk), apl(n_pk);
+  std::vector<int> res(n_pk), half(n_pk), apl(n_pk), tile(n_pk);
   std::vector<double> cost(n_pk);
   for (size_t i = 0; i < n_pk; ++i) {
     const Pack &p = ps.packs[i];
     const HostComplex &h = ctx->cx[p.cx];
     half[i] = ((p.n_slots * h.lpr + 63) / 64) * 64;
     apl[i] = h.apl;
+    // the tile length depends on the options only (not on the pack's block),
+    // so every launch of a complex sums its chunks in the same order
+    tile[i] = complex_tile(ctx, h);
     cost[i] = (double)h.d.pnp * (double)half[i] * h.apl;
     KParams L{};
     size_t need = 0;
-    layout(L, &need, (h.d.pnp + 3) & ~3, h.d.pnk, h.n_chunks, p.n_slots, ns, half[i]);
+    layout(L, &need, (h.d.pnp + 3) & ~3, h.d.pnk, h.n_chunks, p.n_slots, ns, half[i], tile[i]);
     res[i] = (int)std::max<size_t>(1, std::min<size_t>(8, ctx->max_lds / std::max<size_t>(need, 1)));
   }
   // all per-pack keys are indexed by the pack's position before sorting
@@ -693,6 +724,7 @@ void pack_finish(const gd_ctx *ctx, PackSet &ps, int ns) {
   for (size_t i = 0; i < n_pk; ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     if (apl[a] != apl[b]) return apl[a] < apl[b];
+    if (tile[a] != tile[b]) return tile[a] < tile[b];
     if (res[a] != res[b]) return res[a] < res[b];
     return cost[a] > cost[b];
   });
@@ -707,7 +739,8 @@ void pack_finish(const gd_ctx *ctx, PackSet &ps, int ns) {
     lc.apl = a0;
     lc.begin = (int)b;
     int pnp_max = 0, pnk_max = 0, nch_max = 0, slots_max = 1, half_max = 64;
-    while (e < n_pk && apl[order[e]] == a0 && res[order[e]] == r0) {
+    while (e < n_pk && apl[order[e]] == a0 && res[order[e]] == r0 &&
+           tile[order[e]] == tile[order[b]]) {
       const Pack &p = ps.packs[e];
       const HostComplex &h = ctx->cx[p.cx];
       pnp_max = std::max(pnp_max, h.d.pnp);
@@ -719,7 +752,9 @@ void pack_finish(const gd_ctx *ctx, PackSet &ps, int ns) {
     }
     lc.end = (int)e;
     lc.block = half_max * ns;
-    layout(lc.lay, &lc.lds, (pnp_max + 3) & ~3, pnk_max, nch_max, slots_max, ns, half_max);
+    lc.tile = tile[order[b]];
+    layout(lc.lay, &lc.lds, (pnp_max + 3) & ~3, pnk_max, nch_max, slots_max, ns, half_max,
+           lc.tile);
     ps.classes.push_back(lc);
     b = e;
   }
@@ -858,12 +893,12 @@ KParams base_params(gd_ctx *ctx) {
 
 // Kernel instance for atoms per lane `apl` and mode `da`: 0 plain,
 // 1 delayed acceptance with the network per atom, 2 with its tables.
-const void *mc_kernel_fn(int apl, int da) {
+const void *mc_kernel_fn(int apl, int da, bool tiled) {
   switch (apl) {
-    case 1: return mc_kernel_handle<1>(da);
-    case 2: return mc_kernel_handle<2>(da);
-    case 3: return mc_kernel_handle<3>(da);
-    default: return mc_kernel_handle<4>(da);
+    case 1: return mc_kernel_handle<1>(da, tiled);
+    case 2: return mc_kernel_handle<2>(da, tiled);
+    case 3: return mc_kernel_handle<3>(da, tiled);
+    default: return mc_kernel_handle<4>(da, tiled);
   }
 }
 
@@ -915,7 +950,7 @@ hipError_t plan_classes(gd_ctx *ctx, const std::vector<LaunchClass> &classes, co
       P.queue = queue + i;
       P.n_queue = n_pk;
     }
-    cl.f = mc_kernel_fn(lc.apl, da);
+    cl.f = mc_kernel_fn(lc.apl, da, lc.tile > 0);
     const hipError_t e = hipFuncSetAttribute(cl.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cl.lds);
     if (e != hipSuccess) return e;
     out.push_back(cl);

@@ -111,6 +111,7 @@ struct KParams {
   // partial sums through LDS at off_xp (two parity buffers of xp_floats)
   int n_split, half, off_xp, xp_floats, xp_slots;
   int off_da;  // per-wave surrogate scratch (direct network evaluation)
+  int tile;    // receptor points per LDS tile of tiled classes (0: receptor resident in LDS)
   int sur_tab;  // 1: surrogate from the tabulated network outputs
   float *ev_sur;  // MODE_SUREVAL output, indexed by slot value
   // delayed acceptance: k surrogate sub-steps per exact evaluation (0 = plain)
@@ -120,14 +121,15 @@ struct KParams {
   const float *sb2, *sw3, *sb3;
 };
 
-// Host handle of the MC kernel instance for `APL` ligand atoms per lane and
+// Host handle of the MC kernel instance for `APL` ligand atoms per lane,
 // mode `da` (0 plain, 1 delayed acceptance with the network per atom, 2 with
-// its tabulated outputs); defined in gd_mc_inst.hip, one object per APL.
+// its tabulated outputs) and receptor tiling; defined in gd_mc_inst.hip, one
+// object per APL.
 template <int APL>
-const void *mc_kernel_handle(int da);
-template <> const void *mc_kernel_handle<1>(int da);
-template <> const void *mc_kernel_handle<2>(int da);
-template <> const void *mc_kernel_handle<3>(int da);
-template <> const void *mc_kernel_handle<4>(int da);
+const void *mc_kernel_handle(int da, bool tiled);
+template <> const void *mc_kernel_handle<1>(int da, bool tiled);
+template <> const void *mc_kernel_handle<2>(int da, bool tiled);
+template <> const void *mc_kernel_handle<3>(int da, bool tiled);
+template <> const void *mc_kernel_handle<4>(int da, bool tiled);
 
 }  // namespace gd

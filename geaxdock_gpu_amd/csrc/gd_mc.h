@@ -28,7 +28,7 @@ struct Lds {
   const float *hpp;
   const int *psp;
   const float *par;
-  const float4 *kde;
+  const float4 *kde;  // LDS, or HBM for tiled receptors
   const int2 *kr;
   const int4 *ch;
   float *xe;
@@ -36,6 +36,15 @@ struct Lds {
   int *xa;
   int *xo;
   float *xp;
+  // tiled receptors: the pack's conformer in HBM (sorted order) and the two
+  // LDS tile buffers of `tile` points each
+  const float4 *gx;
+  const float *ghpp;
+  const int *gpsp;
+  float4 *tx;
+  float *th;
+  int *tp;
+  int tile;
 };
 
 // Receptor split of one workgroup: lanes [h*half, (h+1)*half) form group h
@@ -290,7 +299,15 @@ __device__ __forceinline__ float mcs_sum(const DevComplex &C, int lpr, const flo
 // evaluate; the groups exchange their partial sums through LDS behind one
 // workgroup barrier. Two-operand float sums are order-free, so both groups
 // end with identical bits.
-template <int APL>
+//
+// TILE: the receptor does not stay resident in LDS; its sorted points stream
+// through two LDS tile buffers of S.tile points. While a tile is evaluated
+// the next one is read from HBM into registers (one point per thread; points
+// beyond the workgroup size after the evaluation) and written to the other
+// buffer; one workgroup barrier per tile.
+// Every thread runs the tile loop (waves without a pose only stage). Chunks
+// that cross a tile boundary are evaluated as one fragment per tile.
+template <int APL, bool TILE>
 __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, int lpr, Split &sp,
                             bool live, const float (&x)[APL],
                             const float (&y)[APL], const float (&z)[APL],
@@ -317,12 +334,13 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
     psprow[k] = C.psp + (size_t)lt[k] * C.n_res;
   }
 
-  for (int ci = 0; live && ci < n_chunks; ++ci) {
+  // fragment [fb, fe) of chunk ci, indices into T's point arrays; each lane
+  // group of a receptor split takes its share
+  auto fragment = [&](const Lds &T, int ci, int fb, int fe) {
     const int4 ch = S.ch[ci];
-    const int cb = __builtin_amdgcn_readfirstlane(ch.x);
-    const int cl = __builtin_amdgcn_readfirstlane(ch.y);
-    const int beg = cb + (cl * sp.h) / sp.ns;
-    const int end = cb + (cl * (sp.h + 1)) / sp.ns;
+    const int fl = fe - fb;
+    const int beg = fb + (fl * sp.h) / sp.ns;
+    const int end = fb + (fl * (sp.h + 1)) / sp.ns;
     const int type = __builtin_amdgcn_readfirstlane(ch.z);
     const int kind = __builtin_amdgcn_readfirstlane(ch.w);
     ChunkPar cp[APL];
@@ -363,7 +381,7 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
     const bool hb = __ballot(hbond) != 0;
     auto pairs = [&](auto kind_c, auto clamp_c, auto hb_c) {
       chunk_pairs<APL, decltype(kind_c)::value, decltype(clamp_c)::value,
-                  decltype(hb_c)::value>(S, beg, end, K, nr2max, x, y, z, cp, psprow, vdw, ele,
+                  decltype(hb_c)::value>(T, beg, end, K, nr2max, x, y, z, cp, psprow, vdw, ele,
                                          pmf, psp, hdb, hpc, cnt);
     };
     auto by_hb = [&](auto kind_c, auto clamp_c) {
@@ -377,6 +395,70 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
     if (kind == KIND_PLAIN) by_clamp(std::integral_constant<int, KIND_PLAIN>{});
     else if (kind == KIND_HPC) by_clamp(std::integral_constant<int, KIND_HPC>{});
     else by_clamp(std::integral_constant<int, KIND_PSP>{});
+  };
+
+  if constexpr (!TILE) {
+    for (int ci = 0; live && ci < n_chunks; ++ci) {
+      const int4 ch = S.ch[ci];
+      const int cb = __builtin_amdgcn_readfirstlane(ch.x);
+      const int cl = __builtin_amdgcn_readfirstlane(ch.y);
+      fragment(S, ci, cb, cb + cl);
+    }
+  } else {
+    const int T = S.tile, pnp = C.pnp;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int n_tiles = (pnp + T - 1) / T;
+    for (int i = tid; i < min(T, pnp); i += nt) {
+      S.tx[i] = S.gx[i];
+      S.th[i] = 0.5f * S.ghpp[i];
+      S.tp[i] = S.gpsp[i];
+    }
+    __syncthreads();
+    int ci = 0;
+    for (int t = 0; t < n_tiles; ++t) {
+      const int tb = t * T, te = min(tb + T, pnp);
+      // this thread's point of the next tile, read while this tile is evaluated
+      const int ni = te + tid;
+      const bool stage = tid < T && ni < min(te + T, pnp);
+      float4 nx = make_float4(0.f, 0.f, 0.f, 0.f);
+      float nh = 0.0f;
+      int np = 0;
+      if (stage) {
+        nx = S.gx[ni];
+        nh = S.ghpp[ni];
+        np = S.gpsp[ni];
+      }
+      if (live) {
+        Lds Tl = S;
+        const int cur = (t & 1) * T;
+        Tl.xyze = S.tx + cur;
+        Tl.hpp = S.th + cur;
+        Tl.psp = S.tp + cur;
+        while (ci < n_chunks) {
+          const int4 ch = S.ch[ci];
+          const int cb = __builtin_amdgcn_readfirstlane(ch.x);
+          const int ce = cb + __builtin_amdgcn_readfirstlane(ch.y);
+          const int fb = max(cb, tb), fe = min(ce, te);
+          if (fb < fe) fragment(Tl, ci, fb - tb, fe - tb);
+          if (ce > te) break;  // the chunk continues in the next tile
+          ++ci;
+        }
+      }
+      if (stage) {
+        const int o = ((t + 1) & 1) * T + tid;
+        S.tx[o] = nx;
+        S.th[o] = 0.5f * nh;
+        S.tp[o] = np;
+      }
+      // a tile longer than the workgroup: the rest of it, read now
+      for (int i = ni + nt; i < min(te + T, pnp); i += nt) {
+        const int o = ((t + 1) & 1) * T + (i - te);
+        S.tx[o] = S.gx[i];
+        S.th[o] = 0.5f * S.ghpp[i];
+        S.tp[o] = S.gpsp[i];
+      }
+      __syncthreads();
+    }
   }
 
   // pair sums that are linear in the receptor points
@@ -456,7 +538,7 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
     const int2 *rl = C.ref + (size_t)li[k] * KMAX;
     for (int j = 0; j < nr; ++j) {
       const int2 r = rl[j];
-      const float4 P = S.xyze[r.x];
+      const float4 P = TILE ? S.gx[r.x] : S.xyze[r.x];
       const float d2 = pair_d2(x[k], y[k], z[k], P.x, P.y, P.z);
       const float d = d2 * fast_rsq(d2);
       t_tp += (d <= __int_as_float(r.y)) ? 1 : 0;
@@ -695,7 +777,7 @@ __device__ __forceinline__ void exchange_ladder(int n_tmp, int mode_t, const flo
 
 // DA: 0 plain Metropolis; delayed acceptance with the surrogate from the
 // network evaluated per atom (1) or from its tabulated outputs (2).
-template <int APL, int DA>
+template <int APL, int DA, bool TILE>
 __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const DevComplex &C, const Lds &S,
                          int lpr, int step0) {
   const int tid = threadIdx.x;
@@ -779,7 +861,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   if (P.mode != MODE_MC) {
     place(mo);
     gd_energy E;
-    pose_energy<APL>(C, S, lpr, sp, true, x, y, z, ox, oy, oz, lt, q, li, val, mo, E);
+    pose_energy<APL, TILE>(C, S, lpr, sp, true, x, y, z, ox, oy, oz, lt, q, li, val, mo, E);
     if (active && lead) {
       if (P.mode == MODE_EVAL) {
         P.ev_out[sv] = E;
@@ -869,7 +951,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
         place(mn);
       }
       gd_energy E;
-      pose_energy<APL>(C, S, lpr, sp, wave_active, x, y, z, ox, oy, oz, lt, q, li, val, mn, E);
+      pose_energy<APL, TILE>(C, S, lpr, sp, wave_active, x, y, z, ox, oy, oz, lt, q, li, val, mn, E);
       if (wave_active && metropolis(E.e[GD_MAXWEI - 1], eo.e[GD_MAXWEI - 1], mb, u[6])) {
 #pragma unroll
         for (int i = 0; i < 6; ++i) mo[i] = mn[i];
@@ -904,7 +986,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
       const bool eval = wave_active && __ballot(moved) != 0;
       if (eval) place(yv);
       gd_energy E;
-      pose_energy<APL>(C, S, lpr, sp, eval, x, y, z, ox, oy, oz, lt, q, li, val, yv, E);
+      pose_energy<APL, TILE>(C, S, lpr, sp, eval, x, y, z, ox, oy, oz, lt, q, li, val, yv, E);
       if (eval) {
         const float d1 = E.e[GD_MAXWEI - 1] - eo.e[GD_MAXWEI - 1];
         const float d2 = s_y - s_old;
@@ -953,7 +1035,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   }
 }
 
-template <int APL, int DA>
+template <int APL, int DA, bool TILE>
 __device__ __forceinline__ void mc_kernel_body(const KParams &P, int idx, int step0) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   const Pack pk = P.packs[P.pack_base + idx];
@@ -971,9 +1053,18 @@ __device__ __forceinline__ void mc_kernel_body(const KParams &P, int idx, int st
   S.xa = reinterpret_cast<int *>(lds + P.off_xa);
   S.xo = reinterpret_cast<int *>(lds + P.off_xo);
   S.xp = reinterpret_cast<float *>(lds + P.off_xp);
-
-  // stage the pack's receptor conformer and tables
   const int pnp = C.pnp;
+  const float4 *src = C.prt_xyze + (size_t)pk.prt * pnp;
+  S.gx = src;
+  S.ghpp = C.prt_hpp;
+  S.gpsp = C.prt_psp;
+  S.tx = reinterpret_cast<float4 *>(lds);
+  S.th = reinterpret_cast<float *>(lds + P.off_hpp);
+  S.tp = reinterpret_cast<int *>(lds + P.off_psp);
+  S.tile = P.tile;
+  if constexpr (TILE) S.kde = C.kde;  // KDE points are read from HBM
+
+  // stage the pack's receptor conformer (unless it streams in tiles) and tables
   const int nt = blockDim.x;
   {
     float4 *xyze = reinterpret_cast<float4 *>(lds);
@@ -983,14 +1074,15 @@ __device__ __forceinline__ void mc_kernel_body(const KParams &P, int idx, int st
     float4 *kde = reinterpret_cast<float4 *>(lds + P.off_kde);
     int2 *kr = reinterpret_cast<int2 *>(lds + P.off_kr);
     int4 *ch = reinterpret_cast<int4 *>(lds + P.off_ch);
-    const float4 *src = C.prt_xyze + (size_t)pk.prt * pnp;
-    for (int i = threadIdx.x; i < pnp; i += nt) {
-      xyze[i] = src[i];
-      hpp[i] = 0.5f * C.prt_hpp[i];
-      psp[i] = C.prt_psp[i];
+    if constexpr (!TILE) {
+      for (int i = threadIdx.x; i < pnp; i += nt) {
+        xyze[i] = src[i];
+        hpp[i] = 0.5f * C.prt_hpp[i];
+        psp[i] = C.prt_psp[i];
+      }
+      for (int i = threadIdx.x; i < C.pnk; i += nt) kde[i] = C.kde[i];
     }
     for (int i = threadIdx.x; i < PAR_FLOATS; i += nt) par[i] = C.params[i];
-    for (int i = threadIdx.x; i < C.pnk; i += nt) kde[i] = C.kde[i];
     for (int i = threadIdx.x; i < NLT; i += nt) kr[i] = C.kde_range[i];
     for (int i = threadIdx.x; i < C.n_chunks; i += nt) ch[i] = C.chunks[i];
     if (P.mode == MODE_MC) {
@@ -1003,14 +1095,14 @@ __device__ __forceinline__ void mc_kernel_body(const KParams &P, int idx, int st
   }
   __syncthreads();
 
-  run_pack<APL, DA>(P, pk, C, S, C.lpr, step0);
+  run_pack<APL, DA, TILE>(P, pk, C, S, C.lpr, step0);
 }
 
 // MC kernel for packs whose complexes place APL ligand atoms on each lane:
 // plain MC, start-pose evaluation and pose evaluation (DA = false), or
 // delayed acceptance with surrogate sub-steps (DA = true). One instance per
 // APL keeps each variant's register allocation to what it needs.
-template <int APL, int DA>
+template <int APL, int DA, bool TILE>
 __global__ void __launch_bounds__(512) gd_mc_kernel(KParams P) {
   const int step0 = P.step_dev ? *P.step_dev : P.step0;
   // persistent launches: each workgroup takes packs in LPT order from a
@@ -1027,7 +1119,7 @@ __global__ void __launch_bounds__(512) gd_mc_kernel(KParams P) {
       __syncthreads();  // every thread has read the index; the previous pack's LDS is free
       if (idx >= P.n_queue) break;
     }
-    mc_kernel_body<APL, DA>(P, idx, step0);
+    mc_kernel_body<APL, DA, TILE>(P, idx, step0);
     if (!persistent) break;
   }
 }

@@ -6,14 +6,22 @@
 
 namespace gd {
 
-// Host handle of gd_mc_kernel<GD_MC_APL, da>: 0 plain MC and pose evaluation,
-// 1 delayed acceptance with the network per atom, 2 with its tabulated outputs.
+// Host handle of gd_mc_kernel<GD_MC_APL, da, tiled>: da 0 plain MC and pose
+// evaluation, 1 delayed acceptance with the network per atom, 2 with its
+// tabulated outputs; tiled: the receptor streams through LDS tiles.
 template <>
-const void *mc_kernel_handle<GD_MC_APL>(int da) {
+const void *mc_kernel_handle<GD_MC_APL>(int da, bool tiled) {
+  if (tiled) {
+    switch (da) {
+      case 1: return (const void *)gd_mc_kernel<GD_MC_APL, 1, true>;
+      case 2: return (const void *)gd_mc_kernel<GD_MC_APL, 2, true>;
+      default: return (const void *)gd_mc_kernel<GD_MC_APL, 0, true>;
+    }
+  }
   switch (da) {
-    case 1: return (const void *)gd_mc_kernel<GD_MC_APL, 1>;
-    case 2: return (const void *)gd_mc_kernel<GD_MC_APL, 2>;
-    default: return (const void *)gd_mc_kernel<GD_MC_APL, 0>;
+    case 1: return (const void *)gd_mc_kernel<GD_MC_APL, 1, false>;
+    case 2: return (const void *)gd_mc_kernel<GD_MC_APL, 2, false>;
+    default: return (const void *)gd_mc_kernel<GD_MC_APL, 0, false>;
   }
 }
 

@@ -167,6 +167,9 @@ typedef struct gd_opts {
   int ladders_per_pack;  /* 0 = auto; else temperature ladders per workgroup (replicas
                             per workgroup = this x n_tmp), capped by 512 lanes */
   int launch;            /* GD_LAUNCH_* flags; 0 = graph-captured, non-persistent */
+  int receptor_tile;     /* 0 = each receptor conformer resident in LDS when it fits,
+                            else streamed in tiles of 256 points; > 0 = every receptor
+                            streamed through LDS in tiles of this many points */
 } gd_opts;
 
 /* gd_opts.launch flags. MC windows are captured once into a HIP graph (all
