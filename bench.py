@@ -96,9 +96,21 @@ def parse(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--grid-spacing", type=float, default=0.5)
     ap.add_argument("--grid-half", type=float, default=14.0)
+    ap.add_argument("--t-floor", type=float, default=None,
+                    help="lowest kT of every ladder (default: the workload's 0.5)")
+    ap.add_argument("--t-ceiling", type=float, default=None,
+                    help="highest kT of every ladder (default: the workload's 5)")
+    ap.add_argument("--trans-sigma", type=float, default=None,
+                    help="translation step sigma, A (default: the workload's 0.2)")
+    ap.add_argument("--rot-sigma-deg", type=float, default=None,
+                    help="rotation step sigma, degrees (default: the workload's 3)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
-    return ap.parse_args(argv)
+    args = ap.parse_args(argv)
+    if args.mode == "da" and args.nc > 0 and args.nc % max(1, args.k_sub):
+        ap.error(f"--nc {args.nc} must be a multiple of --k-sub {args.k_sub}: delayed "
+                 "acceptance exchanges on whole groups of k sub-steps")
+    return args
 
 
 def round_up(n, m):
@@ -131,18 +143,29 @@ def copy_probe(mib, reps=5):
 
 
 def workload(args):
+    """The complexes and their description; the regime flags (ladder, move
+    sizes) override the workload's values and are named in the description."""
     from geaxdock_gpu_amd import synth
+    kw, regime = {}, {}
+    for flag, key in (("t_floor", "t_floor"), ("t_ceiling", "t_ceiling"),
+                      ("trans_sigma", "trans_sigma"), ("rot_sigma_deg", "rot_sigma_deg")):
+        v = getattr(args, flag, None)
+        if v is not None:
+            kw[key] = v
+            regime[key] = v
     if args.config == "single":
-        cxs = [synth.single_complex(seed=0)]
+        cxs = [synth.single_complex(seed=0, **kw)]
         desc = dict(workload="single complex: 1 lig x 1 prt x 8 temps, lna 32, pnp 1200")
     elif args.config == "batch":
-        cxs = synth.uneven_batch(args.batch, seed=2)
+        cxs = synth.uneven_batch(args.batch, seed=2, **kw)
         desc = dict(workload=f"uneven batch: {args.batch} complexes, lna U[12,64], pnp "
                              "U[800,4000], 5 lig x 3 prt x 8 temps each")
     else:
-        cxs = [synth.ensemble(seed=1)]
+        cxs = [synth.ensemble(seed=1, **kw)]
         desc = dict(workload="ensemble: 20 lig x 11 prt x 16 temps = 3520 replicas, lna 48, "
                              "pnp 3000, pnk 200, mcs 8x40")
+    if regime:
+        desc["regime"] = regime
     return cxs, desc
 
 

@@ -313,15 +313,26 @@ def _build_one(job):
     return make_complex(job[0], **job[1])
 
 
+def _gpu_touched():
+    """True once this process has loaded the engine library or initialized
+    torch's GPU context: forking is unsafe from then on."""
+    import sys
+    pkg = sys.modules.get(__package__)
+    if pkg is not None and getattr(pkg, "_lib", None) is not None:
+        return True
+    torch = sys.modules.get("torch")
+    return bool(torch is not None and torch.cuda.is_initialized())
+
+
 def _build_all(jobs, min_parallel=64):
     """make_complex over (seed, kwargs) jobs, in order. Every complex depends
-    only on its own seed, so large batches are built by a process pool (fresh
-    interpreters: the caller may already hold a GPU context)."""
-    if len(jobs) < min_parallel:
+    only on its own seed, so large batches are built by a forked process pool
+    while the process has not touched the GPU (sequentially afterwards)."""
+    if len(jobs) < min_parallel or _gpu_touched():
         return [_build_one(j) for j in jobs]
     import concurrent.futures as cf
     import multiprocessing as mp
     workers = max(1, min(16, (os.cpu_count() or 1)))
-    with cf.ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
+    with cf.ProcessPoolExecutor(workers, mp_context=mp.get_context("fork")) as ex:
         out = list(ex.map(_build_one, jobs, chunksize=max(1, len(jobs) // (4 * workers))))
     return out

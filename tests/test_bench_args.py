@@ -26,7 +26,7 @@ def test_driver_flags_give_whole_da_groups():
 def test_any_step_counts_round_to_valid_groups():
     for steps in (1, 3, 7, 20, 33):
         for warm in (0, 1, 5, 9):
-            for k in (1, 2, 3, 4, 8):
+            for k in (1, 2, 4, 5, 10):
                 a = bench.parse(["--steps", str(steps), "--warmup", str(warm), "--k-sub", str(k)])
                 timed, w, unit = bench.step_plan(a)
                 assert timed >= steps * a.mc_per_step and timed % unit == 0
@@ -45,3 +45,12 @@ def test_more_than_one_gpu_is_refused_with_a_message():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_exchange_interval_must_hold_whole_groups():
+    import pytest
+    with pytest.raises(SystemExit):
+        bench.parse(["--k-sub", "8"])
+    a = bench.parse(["--k-sub", "8", "--nc", "40"])
+    timed, warm, unit = bench.step_plan(a)
+    assert unit == 40 and timed % 40 == 0

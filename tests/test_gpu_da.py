@@ -301,3 +301,36 @@ def test_trained_surrogate_gains_over_no_surrogate(gd):
           f"untrained {v_random:.4g}")
     assert v_trained < 0.6 * v_none
     assert v_random > v_none
+
+
+def test_screen_where_plain_acceptance_is_low(gd):
+    """A cold ladder (kT 0.05-0.5) with large moves (0.8 A, 12 degrees): plain
+    Metropolis accepts well under half its proposals. Delayed acceptance from
+    one start state with the trained network as the screen accepts more
+    composite moves than with the screen reduced to the exact cheap terms
+    (scale 0), at the same exact evaluations."""
+    cx = synth.make_complex(seed=41, lna=32, pnp=1200, n_lig=4, n_prt=2, n_tmp=8, t_floor=0.05,
+                            t_ceiling=0.5, trans_sigma=0.8, rot_sigma_deg=12.0)
+    k, nc, n_steps = 4, 20, 800
+    eng = gd.Engine(seed=7, record=False)
+    cid = eng.add_complex(cx)
+    eng.run(200, nc)
+    st = eng.run(400, nc)
+    plain_acc = st["accepted"] / st["replica_steps"]
+    recs, rcid = sg.log_states(eng, 20, 20, nc)
+    eng.sur_build_grids(0.6, 12.0)
+    ts = sg.build_training_set(eng, recs, rcid, 200000, seed=1)
+    _, sigma = sg.load_training_set(eng, ts)
+    sg.fit(eng, ts, sigma, n_train=int(0.9 * len(ts["t"])), epochs=20)
+    eng.sur_tabulate()
+    blob = eng.checkpoint()
+    acc = {}
+    for tag, scale in (("network", sigma), ("none", 0.0)):
+        eng.restore(blob)
+        st = eng.run_da(n_steps, nc, k, scale)
+        acc[tag] = st["accepted"] / (st["replica_steps"] / k)
+    eng.close()
+    print(f"\nplain acceptance {plain_acc:.3f}; composite acceptance with network "
+          f"{acc['network']:.3f}, without {acc['none']:.3f}")
+    assert plain_acc < 0.5
+    assert acc["network"] > acc["none"] + 0.01
