@@ -105,7 +105,7 @@ struct gd_ctx {
     int64_t adam_t = 0;
     bool have_params = false;
     // dataset (rows padded, last 64 rows zero)
-    unsigned short *X = nullptr, *XT = nullptr;
+    unsigned short *X = nullptr;  // feature rows [cap + 64][64] bf16
     int64_t rows = 0, cap = 0;
     std::vector<int64_t> off{0};  // pose row offsets
     std::vector<float> target;
@@ -116,7 +116,6 @@ struct gd_ctx {
     bool dirty = true;
     // work buffers
     float *y = nullptr, *dS = nullptr, *sq = nullptr, *S = nullptr;
-    unsigned short *A1T = nullptr, *dH2T = nullptr, *dH1T = nullptr;
     float *p1 = nullptr, *p2 = nullptr, *pb2 = nullptr, *pw3 = nullptr;
     int64_t work_rows = 0;
   } sur;
@@ -264,9 +263,8 @@ void gd_destroy(gd_ctx *ctx) {
       if (g) (void)hipFree(g);
     for (float *g : S.tab)
       if (g) (void)hipFree(g);
-    void *qs[] = {S.p, S.m, S.v, S.grad, S.W1T, S.W2T, S.W2R, S.X, S.XT, S.pose_of_row,
-                  S.d_off, S.d_target, S.y, S.dS, S.sq, S.S, S.A1T, S.dH2T, S.dH1T, S.p1,
-                  S.p2, S.pb2, S.pw3};
+    void *qs[] = {S.p, S.m, S.v, S.grad, S.W1T, S.W2T, S.W2R, S.X, S.pose_of_row,
+                  S.d_off, S.d_target, S.y, S.dS, S.sq, S.S, S.p1, S.p2, S.pb2, S.pw3};
     for (void *p : qs)
       if (p) (void)hipFree(p);
   }
@@ -1615,23 +1613,25 @@ gd_status sur_upload_index(gd_ctx *ctx) {
   return GD_OK;
 }
 
-// work buffers for batches of up to `rows` rows (rounded up to 64)
+// Rows per workgroup of the fused backward (gd_mlp_train_kernel).
+constexpr int64_t SUR_CHUNK = 512;
+
+// work buffers for batches of up to `rows` rows (rounded up to 64): the
+// gradient partials of the fused backward's workgroups and per-pose outputs
 gd_status sur_work(gd_ctx *ctx, int64_t rows, int64_t n_pose_total) {
   auto &S = ctx->sur;
   rows = (rows + 63) / 64 * 64 + 64;
-  const int64_t R = 1024;
   if (rows > S.work_rows) {
-    void *qs[] = {S.A1T, S.dH2T, S.dH1T, S.p1, S.p2, S.pb2, S.pw3};
+    void *qs[] = {S.p1, S.p2, S.pb2, S.pw3};
     for (void *p : qs)
       if (p) (void)hipFree(p);
-    HIPCHK(hipMalloc(&S.A1T, (size_t)SUR_H * rows * 2));
-    HIPCHK(hipMalloc(&S.dH2T, (size_t)SUR_H * rows * 2));
-    HIPCHK(hipMalloc(&S.dH1T, (size_t)SUR_H * rows * 2));
-    const int64_t nch = (rows + R - 1) / R;
+    const int64_t nch = (rows + SUR_CHUNK - 1) / SUR_CHUNK;
     HIPCHK(hipMalloc(&S.p1, (size_t)nch * SUR_F * SUR_H * sizeof(float)));
     HIPCHK(hipMalloc(&S.p2, (size_t)nch * SUR_H * SUR_H * sizeof(float)));
     HIPCHK(hipMalloc(&S.pb2, (size_t)nch * SUR_H * sizeof(float)));
-    HIPCHK(hipMalloc(&S.pw3, (size_t)(rows / 16) * (SUR_H + 1) * sizeof(float)));
+    HIPCHK(hipMalloc(&S.pw3, (size_t)nch * 4 * (SUR_H + 1) * sizeof(float)));
+    HIPCHK(hipFuncSetAttribute((const void *)gd_mlp_train_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, SUR_TRAIN_LDS));
     S.work_rows = rows;
   }
   if (S.dS) (void)hipFree(S.dS);
@@ -1657,21 +1657,14 @@ gd_status sur_batch(gd_ctx *ctx, int64_t p0, int64_t p1, bool grad, double *loss
   hipLaunchKernelGGL(gd_mlp_pose_kernel, dim3((np + 255) / 256), dim3(256), 0, ctx->stream, S.y,
                      S.d_off, (int)p0, (int)p1, S.d_target, S.dS, S.sq, S.S);
   if (grad) {
-    hipLaunchKernelGGL(gd_mlp_bwd_kernel, dim3((unsigned)(nr / 64)), dim3(256), 0, ctx->stream, X,
-                       nr, S.pose_of_row + r0, (int)p0, (int)p1, S.dS, W, S.A1T, S.dH2T, S.dH1T,
-                       S.work_rows, S.pw3);
-    const int R = 1024;
-    const int nch = (int)((nr + R - 1) / R);
-    hipLaunchKernelGGL(gd_mlp_gemm_kernel, dim3(nch), dim3(256), 0, ctx->stream,
-                       (const unsigned short *)(S.XT + r0), S.cap + 64, SUR_F,
-                       (const unsigned short *)S.dH1T, S.work_rows, nr, R, S.p1, (float *)nullptr);
-    hipLaunchKernelGGL(gd_mlp_gemm_kernel, dim3(nch), dim3(256), 0, ctx->stream,
-                       (const unsigned short *)S.A1T, S.work_rows, SUR_H,
-                       (const unsigned short *)S.dH2T, S.work_rows, nr, R, S.p2, S.pb2);
+    const int nch = (int)((nr + SUR_CHUNK - 1) / SUR_CHUNK);
+    hipLaunchKernelGGL(gd_mlp_train_kernel, dim3(nch), dim3(256), SUR_TRAIN_LDS, ctx->stream, X,
+                       nr, (int)SUR_CHUNK, S.pose_of_row + r0, (int)p0, (int)p1, S.dS, W, S.p1,
+                       S.p2, S.pb2, S.pw3);
     hipLaunchKernelGGL(gd_mlp_gradsum_kernel, dim3((SUR_NPARAM + 255) / 256), dim3(256), 0,
-                       ctx->stream, S.p1, nch, S.p2, nch, S.pb2, S.pw3, nr / 16, S.grad);
+                       ctx->stream, S.p1, nch, S.p2, nch, S.pb2, S.pw3, (int64_t)nch * 4, S.grad);
     hipLaunchKernelGGL(gd_mlp_w3sum_kernel, dim3(SUR_H + 1), dim3(256), 0, ctx->stream, S.pw3,
-                       (int64_t)(nr / 16), S.grad);
+                       (int64_t)nch * 4, S.grad);
   }
   HIPCHK(hipGetLastError());
   if (loss) {
@@ -1864,23 +1857,15 @@ gd_status gd_sur_dataset_reserve(gd_ctx *ctx, int64_t rows) {
   auto &S = ctx->sur;
   if (rows <= S.cap) return GD_OK;
   HIPCHK(hipSetDevice(ctx->device));
-  unsigned short *X = nullptr, *XT = nullptr;
+  unsigned short *X = nullptr;
   const int64_t cap = (rows + 63) / 64 * 64;
   HIPCHK(hipMalloc(&X, (size_t)(cap + 64) * SUR_F * 2));
-  HIPCHK(hipMalloc(&XT, (size_t)SUR_F * (cap + 64) * 2));
   HIPCHK(hipMemsetAsync(X, 0, (size_t)(cap + 64) * SUR_F * 2, ctx->stream));
-  HIPCHK(hipMemsetAsync(XT, 0, (size_t)SUR_F * (cap + 64) * 2, ctx->stream));
-  if (S.rows > 0) {
+  if (S.rows > 0)
     HIPCHK(hipMemcpyAsync(X, S.X, (size_t)S.rows * SUR_F * 2, hipMemcpyDeviceToDevice, ctx->stream));
-    for (int f = 0; f < SUR_F; ++f)
-      HIPCHK(hipMemcpyAsync(XT + (size_t)f * (cap + 64), S.XT + (size_t)f * (S.cap + 64),
-                            (size_t)S.rows * 2, hipMemcpyDeviceToDevice, ctx->stream));
-  }
   HIPCHK(hipStreamSynchronize(ctx->stream));
   if (S.X) (void)hipFree(S.X);
-  if (S.XT) (void)hipFree(S.XT);
   S.X = X;
-  S.XT = XT;
   S.cap = cap;
   S.dirty = true;
   return GD_OK;
@@ -1910,7 +1895,7 @@ gd_status gd_sur_dataset_add(gd_ctx *ctx, int cid, int n, const int *idx_lig, co
     return s;
   }
   hipLaunchKernelGGL(gd_sur_feat_kernel, dim3((unsigned)((add + 255) / 256)), dim3(256), 0,
-                     ctx->stream, ctx->d_cx, cid, S.g[cid], n, dl, dp, dm, S.rows, S.X, S.XT,
+                     ctx->stream, ctx->d_cx, cid, S.g[cid], n, dl, dp, dm, S.rows, S.X, nullptr,
                      S.cap + 64, (float *)nullptr);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -1961,7 +1946,7 @@ gd_status gd_sur_dataset_add_multi(gd_ctx *ctx, int n, const int *cid, const int
     return s;
   }
   hipLaunchKernelGGL(gd_sur_feat_multi_kernel, dim3((unsigned)n), dim3(64), 0, ctx->stream,
-                     ctx->d_cx, n, dc, dl, dp, dm, dr, S.X, S.XT, S.cap + 64);
+                     ctx->d_cx, n, dc, dl, dp, dm, dr, S.X, nullptr, S.cap + 64);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   free_all(keep);
@@ -2030,34 +2015,57 @@ gd_status gd_sur_train(gd_ctx *ctx, int pose_begin, int pose_end, int epochs, in
   }
   TRY(sur_work(ctx, maxrows, (int64_t)S.target.size()));
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const int n_batch = (pose_end - pose_begin + batch_poses - 1) / batch_poses;
+  // per-batch sums of squared residuals (before each batch's update)
+  float *d_bl = nullptr;
+  HIPCHK(hipMalloc(&d_bl, (size_t)n_batch * sizeof(float)));
+  std::vector<float> bl(n_batch);
   const auto t0 = std::chrono::steady_clock::now();
   S.tab_valid = false;  // the parameters change
-  for (int ep = 0; ep < epochs; ++ep) {
-    for (int p = pose_begin; p < pose_end; p += batch_poses) {
+  gd_status st = GD_OK;
+  for (int ep = 0; ep < epochs && st == GD_OK; ++ep) {
+    int b = 0;
+    for (int p = pose_begin; p < pose_end && st == GD_OK; p += batch_poses, ++b) {
       const int q = std::min(pose_end, p + batch_poses);
-      TRY(sur_batch(ctx, p, q, true, nullptr));
+      st = sur_batch(ctx, p, q, true, nullptr);
+      if (st != GD_OK) break;
+      hipLaunchKernelGGL(gd_mlp_sqsum_kernel, dim3(1), dim3(256), 0, ctx->stream, S.sq + p,
+                         q - p, d_bl + b);
       S.adam_t += 1;
       const float c1 = 1.0f - std::pow(b1, (float)S.adam_t);
       const float c2 = 1.0f - std::pow(b2, (float)S.adam_t);
       hipLaunchKernelGGL(gd_mlp_adam_kernel, dim3((SUR_NPARAM + 255) / 256), dim3(256), 0,
                          ctx->stream, S.p, S.m, S.v, S.grad, lr, b1, b2, eps, c1, c2, S.W1T, S.W2T,
                          S.W2R);
-      HIPCHK(hipGetLastError());
+      if (hipGetLastError() != hipSuccess) st = fail(ctx, GD_ERR_HIP, "training launch failed");
     }
-    if (epoch_loss) {
-      // loss over the whole range with the epoch's final parameters
+    if (st != GD_OK || !epoch_loss) continue;
+    if (ep + 1 < epochs) {
+      // the epoch's batch losses, each taken before its batch's update
+      if (hipMemcpyAsync(bl.data(), d_bl, (size_t)n_batch * sizeof(float),
+                         hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        st = fail(ctx, GD_ERR_HIP, "batch loss copy failed");
+        break;
+      }
       double tot = 0.0;
-      int64_t cnt = 0;
-      for (int p = pose_begin; p < pose_end; p += batch_poses) {
+      for (float v : bl) tot += v;
+      epoch_loss[ep] = tot / (double)(pose_end - pose_begin);
+    } else {
+      // last epoch: the loss over the whole range with the final parameters
+      double tot = 0.0;
+      for (int p = pose_begin; p < pose_end && st == GD_OK; p += batch_poses) {
         const int q = std::min(pose_end, p + batch_poses);
         double l = 0.0;
-        TRY(sur_batch(ctx, p, q, false, &l));
+        st = sur_batch(ctx, p, q, false, &l);
         tot += l * (q - p);
-        cnt += q - p;
       }
-      epoch_loss[ep] = tot / (double)cnt;
+      epoch_loss[ep] = tot / (double)(pose_end - pose_begin);
     }
   }
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d_bl);
+  TRY(st);
   HIPCHK(hipStreamSynchronize(ctx->stream));
   if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return GD_OK;

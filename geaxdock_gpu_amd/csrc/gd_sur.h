@@ -9,11 +9,11 @@
 // carries the first-layer bias. MLP 56 -> 128 -> 128 -> 1 with SiLU; a pose's
 // value is the sum over its atoms. Rows are padded to 64 columns (bf16).
 //
-// MFMA: v_mfma_f32_16x16x16_bf16 on wave64. Fragment layout used throughout
-// (lane l, i = 0..3):
-//   A[16x16]: A[l % 16][4 (l / 16) + i]      (row-major run of 4)
-//   B[16x16]: B[4 (l / 16) + i][l % 16]      (column-major run of 4)
-//   C[16x16]: C[4 (l / 16) + i][l % 16]
+// MFMA: v_mfma_f32_16x16x32_bf16 (gfx950) on wave64, C[16x16] += A[16x32]
+// B[32x16]. Fragment layout used throughout (lane l, i = 0..7, j = 0..3):
+//   A[16x32]: A[l % 16][8 (l / 16) + i]      (row-major run of 8)
+//   B[32x16]: B[8 (l / 16) + i][l % 16]      (column-major run of 8)
+//   C[16x16]: C[4 (l / 16) + j][l % 16]
 #pragma once
 
 #include "gd_device.h"
@@ -35,7 +35,7 @@ constexpr int SP_W3 = SP_B2 + SUR_H;
 constexpr int SP_B3 = SP_W3 + SUR_H;
 constexpr int SUR_NPARAM = SP_B3 + 1;
 
-typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4_t;
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
 
 __host__ __device__ inline unsigned short f2bf(float f) {
@@ -52,11 +52,12 @@ __device__ inline float dsilu(float x) {
   return s * (1.0f + x * (1.0f - s));
 }
 
-__device__ inline f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+__device__ inline f32x4_t mfma32(bf16x8_t a, bf16x8_t b, f32x4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
-__device__ inline bf16x4_t ld4(const unsigned short *p) {
-  return *reinterpret_cast<const bf16x4_t *>(p);
+// eight bf16 at p (16-byte aligned)
+__device__ inline bf16x8_t ld8(const unsigned short *p) {
+  return *reinterpret_cast<const bf16x8_t *>(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -171,15 +172,15 @@ __device__ inline void mlp_fwd16(const unsigned short *X, int64_t r0, const SurW
                                  unsigned short *a1s, f32x4_t (&h1)[8], f32x4_t (&h2)[8]) {
   const int l = threadIdx.x & 63;
   const int lr = l & 15, lq = l >> 4;
-  bf16x4_t xa[4];
+  bf16x8_t xa[2];
 #pragma unroll
-  for (int ks = 0; ks < 4; ++ks) xa[ks] = ld4(X + (r0 + lr) * SUR_F + 16 * ks + 4 * lq);
+  for (int ks = 0; ks < 2; ++ks) xa[ks] = ld8(X + (r0 + lr) * SUR_F + 32 * ks + 8 * lq);
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
     f32x4_t c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
-      c = mfma16(xa[ks], ld4(W.W1T + (16 * t + lr) * SUR_F + 16 * ks + 4 * lq), c);
+    for (int ks = 0; ks < 2; ++ks)
+      c = mfma32(xa[ks], ld8(W.W1T + (16 * t + lr) * SUR_F + 32 * ks + 8 * lq), c);
     h1[t] = c;
 #pragma unroll
     for (int i = 0; i < 4; ++i) a1s[(4 * lq + i) * SUR_H + 16 * t + lr] = f2bf(silu(c[i]));
@@ -190,9 +191,9 @@ __device__ inline void mlp_fwd16(const unsigned short *X, int64_t r0, const SurW
   for (int u = 0; u < 8; ++u) {
     f32x4_t c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks)
-      c = mfma16(ld4(a1s + lr * SUR_H + 16 * ks + 4 * lq),
-                 ld4(W.W2T + (16 * u + lr) * SUR_H + 16 * ks + 4 * lq), c);
+    for (int ks = 0; ks < 4; ++ks)
+      c = mfma32(ld8(a1s + lr * SUR_H + 32 * ks + 8 * lq),
+                 ld8(W.W2T + (16 * u + lr) * SUR_H + 32 * ks + 8 * lq), c);
     const float bb = W.b2[16 * u + lr];
 #pragma unroll
     for (int i = 0; i < 4; ++i) c[i] += bb;

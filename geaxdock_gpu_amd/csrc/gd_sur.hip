@@ -65,7 +65,7 @@ extern "C" __global__ void gd_sur_feat_multi_kernel(const DevComplex *cxs, int n
 extern "C" __global__ void __launch_bounds__(256) gd_mlp_fwd_kernel(const unsigned short *X,
                                                                     int64_t n_rows, SurW W,
                                                                     float *y) {
-  __shared__ unsigned short a1s[4][16 * SUR_H];
+  __shared__ __attribute__((aligned(16))) unsigned short a1s[4][16 * SUR_H];
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int64_t r0 = ((int64_t)blockIdx.x * 4 + wv) * 16;
   if (r0 >= n_rows) return;  // whole wave leaves; no block barrier below
@@ -87,8 +87,8 @@ extern "C" __global__ void __launch_bounds__(256) gd_sur_tab_kernel(const float 
                                                                     const int *types,
                                                                     int n_types, SurW W,
                                                                     float *Y) {
-  __shared__ unsigned short xs[4][16 * SUR_F];
-  __shared__ unsigned short a1s[4][16 * SUR_H];
+  __shared__ __attribute__((aligned(16))) unsigned short xs[4][16 * SUR_F];
+  __shared__ __attribute__((aligned(16))) unsigned short a1s[4][16 * SUR_H];
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int tiles = (n_pts + 15) / 16;
   const int64_t job = (int64_t)blockIdx.x * 4 + wv;
@@ -142,134 +142,205 @@ extern "C" __global__ void gd_mlp_pose_kernel(const float *y, const int64_t *off
   if (S_out) S_out[p] = s;
 }
 
-// Backward of one batch: recomputes the forward, then per wave
-//   dH2 = dy w3^T * silu'(H2);  dA1 = dH2 W2^T;  dH1 = dA1 * silu'(H1)
-// and writes A1T[128][ld], dH2T[128][ld], dH1T[128][ld] (bf16, column per
-// row) for the weight-gradient GEMMs, plus per-wave partials of grad w3, b3.
-extern "C" __global__ void __launch_bounds__(256) gd_mlp_bwd_kernel(
-    const unsigned short *X, int64_t n_rows, const int *pose_of_row, int p0, int p1,
-    const float *dS, SurW W, unsigned short *A1T, unsigned short *dH2T, unsigned short *dH1T,
-    int64_t ld, float *gw3_part) {
-  __shared__ unsigned short tile[4][16 * SUR_H];
+// Row stride (bf16) of the transposed LDS tiles of gd_mlp_train_kernel: 64
+// rows and 8 of padding, 144 bytes (a multiple of 16 for 16-byte reads).
+constexpr int SUR_TR = 72;
+constexpr int SUR_STEP = 64;  // rows per step of gd_mlp_train_kernel (4 waves x 16)
+
+// Backward of one batch fused with its weight gradients. Workgroup b owns
+// rows [b R, (b + 1) R) (R a multiple of 64) and walks them 64 at a time;
+// wave w recomputes the forward of rows 16 w .. 16 w + 15 of the step, forms
+//   dy = dS of the row's pose,  dH2 = dy w3 * silu'(H2),
+//   dA1 = dH2 W2^T,             dH1 = dA1 * silu'(H1)
+// and writes X, A1 = silu(H1), dH2 and dH1 of its rows transposed into LDS.
+// After a workgroup barrier wave w accumulates its share of the step's
+//   dW2 += A1^T dH2   (row tiles 2w and 2w + 1 of 8),
+//   dW1 += X^T dH1    (row tile w of 4; row 56 is the b1 gradient)
+// in MFMA accumulators. Nothing but the partials goes back to HBM, in a fixed
+// order (gd_mlp_gradsum_kernel and gd_mlp_w3sum_kernel sum them, so training
+// is deterministic): p1g[b][64][128], p2g[b][128][128], pb2[b][128] and
+// pw3[4 b + w][129] (w3, then b3).
+extern "C" __global__ void __launch_bounds__(256) gd_mlp_train_kernel(
+    const unsigned short *X, int64_t n_rows, int R, const int *pose_of_row, int p0, int p1,
+    const float *dS, SurW W, float *p1g, float *p2g, float *pb2, float *pw3) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char sm_train[];
+  unsigned short *a1t = reinterpret_cast<unsigned short *>(sm_train);  // A1^T [128][SUR_TR]
+  unsigned short *h2t = a1t + SUR_H * SUR_TR;                          // dH2^T
+  unsigned short *h1t = h2t + SUR_H * SUR_TR;                          // dH1^T
+  unsigned short *xt = h1t + SUR_H * SUR_TR;                           // X^T [64][SUR_TR]
+  unsigned short *rows = xt + SUR_F * SUR_TR;                          // [4][16][128] row-major
+  float *red = reinterpret_cast<float *>(rows + 4 * 16 * SUR_H);      // [4][128]
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int lr = l & 15, lq = l >> 4;
-  const int64_t wave = (int64_t)blockIdx.x * 4 + wv;
-  const int64_t r0 = wave * 16;
-  if (r0 >= n_rows) return;
-  unsigned short *ts = tile[wv];
-  f32x4_t h1[8], h2[8];
-  mlp_fwd16(X, r0, W, ts, h1, h2);
-  // A1^T to global (rows r0 + 4 lq + i of column 16 t + lr)
+  const int64_t base = (int64_t)blockIdx.x * R;
+  const int64_t end = base + R < n_rows ? base + R : n_rows;
+  unsigned short *ts = rows + wv * 16 * SUR_H;
+  f32x4_t g2[2][8], g1[8];
 #pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    unsigned short v[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) v[i] = f2bf(silu(h1[t][i]));
-    *reinterpret_cast<uint2 *>(A1T + (int64_t)(16 * t + lr) * ld + r0 + 4 * lq) =
-        make_uint2(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16));
+  for (int n = 0; n < 8; ++n) {
+    g2[0][n] = g2[1][n] = g1[n] = f32x4_t{0.f, 0.f, 0.f, 0.f};
   }
-  float dy[4];
+  float gw3[8], gb2[8], gb3 = 0.0f;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int p = pose_of_row[r0 + 4 * lq + i];
-    dy[i] = (p >= p0 && p < p1) ? dS[p] : 0.0f;
-  }
-  // grad w3 (column 16 u + lr) and b3
-  float gw[8];
+  for (int u = 0; u < 8; ++u) gw3[u] = gb2[u] = 0.0f;
+
+  for (int64_t s0 = base; s0 < end; s0 += SUR_STEP) {
+    const int64_t r0 = s0 + 16 * wv;
+    const int cw = 16 * wv;  // this wave's columns in the transposed tiles
+    f32x4_t h1[8], h2[8];
+    mlp_fwd16(X, r0, W, ts, h1, h2);
+    // X^T: lane (lr, lq) moves columns 16 lq .. 16 lq + 15 of row lr
+    {
+      const unsigned short *xr = X + (r0 + lr) * SUR_F + 16 * lq;
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    float s = 0.0f;
+      for (int q = 0; q < 16; ++q) xt[(16 * lq + q) * SUR_TR + cw + lr] = xr[q];
+    }
+    // A1^T from the C layout (rows 4 lq + i, column 16 t + lr)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) s = fmaf(silu(h2[u][i]), dy[i], s);
-    s += __shfl_xor(s, 16);
-    s += __shfl_xor(s, 32);
-    gw[u] = s;
-  }
-  float gb = dy[0] + dy[1] + dy[2] + dy[3];
-  gb += __shfl_xor(gb, 16);
-  gb += __shfl_xor(gb, 32);
-  if (lq == 0) {
+    for (int t = 0; t < 8; ++t) {
+      unsigned short v[4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) gw3_part[wave * (SUR_H + 1) + 16 * u + lr] = gw[u];
-    if (lr == 0) gw3_part[wave * (SUR_H + 1) + SUR_H] = gb;
-  }
-  // dH2 into the wave tile (row-major, for dA1) and to global (transposed)
-  __builtin_amdgcn_wave_barrier();
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const float w = W.w3[16 * u + lr];
-    unsigned short v[4];
+      for (int i = 0; i < 4; ++i) v[i] = f2bf(silu(h1[t][i]));
+      *reinterpret_cast<uint2 *>(a1t + (16 * t + lr) * SUR_TR + cw + 4 * lq) =
+          make_uint2(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16));
+    }
+    float dy[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      v[i] = f2bf(dy[i] * w * dsilu(h2[u][i]));
-      ts[(4 * lq + i) * SUR_H + 16 * u + lr] = v[i];
+      const int p = pose_of_row[r0 + 4 * lq + i];
+      dy[i] = (p >= p0 && p < p1) ? dS[p] : 0.0f;
     }
-    *reinterpret_cast<uint2 *>(dH2T + (int64_t)(16 * u + lr) * ld + r0 + 4 * lq) =
-        make_uint2(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16));
+    // w3 and b3 (rows counted once: b3 from the lr == 0 lanes)
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) gw3[u] = fmaf(silu(h2[u][i]), dy[i], gw3[u]);
+    if (lr == 0) gb3 += (dy[0] + dy[1]) + (dy[2] + dy[3]);
+    // dH2: row-major into the wave's tile (A operand of dA1) and transposed
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // the forward's LDS reads of ts are done
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const float w = W.w3[16 * u + lr];
+      unsigned short v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float g = dy[i] * w * dsilu(h2[u][i]);
+        gb2[u] += g;
+        v[i] = f2bf(g);
+        ts[(4 * lq + i) * SUR_H + 16 * u + lr] = v[i];
+      }
+      *reinterpret_cast<uint2 *>(h2t + (16 * u + lr) * SUR_TR + cw + 4 * lq) =
+          make_uint2(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16));
+    }
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+    // dA1 = dH2 W2^T (B[k][n] = W2[n][k] = W2R[n][k]), dH1 transposed
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      f32x4_t c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks)
+        c = mfma32(ld8(ts + lr * SUR_H + 32 * ks + 8 * lq),
+                   ld8(W.W2R + (16 * t + lr) * SUR_H + 32 * ks + 8 * lq), c);
+      unsigned short v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = f2bf(c[i] * dsilu(h1[t][i]));
+      *reinterpret_cast<uint2 *>(h1t + (16 * t + lr) * SUR_TR + cw + 4 * lq) =
+          make_uint2(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16));
+    }
+    __syncthreads();
+    // weight gradients over the step's 64 rows (two K blocks of 32)
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+      const int ko = 32 * kb + 8 * lq;
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const bf16x8_t A = ld8(a1t + (16 * (2 * wv + a) + lr) * SUR_TR + ko);
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+          g2[a][n] = mfma32(A, ld8(h2t + (16 * n + lr) * SUR_TR + ko), g2[a][n]);
+      }
+      const bf16x8_t A = ld8(xt + (16 * wv + lr) * SUR_TR + ko);
+#pragma unroll
+      for (int n = 0; n < 8; ++n) g1[n] = mfma32(A, ld8(h1t + (16 * n + lr) * SUR_TR + ko), g1[n]);
+    }
+    __syncthreads();  // the tiles are rewritten by the next step
   }
-  __builtin_amdgcn_s_waitcnt(0xc07f);
-  __builtin_amdgcn_wave_barrier();
-  // dA1 = dH2 W2^T: B[j][i] = W2[i][j]
+
+  // partials of this workgroup
+  float *o2 = p2g + (int64_t)blockIdx.x * SUR_H * SUR_H;
+  float *o1 = p1g + (int64_t)blockIdx.x * SUR_F * SUR_H;
 #pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    f32x4_t c = {0.f, 0.f, 0.f, 0.f};
+  for (int n = 0; n < 8; ++n)
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks)
-      c = mfma16(ld4(ts + lr * SUR_H + 16 * ks + 4 * lq),
-                 ld4(W.W2R + (16 * t + lr) * SUR_H + 16 * ks + 4 * lq), c);
-    unsigned short v[4];
+    for (int i = 0; i < 4; ++i) {
+      o2[(16 * (2 * wv) + 4 * lq + i) * SUR_H + 16 * n + lr] = g2[0][n][i];
+      o2[(16 * (2 * wv + 1) + 4 * lq + i) * SUR_H + 16 * n + lr] = g2[1][n][i];
+      o1[(16 * wv + 4 * lq + i) * SUR_H + 16 * n + lr] = g1[n][i];
+    }
+  // w3, b3: sum over the lane quarters (rows), one partial row per wave
 #pragma unroll
-    for (int i = 0; i < 4; ++i) v[i] = f2bf(c[i] * dsilu(h1[t][i]));
-    *reinterpret_cast<uint2 *>(dH1T + (int64_t)(16 * t + lr) * ld + r0 + 4 * lq) =
-        make_uint2(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16));
+  for (int u = 0; u < 8; ++u) {
+    float v = gw3[u];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    gw3[u] = v;
+    float b = gb2[u];
+    b += __shfl_xor(b, 16);
+    b += __shfl_xor(b, 32);
+    gb2[u] = b;
+  }
+  gb3 += __shfl_xor(gb3, 16);
+  gb3 += __shfl_xor(gb3, 32);
+  float *w3o = pw3 + ((int64_t)blockIdx.x * 4 + wv) * (SUR_H + 1);
+  if (lq == 0) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      w3o[16 * u + lr] = gw3[u];
+      red[wv * SUR_H + 16 * u + lr] = gb2[u];
+    }
+    if (lr == 0) w3o[SUR_H] = gb3;
+  }
+  __syncthreads();
+  // b2: the four waves' column sums in a fixed order
+  if (threadIdx.x < SUR_H) {
+    const int j = threadIdx.x;
+    pb2[(int64_t)blockIdx.x * SUR_H + j] =
+        (red[j] + red[SUR_H + j]) + (red[2 * SUR_H + j] + red[3 * SUR_H + j]);
   }
 }
 
-// Weight gradient partials G[M][128] = P^T Q over rows [c*R, (c+1)*R):
-// P^T given as PT[M][ld] (M = 64 or 128), Q^T as QT[128][ld]. One workgroup
-// (4 waves) per chunk; wave w owns output rows [16*mt, ...) for
-// mt = w, w + 4, .... Column sums of Q (bias gradient) when qsum != null.
-extern "C" __global__ void __launch_bounds__(256) gd_mlp_gemm_kernel(
-    const unsigned short *PT, int64_t ldp, int M, const unsigned short *QT, int64_t ldq,
-    int64_t n_rows, int R, float *part, float *qsum) {
-  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int lr = l & 15, lq = l >> 4;
-  const int64_t b = (int64_t)blockIdx.x * R;
-  const int64_t e = b + R < n_rows ? b + R : n_rows;
-  const int mtiles = M / 16;
-  float *out = part + (int64_t)blockIdx.x * M * SUR_H;
-  for (int mt = wv; mt < mtiles; mt += 4) {
-    f32x4_t c[8];
-#pragma unroll
-    for (int n = 0; n < 8; ++n) c[n] = {0.f, 0.f, 0.f, 0.f};
-    float cs[8];
-#pragma unroll
-    for (int n = 0; n < 8; ++n) cs[n] = 0.0f;
-    for (int64_t k0 = b; k0 < e; k0 += 16) {
-      const bf16x4_t a = ld4(PT + (int64_t)(16 * mt + lr) * ldp + k0 + 4 * lq);
-#pragma unroll
-      for (int n = 0; n < 8; ++n) {
-        const bf16x4_t q = ld4(QT + (int64_t)(16 * n + lr) * ldq + k0 + 4 * lq);
-        c[n] = mfma16(a, q, c[n]);
-        if (qsum && mt == 0)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) cs[n] += bf2f((unsigned short)q[i]);
-      }
-    }
-#pragma unroll
-    for (int n = 0; n < 8; ++n)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) out[(16 * mt + 4 * lq + i) * SUR_H + 16 * n + lr] = c[n][i];
-    if (qsum && mt == 0) {
-#pragma unroll
-      for (int n = 0; n < 8; ++n) {
-        float s = cs[n];
-        s += __shfl_xor(s, 16);
-        s += __shfl_xor(s, 32);
-        if (lq == 0) qsum[(int64_t)blockIdx.x * SUR_H + 16 * n + lr] = s;
-      }
-    }
+// Dynamic LDS bytes of gd_mlp_train_kernel.
+constexpr int SUR_TRAIN_LDS = (3 * SUR_H * SUR_TR + SUR_F * SUR_TR + 4 * 16 * SUR_H) * 2 +
+                              4 * SUR_H * 4;
+
+// Sum of the squared residuals of poses [0, n) into *out: one workgroup,
+// strided partial sums then a fixed-shape tree (deterministic).
+extern "C" __global__ void __launch_bounds__(256) gd_mlp_sqsum_kernel(const float *sq, int n,
+                                                                     float *out) {
+  __shared__ float red[256];
+  float s = 0.0f;
+  for (int i = threadIdx.x; i < n; i += 256) s += sq[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// Sum of n partials part[c * stride + q], c = 0..n-1, in a fixed order:
+// eight interleaved accumulators (eight loads in flight) then a fixed tree.
+__device__ inline float sum_partials(const float *part, int n, int64_t stride, int q) {
+  float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int c = 0;
+  for (; c + 8 <= n; c += 8)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] += part[(int64_t)(c + j) * stride + q];
+  for (int j = 0; c < n; ++c, ++j) a[j] += part[(int64_t)c * stride + q];
+  return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
 }
 
 // Sums gradient partials into g[SUR_NPARAM] (fixed order: deterministic).
@@ -280,11 +351,11 @@ extern "C" __global__ void gd_mlp_gradsum_kernel(const float *p1, int n1, const 
   if (q >= SUR_NPARAM) return;
   float s = 0.0f;
   if (q < SP_W2) {
-    for (int c = 0; c < n1; ++c) s += p1[(int64_t)c * SUR_F * SUR_H + q];
+    s = sum_partials(p1, n1, (int64_t)SUR_F * SUR_H, q);
   } else if (q < SP_B2) {
-    for (int c = 0; c < n2; ++c) s += p2[(int64_t)c * SUR_H * SUR_H + (q - SP_W2)];
+    s = sum_partials(p2, n2, (int64_t)SUR_H * SUR_H, q - SP_W2);
   } else if (q < SP_W3) {
-    for (int c = 0; c < n2; ++c) s += pb2[(int64_t)c * SUR_H + (q - SP_B2)];
+    s = sum_partials(pb2, n2, SUR_H, q - SP_B2);
   } else {
     return;  // w3 and b3: gd_mlp_w3sum_kernel
   }

@@ -338,7 +338,10 @@ gd_status gd_sur_tabulate(gd_ctx *ctx);
 gd_status gd_sur_surrogate(gd_ctx *ctx, int complex_id, int n, const int *idx_lig,
                            const int *idx_prt, const float *movematrix, float sur_scale,
                            float *out);
-/* Adam over contiguous batches of batch_poses poses; epoch_loss[epochs]. */
+/* Adam over contiguous batches of batch_poses poses. epoch_loss[epochs]
+ * (optional): for each epoch the mean squared residual of its batches, each
+ * taken before its batch's update; the last entry is the loss over the whole
+ * range with the final parameters. */
 gd_status gd_sur_train(gd_ctx *ctx, int pose_begin, int pose_end, int epochs, int batch_poses,
                        float lr, double *epoch_loss, double *seconds);
 
