@@ -51,6 +51,9 @@ sys.path.insert(0, ROOT)
 FLOPS_PER_PAIR = 36.0
 PEAK_FP32 = 157.3e12
 PEAK_FP32_UNPACKED = 78.6e12
+PEAK_BF16 = 2.5e15
+SUR_FWD_FLOPS_PER_ROW = 2.0 * (64 * 128 + 128 * 128 + 128)
+SUR_TRAIN_FLOPS_PER_ROW = SUR_FWD_FLOPS_PER_ROW + 2.0 * (128 * 128 + 128 * 128 + 64 * 128)
 METRIC = "MC replica-steps/s, plain vs surrogate-DA, at matched best-pose energy"
 
 
@@ -387,6 +390,12 @@ def da_compare(args, cxs, desc):
     fit = sg.fit(eng, ts, sigma, n_train=int(0.9 * len(ts["t"])), epochs=args.epochs,
                  batch_poses=args.batch_poses, lr=args.lr)
     progress(f"trained: held-out RMSE {fit['test_rmse_energy']:.4f} ({fit['seconds']:.1f} s)")
+    # BF16 MFMA work of the training: per atom row and epoch the forward
+    # (2 (64 x 128 + 128 x 128 + 128) flops) and backward (dA1 2 x 128 x 128,
+    # dW2 2 x 128 x 128, dW1 2 x 64 x 128), plus the last epoch's evaluation
+    # of the training range
+    rows_train = float(ts["lna"][:fit["n_train"]].sum())
+    train_flops = rows_train * (args.epochs * SUR_TRAIN_FLOPS_PER_ROW + SUR_FWD_FLOPS_PER_ROW)
     params = eng.sur_get_params()
     blob = eng.checkpoint()
     eng.close()
@@ -506,7 +515,17 @@ def da_compare(args, cxs, desc):
                       "epochs": args.epochs, "train_seconds": fit["seconds"],
                       "train_loss": fit["train_loss"], "test_rmse_std": fit["test_rmse_std"],
                       "test_rmse_energy": fit["test_rmse_energy"],
-                      "test_energy_std": fit["test_energy_std"]},
+                      "test_energy_std": fit["test_energy_std"],
+                      "train_rows_per_epoch": rows_train,
+                      "mfma": {"bound": "mfma", "unit": "TFLOP/s",
+                               "achieved": train_flops / fit["seconds"] / 1e12,
+                               "peak": PEAK_BF16 / 1e12,
+                               "frac": train_flops / fit["seconds"] / PEAK_BF16,
+                               "note": "BF16 MFMA work of the whole training call (forward "
+                                       "49,408 + backward 81,920 flops per atom row and epoch, "
+                                       "plus one evaluation pass) over its wall time, all "
+                                       "kernels and launches included; peak: MI355X dense "
+                                       "BF16 2.5 PFLOP/s"}},
     }
 
 

@@ -118,6 +118,7 @@ struct gd_ctx {
     float *y = nullptr, *dS = nullptr, *sq = nullptr, *S = nullptr;
     float *p1 = nullptr, *p2 = nullptr, *pb2 = nullptr, *pw3 = nullptr;
     int64_t work_rows = 0;
+    int last_groups = 0;  // backward workgroups (partials) of the last batch
   } sur;
 };
 
@@ -1613,8 +1614,14 @@ gd_status sur_upload_index(gd_ctx *ctx) {
   return GD_OK;
 }
 
-// Rows per workgroup of the fused backward (gd_mlp_train_kernel).
-constexpr int64_t SUR_CHUNK = 512;
+// Workgroups of the batch kernels: the fused backward runs one per CU (its
+// LDS tiles take 106 KB), the forward two; each walks 64-row steps.
+int sur_train_groups(const gd_ctx *ctx, int64_t rows) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(rows / 64, ctx->n_cu));
+}
+int sur_fwd_groups(const gd_ctx *ctx, int64_t rows) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(rows / 64, 2 * (int64_t)ctx->n_cu));
+}
 
 // work buffers for batches of up to `rows` rows (rounded up to 64): the
 // gradient partials of the fused backward's workgroups and per-pose outputs
@@ -1625,11 +1632,11 @@ gd_status sur_work(gd_ctx *ctx, int64_t rows, int64_t n_pose_total) {
     void *qs[] = {S.p1, S.p2, S.pb2, S.pw3};
     for (void *p : qs)
       if (p) (void)hipFree(p);
-    const int64_t nch = (rows + SUR_CHUNK - 1) / SUR_CHUNK;
-    HIPCHK(hipMalloc(&S.p1, (size_t)nch * SUR_F * SUR_H * sizeof(float)));
-    HIPCHK(hipMalloc(&S.p2, (size_t)nch * SUR_H * SUR_H * sizeof(float)));
-    HIPCHK(hipMalloc(&S.pb2, (size_t)nch * SUR_H * sizeof(float)));
-    HIPCHK(hipMalloc(&S.pw3, (size_t)nch * 4 * (SUR_H + 1) * sizeof(float)));
+    const int64_t ng = std::max(1, ctx->n_cu);
+    HIPCHK(hipMalloc(&S.p1, (size_t)ng * SUR_F * SUR_H * sizeof(float)));
+    HIPCHK(hipMalloc(&S.p2, (size_t)ng * SUR_H * SUR_H * sizeof(float)));
+    HIPCHK(hipMalloc(&S.pb2, (size_t)ng * SUR_H * sizeof(float)));
+    HIPCHK(hipMalloc(&S.pw3, (size_t)ng * (SUR_H + 1) * sizeof(float)));
     HIPCHK(hipFuncSetAttribute((const void *)gd_mlp_train_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, SUR_TRAIN_LDS));
     S.work_rows = rows;
@@ -1645,26 +1652,29 @@ gd_status sur_work(gd_ctx *ctx, int64_t rows, int64_t n_pose_total) {
 }
 
 // forward (+ backward and gradient when grad) over poses [p0, p1) of the dataset
-gd_status sur_batch(gd_ctx *ctx, int64_t p0, int64_t p1, bool grad, double *loss) {
+// With `update`, the gradient partials are left for gd_mlp_update_kernel
+// (the training step sums them and applies Adam in one launch).
+gd_status sur_batch(gd_ctx *ctx, int64_t p0, int64_t p1, bool grad, double *loss,
+                    bool update = false) {
   auto &S = ctx->sur;
   const int64_t r0 = S.off[p0], r1 = S.off[p1];
   const int64_t nr = (r1 - r0 + 63) / 64 * 64;  // padded rows read from the dataset
   const int np = (int)(p1 - p0);
   const SurW W = sur_weights(ctx);
   const unsigned short *X = S.X + r0 * SUR_F;
-  hipLaunchKernelGGL(gd_mlp_fwd_kernel, dim3((unsigned)(nr / 64)), dim3(256), 0, ctx->stream, X,
+  hipLaunchKernelGGL(gd_mlp_fwd_kernel, dim3(sur_fwd_groups(ctx, nr)), dim3(256), 0, ctx->stream, X,
                      nr, W, S.y + r0);
   hipLaunchKernelGGL(gd_mlp_pose_kernel, dim3((np + 255) / 256), dim3(256), 0, ctx->stream, S.y,
                      S.d_off, (int)p0, (int)p1, S.d_target, S.dS, S.sq, S.S);
   if (grad) {
-    const int nch = (int)((nr + SUR_CHUNK - 1) / SUR_CHUNK);
-    hipLaunchKernelGGL(gd_mlp_train_kernel, dim3(nch), dim3(256), SUR_TRAIN_LDS, ctx->stream, X,
-                       nr, (int)SUR_CHUNK, S.pose_of_row + r0, (int)p0, (int)p1, S.dS, W, S.p1,
-                       S.p2, S.pb2, S.pw3);
-    hipLaunchKernelGGL(gd_mlp_gradsum_kernel, dim3((SUR_NPARAM + 255) / 256), dim3(256), 0,
-                       ctx->stream, S.p1, nch, S.p2, nch, S.pb2, S.pw3, (int64_t)nch * 4, S.grad);
-    hipLaunchKernelGGL(gd_mlp_w3sum_kernel, dim3(SUR_H + 1), dim3(256), 0, ctx->stream, S.pw3,
-                       (int64_t)nch * 4, S.grad);
+    const int ng = sur_train_groups(ctx, nr);
+    hipLaunchKernelGGL(gd_mlp_train_kernel, dim3(ng), dim3(256), SUR_TRAIN_LDS, ctx->stream, X,
+                       nr, S.pose_of_row + r0, (int)p0, (int)p1, S.dS, W, S.p1, S.p2, S.pb2,
+                       S.pw3);
+    S.last_groups = ng;
+    if (!update)
+      hipLaunchKernelGGL(gd_mlp_gradsum_kernel, dim3((SUR_NPARAM + 255) / 256), dim3(256), 0,
+                         ctx->stream, S.p1, ng, S.p2, ng, S.pb2, S.pw3, (int64_t)ng, S.grad);
   }
   HIPCHK(hipGetLastError());
   if (loss) {
@@ -2027,16 +2037,16 @@ gd_status gd_sur_train(gd_ctx *ctx, int pose_begin, int pose_end, int epochs, in
     int b = 0;
     for (int p = pose_begin; p < pose_end && st == GD_OK; p += batch_poses, ++b) {
       const int q = std::min(pose_end, p + batch_poses);
-      st = sur_batch(ctx, p, q, true, nullptr);
+      st = sur_batch(ctx, p, q, true, nullptr, true);
       if (st != GD_OK) break;
       hipLaunchKernelGGL(gd_mlp_sqsum_kernel, dim3(1), dim3(256), 0, ctx->stream, S.sq + p,
                          q - p, d_bl + b);
       S.adam_t += 1;
       const float c1 = 1.0f - std::pow(b1, (float)S.adam_t);
       const float c2 = 1.0f - std::pow(b2, (float)S.adam_t);
-      hipLaunchKernelGGL(gd_mlp_adam_kernel, dim3((SUR_NPARAM + 255) / 256), dim3(256), 0,
-                         ctx->stream, S.p, S.m, S.v, S.grad, lr, b1, b2, eps, c1, c2, S.W1T, S.W2T,
-                         S.W2R);
+      hipLaunchKernelGGL(gd_mlp_update_kernel, dim3((SUR_NPARAM + 255) / 256), dim3(256), 0,
+                         ctx->stream, S.p1, S.p2, S.pb2, S.pw3, S.last_groups, S.p, S.m, S.v,
+                         S.grad, lr, b1, b2, eps, c1, c2, S.W1T, S.W2T, S.W2R);
       if (hipGetLastError() != hipSuccess) st = fail(ctx, GD_ERR_HIP, "training launch failed");
     }
     if (st != GD_OK || !epoch_loss) continue;

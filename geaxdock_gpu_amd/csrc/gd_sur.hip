@@ -61,21 +61,174 @@ extern "C" __global__ void gd_sur_feat_multi_kernel(const DevComplex *cxs, int n
              ld, nullptr);
 }
 
-// Per-row outputs of a batch: 4 waves x 16 rows per workgroup.
+// Row strides (bf16) of the LDS tiles of the batch kernels: 64 + 8 and
+// 128 + 8 (multiples of 8, so 16-byte reads stay aligned; the padding
+// spreads a fragment's rows over the LDS banks).
+constexpr int SUR_XS = SUR_F + 8;
+constexpr int SUR_AS = SUR_H + 8;
+constexpr int SUR_TR = 72;    // transposed tiles: 64 rows + 8
+constexpr int SUR_STEP = 64;  // rows per step: 4 row tiles of 16
+
+// Column-split MLP of the batch kernels. A workgroup of 4 waves takes 64
+// rows at a time; wave w owns columns [32 w, 32 w + 32) of both 128-wide
+// layers for all 64 rows, so its slices of W1, W2 and W2^T stay in
+// registers for the whole kernel (B fragments: col tile c, K block kb), and
+// the rows' X and A1 tiles are shared through LDS.
+struct ColW {
+  bf16x8_t w1[2][2];   // W1[:, 32w + 16c ..]: K = 64
+  bf16x8_t w2[2][4];   // W2[:, 32w + 16c ..]: K = 128
+  float b2[2], w3[2];  // per lane: column 32w + 16c + l % 16
+};
+
+__device__ inline void load_colw(const SurW &W, int wv, bool with_w2r, ColW &cw,
+                                 bf16x8_t (&w2r)[2][4]) {
+  const int l = threadIdx.x & 63, lr = l & 15, lq = l >> 4;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int n = 32 * wv + 16 * c + lr;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) cw.w1[c][kb] = ld8(W.W1T + n * SUR_F + 32 * kb + 8 * lq);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      cw.w2[c][kb] = ld8(W.W2T + n * SUR_H + 32 * kb + 8 * lq);
+      if (with_w2r) w2r[c][kb] = ld8(W.W2R + n * SUR_H + 32 * kb + 8 * lq);
+    }
+    cw.b2[c] = W.b2[n];
+    cw.w3[c] = W.w3[n];
+  }
+}
+
+// X rows [r0, r0 + 64) into xs (row-major) and, when xt is given, xt
+// (transposed, [64 features][SUR_TR]). All 256 threads.
+__device__ inline void stage_x(const unsigned short *X, int64_t r0, unsigned short *xs,
+                               unsigned short *xt) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int e = t + 256 * h;       // 512 pieces of 8 bf16
+    const int r = e >> 3, c8 = (e & 7) * 8;
+    const uint4 v = *reinterpret_cast<const uint4 *>(X + (r0 + r) * SUR_F + c8);
+    *reinterpret_cast<uint4 *>(xs + r * SUR_XS + c8) = v;
+    if (xt) {
+      const unsigned int w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        xt[(c8 + 2 * q) * SUR_TR + r] = (unsigned short)(w[q] & 0xFFFFu);
+        xt[(c8 + 2 * q + 1) * SUR_TR + r] = (unsigned short)(w[q] >> 16);
+      }
+    }
+  }
+}
+
+// Layer 1 of the wave's column slice: h1[rt][c] = C tile (rows 16 rt.., cols
+// 32w + 16c ..), from the staged X tile.
+__device__ inline void col_layer1(const unsigned short *xs, const ColW &cw, f32x4_t (&h1)[4][2]) {
+  const int l = threadIdx.x & 63, lr = l & 15, lq = l >> 4;
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) h1[rt][0] = h1[rt][1] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      const bf16x8_t a = ld8(xs + (16 * rt + lr) * SUR_XS + 32 * kb + 8 * lq);
+      h1[rt][0] = mfma32(a, cw.w1[0][kb], h1[rt][0]);
+      h1[rt][1] = mfma32(a, cw.w1[1][kb], h1[rt][1]);
+    }
+}
+
+// Layer 2 of the wave's column slice from the shared A1 tile (+ b2).
+__device__ inline void col_layer2(const unsigned short *a1s, const ColW &cw, f32x4_t (&h2)[4][2]) {
+  const int l = threadIdx.x & 63, lr = l & 15, lq = l >> 4;
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) h2[rt][0] = h2[rt][1] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      const bf16x8_t a = ld8(a1s + (16 * rt + lr) * SUR_AS + 32 * kb + 8 * lq);
+      h2[rt][0] = mfma32(a, cw.w2[0][kb], h2[rt][0]);
+      h2[rt][1] = mfma32(a, cw.w2[1][kb], h2[rt][1]);
+    }
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h2[rt][c][i] += cw.b2[c];
+}
+
+// Store the C tiles v (rows 16 rt + 4 lq + i, column 32w + 16c + lr) as bf16:
+// row-major into rm (row stride SUR_AS) and, when tr is given, transposed
+// into tr ([128][SUR_TR]).
+__device__ inline void put_tiles(const float (&v)[4][2][4], int wv, unsigned short *rm,
+                                 unsigned short *tr) {
+  const int l = threadIdx.x & 63, lr = l & 15, lq = l >> 4;
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = 32 * wv + 16 * c + lr;
+      unsigned short b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        b[i] = f2bf(v[rt][c][i]);
+        if (rm) rm[(16 * rt + 4 * lq + i) * SUR_AS + col] = b[i];
+      }
+      if (tr)
+        *reinterpret_cast<uint2 *>(tr + col * SUR_TR + 16 * rt + 4 * lq) =
+            make_uint2(b[0] | ((unsigned)b[1] << 16), b[2] | ((unsigned)b[3] << 16));
+    }
+}
+
+// Per-row outputs of a batch: y[r] = w3 . silu(H2[r]) + b3. Workgroup b
+// takes 64-row steps b, b + G, ... (G = gridDim.x).
 extern "C" __global__ void __launch_bounds__(256) gd_mlp_fwd_kernel(const unsigned short *X,
                                                                     int64_t n_rows, SurW W,
                                                                     float *y) {
-  __shared__ __attribute__((aligned(16))) unsigned short a1s[4][16 * SUR_H];
-  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int64_t r0 = ((int64_t)blockIdx.x * 4 + wv) * 16;
-  if (r0 >= n_rows) return;  // whole wave leaves; no block barrier below
-  f32x4_t h1[8], h2[8];
-  mlp_fwd16(X, r0, W, a1s[wv], h1, h2);
-  float yy[4];
-  mlp_out16(h2, W, yy);
-  if ((l & 15) == 0)
+  __shared__ __attribute__((aligned(16))) unsigned short xs[SUR_STEP * SUR_XS];
+  __shared__ __attribute__((aligned(16))) unsigned short a1s[SUR_STEP * SUR_AS];
+  __shared__ float red[4][SUR_STEP];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lq = l >> 4;
+  ColW cw;
+  bf16x8_t unused[2][4];
+  load_colw(W, wv, false, cw, unused);
+  const float b3 = W.b3[0];
+  const int64_t n_steps = n_rows / SUR_STEP;
+  for (int64_t st = blockIdx.x; st < n_steps; st += gridDim.x) {
+    const int64_t r0 = st * SUR_STEP;
+    stage_x(X, r0, xs, nullptr);
+    __syncthreads();
+    f32x4_t h1[4][2], h2[4][2];
+    col_layer1(xs, cw, h1);
+    float a1[4][2][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) y[r0 + 4 * (l >> 4) + i] = yy[i];
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a1[rt][c][i] = silu(h1[rt][c][i]);
+    put_tiles(a1, wv, a1s, nullptr);
+    __syncthreads();
+    col_layer2(a1s, cw, h2);
+    // this wave's 32 columns of w3 . silu(H2) per row, then across the waves
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float v = fmaf(silu(h2[rt][0][i]), cw.w3[0], silu(h2[rt][1][i]) * cw.w3[1]);
+        v += __shfl_xor(v, 1);
+        v += __shfl_xor(v, 2);
+        v += __shfl_xor(v, 4);
+        v += __shfl_xor(v, 8);
+        if (lr == 0) red[wv][16 * rt + 4 * lq + i] = v;
+      }
+    __syncthreads();
+    if (threadIdx.x < SUR_STEP) {
+      const int r = threadIdx.x;
+      y[r0 + r] = ((red[0][r] + red[1][r]) + (red[2][r] + red[3][r])) + b3;
+    }
+    __syncthreads();  // xs, a1s and red are rewritten by the next step
+  }
 }
 
 // Network output at every grid point for each listed ligand atom type:
@@ -142,178 +295,179 @@ extern "C" __global__ void gd_mlp_pose_kernel(const float *y, const int64_t *off
   if (S_out) S_out[p] = s;
 }
 
-// Row stride (bf16) of the transposed LDS tiles of gd_mlp_train_kernel: 64
-// rows and 8 of padding, 144 bytes (a multiple of 16 for 16-byte reads).
-constexpr int SUR_TR = 72;
-constexpr int SUR_STEP = 64;  // rows per step of gd_mlp_train_kernel (4 waves x 16)
-
-// Backward of one batch fused with its weight gradients. Workgroup b owns
-// rows [b R, (b + 1) R) (R a multiple of 64) and walks them 64 at a time;
-// wave w recomputes the forward of rows 16 w .. 16 w + 15 of the step, forms
+// Backward of one batch fused with its weight gradients, column-split like
+// gd_mlp_fwd_kernel. Workgroup b takes the contiguous 64-row steps
+// [b S / G, (b + 1) S / G) (S steps, G = gridDim.x). Per step, wave w
+// recomputes H1 and H2 of its 32 columns, forms
 //   dy = dS of the row's pose,  dH2 = dy w3 * silu'(H2),
-//   dA1 = dH2 W2^T,             dH1 = dA1 * silu'(H1)
-// and writes X, A1 = silu(H1), dH2 and dH1 of its rows transposed into LDS.
-// After a workgroup barrier wave w accumulates its share of the step's
-//   dW2 += A1^T dH2   (row tiles 2w and 2w + 1 of 8),
-//   dW1 += X^T dH1    (row tile w of 4; row 56 is the b1 gradient)
-// in MFMA accumulators. Nothing but the partials goes back to HBM, in a fixed
-// order (gd_mlp_gradsum_kernel and gd_mlp_w3sum_kernel sum them, so training
-// is deterministic): p1g[b][64][128], p2g[b][128][128], pb2[b][128] and
-// pw3[4 b + w][129] (w3, then b3).
+//   dA1 = dH2 W2^T (all of dH2 from LDS, W2^T slice in registers),
+//   dH1 = dA1 * silu'(H1)
+// and accumulates its 32 columns of
+//   dW2 += A1^T dH2   (A1^T of all waves from LDS),
+//   dW1 += X^T dH1    (row 56 of dW1 is the b1 gradient)
+// in MFMA accumulators, with b2, w3 and b3. Only fixed-order partials go to
+// HBM: p1g[b][64][128], p2g[b][128][128], pb2[b][128], pw3[b][129]
+// (summed in a fixed order by gd_mlp_update_kernel or gd_mlp_gradsum_kernel,
+// so training is deterministic).
 extern "C" __global__ void __launch_bounds__(256) gd_mlp_train_kernel(
-    const unsigned short *X, int64_t n_rows, int R, const int *pose_of_row, int p0, int p1,
+    const unsigned short *X, int64_t n_rows, const int *pose_of_row, int p0, int p1,
     const float *dS, SurW W, float *p1g, float *p2g, float *pb2, float *pw3) {
   extern __shared__ __attribute__((aligned(16))) unsigned char sm_train[];
-  unsigned short *a1t = reinterpret_cast<unsigned short *>(sm_train);  // A1^T [128][SUR_TR]
-  unsigned short *h2t = a1t + SUR_H * SUR_TR;                          // dH2^T
-  unsigned short *h1t = h2t + SUR_H * SUR_TR;                          // dH1^T
-  unsigned short *xt = h1t + SUR_H * SUR_TR;                           // X^T [64][SUR_TR]
-  unsigned short *rows = xt + SUR_F * SUR_TR;                          // [4][16][128] row-major
-  float *red = reinterpret_cast<float *>(rows + 4 * 16 * SUR_H);      // [4][128]
-  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int lr = l & 15, lq = l >> 4;
-  const int64_t base = (int64_t)blockIdx.x * R;
-  const int64_t end = base + R < n_rows ? base + R : n_rows;
-  unsigned short *ts = rows + wv * 16 * SUR_H;
-  f32x4_t g2[2][8], g1[8];
+  unsigned short *xs = reinterpret_cast<unsigned short *>(sm_train);  // X [64][SUR_XS]
+  unsigned short *xt = xs + SUR_STEP * SUR_XS;                          // X^T [64][SUR_TR]
+  unsigned short *a1s = xt + SUR_F * SUR_TR;                            // A1 [64][SUR_AS]
+  unsigned short *a1t = a1s + SUR_STEP * SUR_AS;                        // A1^T [128][SUR_TR]
+  unsigned short *d2s = a1t + SUR_H * SUR_TR;                           // dH2 [64][SUR_AS]
+  unsigned short *d2t = d2s + SUR_STEP * SUR_AS;                        // dH2^T [128][SUR_TR]
+  unsigned short *d1t = d2t + SUR_H * SUR_TR;                           // dH1^T [128][SUR_TR]
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lq = l >> 4;
+  ColW cw;
+  bf16x8_t w2r[2][4];
+  load_colw(W, wv, true, cw, w2r);
+  f32x4_t g2[8][2], g1[4][2];
 #pragma unroll
-  for (int n = 0; n < 8; ++n) {
-    g2[0][n] = g2[1][n] = g1[n] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  for (int c = 0; c < 2; ++c) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) g2[t][c] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) g1[t][c] = f32x4_t{0.f, 0.f, 0.f, 0.f};
   }
-  float gw3[8], gb2[8], gb3 = 0.0f;
-#pragma unroll
-  for (int u = 0; u < 8; ++u) gw3[u] = gb2[u] = 0.0f;
-
-  for (int64_t s0 = base; s0 < end; s0 += SUR_STEP) {
-    const int64_t r0 = s0 + 16 * wv;
-    const int cw = 16 * wv;  // this wave's columns in the transposed tiles
-    f32x4_t h1[8], h2[8];
-    mlp_fwd16(X, r0, W, ts, h1, h2);
-    // X^T: lane (lr, lq) moves columns 16 lq .. 16 lq + 15 of row lr
+  float gw3[2] = {0.f, 0.f}, gb2[2] = {0.f, 0.f}, gb3 = 0.0f;
+  const int64_t n_steps = n_rows / SUR_STEP;
+  const int64_t s_beg = n_steps * blockIdx.x / gridDim.x;
+  const int64_t s_end = n_steps * (blockIdx.x + 1) / gridDim.x;
+  for (int64_t st = s_beg; st < s_end; ++st) {
+    const int64_t r0 = st * SUR_STEP;
+    stage_x(X, r0, xs, xt);
+    __syncthreads();
+    f32x4_t h1[4][2], h2[4][2];
+    col_layer1(xs, cw, h1);
     {
-      const unsigned short *xr = X + (r0 + lr) * SUR_F + 16 * lq;
+      float a1[4][2][4];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) xt[(16 * lq + q) * SUR_TR + cw + lr] = xr[q];
-    }
-    // A1^T from the C layout (rows 4 lq + i, column 16 t + lr)
+      for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      unsigned short v[4];
+        for (int c = 0; c < 2; ++c)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] = f2bf(silu(h1[t][i]));
-      *reinterpret_cast<uint2 *>(a1t + (16 * t + lr) * SUR_TR + cw + 4 * lq) =
-          make_uint2(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16));
-    }
-    float dy[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int p = pose_of_row[r0 + 4 * lq + i];
-      dy[i] = (p >= p0 && p < p1) ? dS[p] : 0.0f;
-    }
-    // w3 and b3 (rows counted once: b3 from the lr == 0 lanes)
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) gw3[u] = fmaf(silu(h2[u][i]), dy[i], gw3[u]);
-    if (lr == 0) gb3 += (dy[0] + dy[1]) + (dy[2] + dy[3]);
-    // dH2: row-major into the wave's tile (A operand of dA1) and transposed
-    __builtin_amdgcn_s_waitcnt(0xc07f);  // the forward's LDS reads of ts are done
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const float w = W.w3[16 * u + lr];
-      unsigned short v[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float g = dy[i] * w * dsilu(h2[u][i]);
-        gb2[u] += g;
-        v[i] = f2bf(g);
-        ts[(4 * lq + i) * SUR_H + 16 * u + lr] = v[i];
-      }
-      *reinterpret_cast<uint2 *>(h2t + (16 * u + lr) * SUR_TR + cw + 4 * lq) =
-          make_uint2(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16));
-    }
-    __builtin_amdgcn_s_waitcnt(0xc07f);
-    __builtin_amdgcn_wave_barrier();
-    // dA1 = dH2 W2^T (B[k][n] = W2[n][k] = W2R[n][k]), dH1 transposed
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      f32x4_t c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < 4; ++ks)
-        c = mfma32(ld8(ts + lr * SUR_H + 32 * ks + 8 * lq),
-                   ld8(W.W2R + (16 * t + lr) * SUR_H + 32 * ks + 8 * lq), c);
-      unsigned short v[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] = f2bf(c[i] * dsilu(h1[t][i]));
-      *reinterpret_cast<uint2 *>(h1t + (16 * t + lr) * SUR_TR + cw + 4 * lq) =
-          make_uint2(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16));
+          for (int i = 0; i < 4; ++i) a1[rt][c][i] = silu(h1[rt][c][i]);
+      put_tiles(a1, wv, a1s, a1t);
     }
     __syncthreads();
+    col_layer2(a1s, cw, h2);
+    // dy of the rows 16 rt + 4 lq + i; w3, b2, b3 gradients
+    float dy[4][4];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int p = pose_of_row[r0 + 16 * rt + 4 * lq + i];
+        dy[rt][i] = (p >= p0 && p < p1) ? dS[p] : 0.0f;
+      }
+    if (wv == 0 && lr == 0)
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) gb3 += (dy[rt][0] + dy[rt][1]) + (dy[rt][2] + dy[rt][3]);
+    {
+      float d2[4][2][4];
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            gw3[c] = fmaf(silu(h2[rt][c][i]), dy[rt][i], gw3[c]);
+            const float g = dy[rt][i] * cw.w3[c] * dsilu(h2[rt][c][i]);
+            gb2[c] += g;
+            d2[rt][c][i] = g;
+          }
+      put_tiles(d2, wv, d2s, d2t);
+    }
+    __syncthreads();
+    // dA1 = dH2 W2^T for this wave's columns, dH1 transposed
+    {
+      f32x4_t da[4][2];
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) da[rt][0] = da[rt][1] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+          const bf16x8_t a = ld8(d2s + (16 * rt + lr) * SUR_AS + 32 * kb + 8 * lq);
+          da[rt][0] = mfma32(a, w2r[0][kb], da[rt][0]);
+          da[rt][1] = mfma32(a, w2r[1][kb], da[rt][1]);
+        }
+      float d1[4][2][4];
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) d1[rt][c][i] = da[rt][c][i] * dsilu(h1[rt][c][i]);
+      put_tiles(d1, wv, nullptr, d1t);
+    }
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // this wave's dH1^T stores are in LDS
+    __builtin_amdgcn_wave_barrier();
     // weight gradients over the step's 64 rows (two K blocks of 32)
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) {
       const int ko = 32 * kb + 8 * lq;
+      bf16x8_t b2v[2], b1v[2];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const bf16x8_t A = ld8(a1t + (16 * (2 * wv + a) + lr) * SUR_TR + ko);
-#pragma unroll
-        for (int n = 0; n < 8; ++n)
-          g2[a][n] = mfma32(A, ld8(h2t + (16 * n + lr) * SUR_TR + ko), g2[a][n]);
+      for (int c = 0; c < 2; ++c) {
+        b2v[c] = ld8(d2t + (32 * wv + 16 * c + lr) * SUR_TR + ko);
+        b1v[c] = ld8(d1t + (32 * wv + 16 * c + lr) * SUR_TR + ko);
       }
-      const bf16x8_t A = ld8(xt + (16 * wv + lr) * SUR_TR + ko);
 #pragma unroll
-      for (int n = 0; n < 8; ++n) g1[n] = mfma32(A, ld8(h1t + (16 * n + lr) * SUR_TR + ko), g1[n]);
+      for (int t = 0; t < 8; ++t) {
+        const bf16x8_t a = ld8(a1t + (16 * t + lr) * SUR_TR + ko);
+        g2[t][0] = mfma32(a, b2v[0], g2[t][0]);
+        g2[t][1] = mfma32(a, b2v[1], g2[t][1]);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const bf16x8_t a = ld8(xt + (16 * t + lr) * SUR_TR + ko);
+        g1[t][0] = mfma32(a, b1v[0], g1[t][0]);
+        g1[t][1] = mfma32(a, b1v[1], g1[t][1]);
+      }
     }
     __syncthreads();  // the tiles are rewritten by the next step
   }
 
-  // partials of this workgroup
+  // partials of this workgroup: its 4 waves own disjoint columns
   float *o2 = p2g + (int64_t)blockIdx.x * SUR_H * SUR_H;
   float *o1 = p1g + (int64_t)blockIdx.x * SUR_F * SUR_H;
 #pragma unroll
-  for (int n = 0; n < 8; ++n)
+  for (int c = 0; c < 2; ++c) {
+    const int col = 32 * wv + 16 * c + lr;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      o2[(16 * (2 * wv) + 4 * lq + i) * SUR_H + 16 * n + lr] = g2[0][n][i];
-      o2[(16 * (2 * wv + 1) + 4 * lq + i) * SUR_H + 16 * n + lr] = g2[1][n][i];
-      o1[(16 * wv + 4 * lq + i) * SUR_H + 16 * n + lr] = g1[n][i];
-    }
-  // w3, b3: sum over the lane quarters (rows), one partial row per wave
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    float v = gw3[u];
+      for (int t = 0; t < 8; ++t) o2[(16 * t + 4 * lq + i) * SUR_H + col] = g2[t][c][i];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) o1[(16 * t + 4 * lq + i) * SUR_H + col] = g1[t][c][i];
+    }
+  }
+  float *w3o = pw3 + (int64_t)blockIdx.x * (SUR_H + 1);
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    float v = gw3[c];
     v += __shfl_xor(v, 16);
     v += __shfl_xor(v, 32);
-    gw3[u] = v;
-    float b = gb2[u];
+    float b = gb2[c];
     b += __shfl_xor(b, 16);
     b += __shfl_xor(b, 32);
-    gb2[u] = b;
+    if (lq == 0) {
+      w3o[32 * wv + 16 * c + lr] = v;
+      pb2[(int64_t)blockIdx.x * SUR_H + 32 * wv + 16 * c + lr] = b;
+    }
   }
   gb3 += __shfl_xor(gb3, 16);
   gb3 += __shfl_xor(gb3, 32);
-  float *w3o = pw3 + ((int64_t)blockIdx.x * 4 + wv) * (SUR_H + 1);
-  if (lq == 0) {
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      w3o[16 * u + lr] = gw3[u];
-      red[wv * SUR_H + 16 * u + lr] = gb2[u];
-    }
-    if (lr == 0) w3o[SUR_H] = gb3;
-  }
-  __syncthreads();
-  // b2: the four waves' column sums in a fixed order
-  if (threadIdx.x < SUR_H) {
-    const int j = threadIdx.x;
-    pb2[(int64_t)blockIdx.x * SUR_H + j] =
-        (red[j] + red[SUR_H + j]) + (red[2 * SUR_H + j] + red[3 * SUR_H + j]);
-  }
+  if (wv == 0 && l == 0) w3o[SUR_H] = gb3;
 }
 
 // Dynamic LDS bytes of gd_mlp_train_kernel.
-constexpr int SUR_TRAIN_LDS = (3 * SUR_H * SUR_TR + SUR_F * SUR_TR + 4 * 16 * SUR_H) * 2 +
-                              4 * SUR_H * 4;
+constexpr int SUR_TRAIN_LDS =
+    (SUR_STEP * SUR_XS + SUR_F * SUR_TR + 2 * SUR_STEP * SUR_AS + 3 * SUR_H * SUR_TR) * 2;
 
 // Sum of the squared residuals of poses [0, n) into *out: one workgroup,
 // strided partial sums then a fixed-shape tree (deterministic).
@@ -357,27 +511,42 @@ extern "C" __global__ void gd_mlp_gradsum_kernel(const float *p1, int n1, const 
   } else if (q < SP_W3) {
     s = sum_partials(pb2, n2, SUR_H, q - SP_B2);
   } else {
-    return;  // w3 and b3: gd_mlp_w3sum_kernel
+    s = sum_partials(pw3, (int)nw, SUR_H + 1, q - SP_W3);
   }
   g[q] = s;
 }
 
-// w3[j] and b3 (j == 128) gradients from the per-wave partials: one
-// workgroup per output, strided partial sums then a fixed-shape tree
-// (deterministic).
-extern "C" __global__ void __launch_bounds__(256) gd_mlp_w3sum_kernel(const float *pw3, int64_t nw,
-                                                                      float *g) {
-  __shared__ float red[256];
-  const int j = blockIdx.x;
-  float s = 0.0f;
-  for (int64_t w = threadIdx.x; w < nw; w += 256) s += pw3[w * (SUR_H + 1) + j];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-    __syncthreads();
+// One training update: each parameter's gradient from the fixed-order sum of
+// the backward's per-workgroup partials (ng of each), then Adam on it and the
+// refresh of its bf16 copies. The gradient is also left in g.
+extern "C" __global__ void gd_mlp_update_kernel(const float *p1, const float *p2, const float *pb2,
+                                                const float *pw3, int ng, float *p, float *m,
+                                                float *v, float *g, float lr, float b1, float b2,
+                                                float eps, float c1, float c2,
+                                                unsigned short *W1T, unsigned short *W2T,
+                                                unsigned short *W2R) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= SUR_NPARAM) return;
+  float gq;
+  if (q < SP_W2) gq = sum_partials(p1, ng, (int64_t)SUR_F * SUR_H, q);
+  else if (q < SP_B2) gq = sum_partials(p2, ng, (int64_t)SUR_H * SUR_H, q - SP_W2);
+  else if (q < SP_W3) gq = sum_partials(pb2, ng, SUR_H, q - SP_B2);
+  else gq = sum_partials(pw3, ng, SUR_H + 1, q - SP_W3);
+  g[q] = gq;
+  const float mq = b1 * m[q] + (1.0f - b1) * gq;
+  const float vq = b2 * v[q] + (1.0f - b2) * gq * gq;
+  m[q] = mq;
+  v[q] = vq;
+  const float x = p[q] - lr * (mq / c1) / (sqrtf(vq / c2) + eps);
+  p[q] = x;
+  if (q < SP_W2) {
+    const int f = q / SUR_H, i = q % SUR_H;
+    W1T[i * SUR_F + f] = f2bf(x);
+  } else if (q < SP_B2) {
+    const int i = (q - SP_W2) / SUR_H, j = (q - SP_W2) % SUR_H;
+    W2T[j * SUR_H + i] = f2bf(x);
+    W2R[i * SUR_H + j] = f2bf(x);
   }
-  if (threadIdx.x == 0) g[SP_W3 + j] = red[0];
 }
 
 // Adam on the FP32 master parameters; refreshes the bf16 layouts.
