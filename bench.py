@@ -73,7 +73,7 @@ def parse(argv=None):
     ap.add_argument("--lpr", type=int, default=0)
     ap.add_argument("--apl", type=int, default=0)
     ap.add_argument("--split", type=int, default=-1,
-                    help="receptor split (0 = engine's choice, 1, 2; default: per workload)")
+                    help="receptor split (0 = engine's per-workgroup choice, or 1, 2, 4, 8)")
     ap.add_argument("--ladders", type=int, default=-1,
                     help="temperature ladders per workgroup (0 = engine's choice)")
     ap.add_argument("--launch", type=int, default=-1, help="GD_LAUNCH_* flags")
@@ -278,13 +278,13 @@ def roofline(pairs_per_step, mc_steps, t_compute, launches, kname):
 
 
 def launch_options(args, cxs):
-    """Launch shape: the autotuner's choice with --autotune 1, else the shape
-    the autotuner picks for the uneven batches (receptor split 2, 3 ladders
-    per workgroup, persistent) for `batch` and the engine's own choice
-    otherwise; explicit flags override."""
+    """Launch shape: the autotuner's choice with --autotune 1, else the
+    engine's automatic packing (whole ladders, then the receptor split that
+    fills each workgroup to 8 waves), persistent for `batch`; explicit flags
+    override."""
     from geaxdock_gpu_amd import abi
     if args.config == "batch":
-        opts = dict(receptor_split=2, ladders_per_pack=3, launch=abi.LAUNCH_PERSISTENT)
+        opts = dict(receptor_split=0, ladders_per_pack=0, launch=abi.LAUNCH_PERSISTENT)
     else:
         opts = dict(receptor_split=0, ladders_per_pack=0, launch=0)
     opts.update(lanes_per_replica=args.lpr, atoms_per_lane=args.apl, receptor_tile=args.tile)

@@ -4,8 +4,9 @@ The engine's launch has four knobs that trade occupancy, lane utilization
 and load balance against each other, and the best setting depends on the
 mix of ligand and receptor sizes:
 
-  receptor_split    1 or 2 lane groups per ladder, each over its share of
-                    every receptor chunk (2 doubles the waves of a pack);
+  receptor_split    0 (per workgroup: whole ladders, then the split that
+                    fills 8 waves), or 1 or 2 lane groups per ladder for every
+                    workgroup, each over its share of every receptor chunk;
   ladders_per_pack  temperature ladders per workgroup (replicas per
                     workgroup = ladders x n_tmp);
   launch            persistent workgroups or one workgroup per pack;
@@ -44,7 +45,7 @@ def candidates(cxs, full=False):
     """Shape option sets to try (default tiling). The default grid covers the
     knobs that matter on uneven batches; `full` adds the lane-shape axis."""
     out = []
-    for split in (1, 2):
+    for split in (0, 1, 2):
         for ladders in (0, 1, 2, 3, 4):
             for launch in (0, abi.LAUNCH_PERSISTENT):
                 out.append(dict(receptor_split=split, ladders_per_pack=ladders, launch=launch,

@@ -33,10 +33,11 @@ namespace {
 // kernel instance, the LDS layout and the block shape.
 struct LaunchClass {
   int apl = 1, begin = 0, end = 0;
-  int tile = 0;    // receptor tile length (0: receptor resident in LDS)
-  int block = 64;  // lanes: n_split groups of `half`
-  size_t lds = 0;  // dynamic LDS bytes
-  KParams lay{};   // LDS offsets and receptor-split fields
+  int tile = 0;      // receptor tile length (0: receptor resident in LDS)
+  int block = 64;    // lanes: each pack's ns groups of its `half` lanes
+  int half_max = 64;  // largest pack half (the block of an unsplit launch)
+  size_t lds = 0;    // dynamic LDS bytes
+  KParams lay{};     // LDS offsets and exchange-buffer fields
 };
 
 struct HostComplex {
@@ -46,6 +47,7 @@ struct HostComplex {
   bool tiled = false;  // receptor streamed through LDS tiles
   bool ref_built = false;
   std::vector<int> lig_t;  // ligand atom types (host copy)
+  std::vector<int> lig_ns;  // receptor split of each ligand conformer's MC packs
   DevComplex dev{};
   std::vector<void *> bufs;  // device allocations owned by this complex
   size_t lds_bytes = 0;
@@ -83,7 +85,7 @@ struct gd_ctx {
   // packs of the MC launch
   Pack *d_packs = nullptr;
   int *d_slots = nullptr;
-  int n_packs = 0, n_split = 1, ladders_per_pack = 0;
+  int n_packs = 0, ladders_per_pack = 0;
   std::vector<LaunchClass> mc_classes;  // kernel launches of the MC pack set
   // device control words: [0] first step of a graph replay, [1 + i] pack
   // counter of class i in persistent launches
@@ -626,10 +628,13 @@ int complex_tile(const gd_ctx *ctx, const HostComplex &h) {
   return ctx->opts.receptor_tile > 0 ? ctx->opts.receptor_tile : DEFAULT_TILE;
 }
 
-// LDS layout for a launch covering packs with these maxima. With tile > 0
-// the receptor arrays are two tiles of `tile` points and KDE stays in HBM.
+// LDS layout for a launch covering packs with these maxima, in blocks of
+// `block` lanes; `split` adds the partial-sum exchange buffers of packs with
+// more than one lane group ([ns*half][4] HPC sums and [ns][half/8][8] replica
+// sums, ns*half = block). With tile > 0 the receptor arrays are two tiles of
+// `tile` points and KDE stays in HBM.
 void layout(KParams &P, size_t *bytes, int pnp_max, int pnk_max, int nch_max, int slots_max,
-            int n_split, int half, int tile = 0) {
+            int block, bool split, int tile = 0) {
   auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
   P.tile = tile;
   if (tile > 0) {
@@ -657,11 +662,7 @@ void layout(KParams &P, size_t *bytes, int pnp_max, int pnk_max, int nch_max, in
   o = al(o + (size_t)slots_max * 4);
   P.off_xo = (int)o;
   o = al(o + (size_t)slots_max * 4);
-  P.n_split = n_split;
-  P.half = half;
-  // every lane of a group maps to a slot below half / 8 (lpr >= 8), active or not
-  P.xp_slots = half / 8;
-  P.xp_floats = n_split > 1 ? n_split * half * 4 + n_split * P.xp_slots * 8 : 0;
+  P.xp_floats = split ? 5 * block : 0;
   P.off_xp = (int)o;
   o = al(o + (size_t)2 * P.xp_floats * 4);
   P.off_q = (int)o;
@@ -687,36 +688,42 @@ void apply_layout(KParams &P, const KParams &L) {
   P.off_xi = L.off_xi;
   P.off_xa = L.off_xa;
   P.off_xo = L.off_xo;
-  P.n_split = L.n_split;
-  P.half = L.half;
   P.off_xp = L.off_xp;
   P.xp_floats = L.xp_floats;
-  P.xp_slots = L.xp_slots;
   P.off_q = L.off_q;
   P.tile = L.tile;
 }
 
-// Group packs into launch classes by (atoms per lane, workgroups per CU that
-// the pack's LDS allows), each class longest packs first so the hardware
-// dispatcher fills the CUs in LPT order. A class's LDS is the largest need
-// among its packs, so small complexes keep several workgroups per CU.
-void pack_finish(const gd_ctx *ctx, PackSet &ps, int ns) {
+// Group packs into launch classes by (atoms per lane, tile, block shape,
+// workgroups per CU that the LDS allows), each class longest packs first so
+// the hardware dispatcher (or the persistent queue) fills the CUs in LPT
+// order. Every pack of a class has the class's block (its ns groups of `half`
+// lanes), so no wave of a workgroup is idle. A class's LDS is the largest need
+// among its packs: a class closes when the next pack would lower the
+// workgroups per CU it allows. Workgroups of 512 lanes take a CU's register
+// file alone (two waves per SIMD), so LDS does not split their classes.
+void pack_finish(const gd_ctx *ctx, PackSet &ps) {
   const size_t n_pk = ps.packs.size();
-  std::vector<int> res(n_pk), half(n_pk), apl(n_pk), tile(n_pk);
+  std::vector<int> res(n_pk), block(n_pk), apl(n_pk), tile(n_pk);
   std::vector<double> cost(n_pk);
   for (size_t i = 0; i < n_pk; ++i) {
-    const Pack &p = ps.packs[i];
+    Pack &p = ps.packs[i];
     const HostComplex &h = ctx->cx[p.cx];
-    half[i] = ((p.n_slots * h.lpr + 63) / 64) * 64;
+    p.half = ((p.n_slots * h.lpr + 63) / 64) * 64;
+    block[i] = p.half * p.ns;
     apl[i] = h.apl;
     // the tile length depends on the options only (not on the pack's block),
     // so every launch of a complex sums its chunks in the same order
     tile[i] = complex_tile(ctx, h);
-    cost[i] = (double)h.d.pnp * (double)half[i] * h.apl;
+    cost[i] = (double)h.d.pnp * (double)p.half * h.apl;
     KParams L{};
     size_t need = 0;
-    layout(L, &need, (h.d.pnp + 3) & ~3, h.d.pnk, h.n_chunks, p.n_slots, ns, half[i], tile[i]);
-    res[i] = (int)std::max<size_t>(1, std::min<size_t>(8, ctx->max_lds / std::max<size_t>(need, 1)));
+    layout(L, &need, (h.d.pnp + 3) & ~3, h.d.pnk, h.n_chunks, p.n_slots, block[i], p.ns > 1,
+           tile[i]);
+    res[i] = block[i] >= 512
+                 ? 1
+                 : (int)std::max<size_t>(1, std::min<size_t>(512 / block[i],
+                                                             ctx->max_lds / std::max<size_t>(need, 1)));
   }
   // all per-pack keys are indexed by the pack's position before sorting
   std::vector<int> order(n_pk);
@@ -724,6 +731,7 @@ void pack_finish(const gd_ctx *ctx, PackSet &ps, int ns) {
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     if (apl[a] != apl[b]) return apl[a] < apl[b];
     if (tile[a] != tile[b]) return tile[a] < tile[b];
+    if (block[a] != block[b]) return block[a] > block[b];
     if (res[a] != res[b]) return res[a] < res[b];
     return cost[a] > cost[b];
   });
@@ -732,59 +740,130 @@ void pack_finish(const gd_ctx *ctx, PackSet &ps, int ns) {
   ps.packs.swap(sorted);
   ps.classes.clear();
   for (size_t b = 0; b < n_pk;) {
-    size_t e = b;
-    const int a0 = apl[order[b]], r0 = res[order[b]];
+    const int o0 = order[b];
     LaunchClass lc;
-    lc.apl = a0;
+    lc.apl = apl[o0];
     lc.begin = (int)b;
+    lc.block = block[o0];
+    lc.tile = tile[o0];
     int pnp_max = 0, pnk_max = 0, nch_max = 0, slots_max = 1, half_max = 64;
-    while (e < n_pk && apl[order[e]] == a0 && res[order[e]] == r0 &&
-           tile[order[e]] == tile[order[b]]) {
+    bool split = false;
+    size_t e = b;
+    for (; e < n_pk; ++e) {
+      const int oe = order[e];
+      if (apl[oe] != lc.apl || tile[oe] != lc.tile || block[oe] != lc.block || res[oe] != res[o0])
+        break;
       const Pack &p = ps.packs[e];
       const HostComplex &h = ctx->cx[p.cx];
-      pnp_max = std::max(pnp_max, h.d.pnp);
-      pnk_max = std::max(pnk_max, h.d.pnk);
-      nch_max = std::max(nch_max, h.n_chunks);
-      slots_max = std::max(slots_max, p.n_slots);
-      half_max = std::max(half_max, half[order[e]]);
-      ++e;
+      const int np = std::max(pnp_max, h.d.pnp), nk = std::max(pnk_max, h.d.pnk);
+      const int nc = std::max(nch_max, h.n_chunks), ns = std::max(slots_max, p.n_slots);
+      const bool sp = split || p.ns > 1;
+      KParams L{};
+      size_t need = 0;
+      layout(L, &need, (np + 3) & ~3, nk, nc, ns, lc.block, sp, lc.tile);
+      const size_t fit = ctx->max_lds / std::max<size_t>(need, 1);
+      if (e > b && (fit < 1 || (res[o0] > 1 && fit < (size_t)res[o0]))) break;
+      pnp_max = np;
+      pnk_max = nk;
+      nch_max = nc;
+      slots_max = ns;
+      split = sp;
+      half_max = std::max(half_max, p.half);
     }
     lc.end = (int)e;
-    lc.block = half_max * ns;
-    lc.tile = tile[order[b]];
-    layout(lc.lay, &lc.lds, (pnp_max + 3) & ~3, pnk_max, nch_max, slots_max, ns, half_max,
+    lc.half_max = half_max;
+    layout(lc.lay, &lc.lds, (pnp_max + 3) & ~3, pnk_max, nch_max, slots_max, lc.block, split,
            lc.tile);
     ps.classes.push_back(lc);
     b = e;
   }
 }
 
-// MC packs: whole ladders of one (complex, protein conformer), about 256
-// lanes per workgroup and at most max_lanes, or `ladders` ladders when set.
-PackSet build_mc_packs(const gd_ctx *ctx, int max_lanes, int ns, int ladders) {
-  PackSet ps;
-  for (int c = 0; c < (int)ctx->cx.size(); ++c) {
-    const HostComplex &h = ctx->cx[c];
-    const int n_tmp = h.d.n_tmp, n_lig = h.d.n_lig;
-    const int glanes = n_tmp * h.lpr;
+// Ladders of one (complex, protein conformer) grouped into packs, in
+// ligand-conformer order: {first ligand conformer, ladders, receptor split}.
+// The same grouping serves every protein conformer of the complex.
+//
+// A fixed split (receptor_split 1, 2, 4 or 8) takes `ladders_per_pack`
+// ladders per pack (default: about 256 lanes), within 512 lanes per workgroup.
+// The automatic split (0) fills every workgroup to 8 waves, the two waves per
+// SIMD that the kernels' registers allow: ladders of w waves are grouped by
+// powers of two, at most `gmax` ladders (and 8 waves) per pack, and each pack
+// takes the smallest split that fills 8 waves with its groups, so a ladder
+// left over from an odd count runs with more groups instead of leaving waves
+// idle. A ladder of 5 or more waves runs unsplit.
+struct LadderGroup {
+  int l0, ng, ns;
+};
+
+std::vector<LadderGroup> ladder_groups(const gd_ctx *ctx, const HostComplex &h, int gmax) {
+  std::vector<LadderGroup> out;
+  const int n_lig = h.d.n_lig, glanes = h.d.n_tmp * h.lpr;
+  const int fixed = ctx->opts.receptor_split, ladders = ctx->opts.ladders_per_pack;
+  if (fixed > 0) {
     int G = ladders > 0 ? ladders : std::max(1, 256 / glanes);
-    while (G > 1 && G * glanes > max_lanes) --G;
+    while (G > 1 && G * glanes > 512 / fixed) --G;
+    for (int l0 = 0; l0 < n_lig; l0 += G) out.push_back({l0, std::min(G, n_lig - l0), fixed});
+    return out;
+  }
+  const int w = (glanes + 63) / 64;
+  if (w >= 5) {
+    for (int l0 = 0; l0 < n_lig; ++l0) out.push_back({l0, 1, 1});
+    return out;
+  }
+  const int gcap = std::max(1, std::min(gmax, 8 / w));
+  for (int l0 = 0; l0 < n_lig;) {
+    int G = 1;
+    while (2 * G <= std::min(gcap, n_lig - l0)) G *= 2;
+    int ns = 1;
+    while (2 * ns * G * w <= 8) ns *= 2;
+    out.push_back({l0, G, ns});
+    l0 += G;
+  }
+  return out;
+}
+
+// Ladders per pack of the automatic split for the context's complexes:
+// `ladders_per_pack` when set, else the most (up to 8) that still leaves at
+// least four packs per CU for the longest-first balancing, down to one ladder
+// per pack (each then filled by its split) when the batch is small.
+int auto_gmax(const gd_ctx *ctx) {
+  if (ctx->opts.ladders_per_pack > 0) return ctx->opts.ladders_per_pack;
+  int g = 8;
+  for (; g > 1; g /= 2) {
+    int64_t packs = 0;
+    for (const HostComplex &h : ctx->cx) packs += (int64_t)h.d.n_prt * ladder_groups(ctx, h, g).size();
+    if (packs >= (int64_t)4 * ctx->n_cu) break;
+  }
+  return g;
+}
+
+// MC packs: the ladder groups of every (complex, protein conformer).
+PackSet build_mc_packs(gd_ctx *ctx) {
+  PackSet ps;
+  const int gmax = auto_gmax(ctx);
+  for (int c = 0; c < (int)ctx->cx.size(); ++c) {
+    HostComplex &h = ctx->cx[c];
+    const int n_tmp = h.d.n_tmp, n_lig = h.d.n_lig;
+    const std::vector<LadderGroup> groups = ladder_groups(ctx, h, gmax);
+    h.lig_ns.assign(n_lig, 1);
+    for (const LadderGroup &lg : groups)
+      for (int g = 0; g < lg.ng; ++g) h.lig_ns[lg.l0 + g] = lg.ns;
     for (int p = 0; p < h.d.n_prt; ++p)
-      for (int l0 = 0; l0 < n_lig; l0 += G) {
-        const int ng = std::min(G, n_lig - l0);
-        Pack pk;
+      for (const LadderGroup &lg : groups) {
+        Pack pk{};
         pk.cx = c;
         pk.prt = p;
-        pk.n_groups = ng;
-        pk.n_slots = ng * n_tmp;
+        pk.n_groups = lg.ng;
+        pk.n_slots = lg.ng * n_tmp;
+        pk.ns = lg.ns;
         pk.slot_off = (int)ps.slots.size();
-        for (int g = 0; g < ng; ++g)
+        for (int g = 0; g < lg.ng; ++g)
           for (int t = 0; t < n_tmp; ++t)
-            ps.slots.push_back(h.rep_base + n_tmp * n_lig * p + n_lig * t + (l0 + g));
+            ps.slots.push_back(h.rep_base + n_tmp * n_lig * p + n_lig * t + (lg.l0 + g));
         ps.packs.push_back(pk);
       }
   }
-  pack_finish(ctx, ps, ns);
+  pack_finish(ctx, ps);
   return ps;
 }
 
@@ -824,22 +903,16 @@ gd_status alloc_state(gd_ctx *ctx) {
     ctx->rec_cap = ctx->opts.steps_per_dump + 1;
     HIPCHK(hipMalloc(&ctx->d_rec, (size_t)n * ctx->rec_cap * sizeof(gd_record)));
   }
-  // receptor split: two lane groups per ladder pack when one group per pack
-  // would leave fewer than two waves per SIMD
-  int ns = ctx->opts.receptor_split;
-  if (ns < 0 || ns > 2) return fail(ctx, GD_ERR_INVALID, "receptor_split must be 0, 1 or 2");
-  bool fits2 = true;
-  for (auto &h : ctx->cx) fits2 = fits2 && h.d.n_tmp * h.lpr <= 256;
-  if (ns == 2 && !fits2)
-    return fail(ctx, GD_ERR_CAPACITY, "receptor split needs ladders of at most 256 lanes");
-  if (ns == 0) {
-    PackSet p1 = build_mc_packs(ctx, 512, 1, ctx->opts.ladders_per_pack);
-    int64_t waves = 0;
-    for (const Pack &pk : p1.packs) waves += (pk.n_slots * ctx->cx[pk.cx].lpr + 63) / 64;
-    ns = (fits2 && waves < (int64_t)8 * ctx->n_cu) ? 2 : 1;
-  }
-  ctx->n_split = ns;
-  PackSet ps = build_mc_packs(ctx, ns == 2 ? 256 : 512, ns, ctx->opts.ladders_per_pack);
+  // receptor split: automatic (0, per pack) or fixed
+  const int ns = ctx->opts.receptor_split;
+  if (ns != 0 && ns != 1 && ns != 2 && ns != 4 && ns != 8)
+    return fail(ctx, GD_ERR_INVALID, "receptor_split must be 0, 1, 2, 4 or 8");
+  for (auto &h : ctx->cx)
+    if (ns > 1 && h.d.n_tmp * h.lpr > 512 / ns)
+      return fail(ctx, GD_ERR_CAPACITY,
+                  "receptor split %d needs ladders of at most %d lanes (a ladder has %d)", ns,
+                  512 / ns, h.d.n_tmp * h.lpr);
+  PackSet ps = build_mc_packs(ctx);
   HIPCHK(hipMalloc(&ctx->d_packs, ps.packs.size() * sizeof(Pack)));
   HIPCHK(hipMalloc(&ctx->d_slots, ps.slots.size() * sizeof(int)));
   HIPCHK(hipMemcpyAsync(ctx->d_packs, ps.packs.data(), ps.packs.size() * sizeof(Pack),
@@ -860,11 +933,19 @@ gd_status alloc_state(gd_ctx *ctx) {
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->state_alloc = true;
   if (ctx->opts.verbose) {
-    fprintf(stderr, "[geauxdock] %d replicas, %d packs, split %d, %zu launch classes\n", n,
-            ctx->n_packs, ctx->n_split, ctx->mc_classes.size());
-    for (const LaunchClass &lc : ctx->mc_classes)
-      fprintf(stderr, "[geauxdock]   apl %d packs %d block %d LDS %zu B\n", lc.apl,
-              lc.end - lc.begin, lc.block, lc.lds);
+    fprintf(stderr, "[geauxdock] %d replicas, %d packs, split %s, %zu launch classes\n", n,
+            ctx->n_packs, ns ? "fixed" : "auto", ctx->mc_classes.size());
+    for (const LaunchClass &lc : ctx->mc_classes) {
+      int nsp[4] = {0, 0, 0, 0};
+      for (int i = lc.begin; i < lc.end; ++i) {
+        const int v = ps.packs[i].ns;
+        ++nsp[v >= 8 ? 3 : v >= 4 ? 2 : v >= 2 ? 1 : 0];
+      }
+      fprintf(stderr,
+              "[geauxdock]   apl %d packs %d block %d LDS %zu B tile %d (split 1/2/4/8: %d/%d/%d/%d)\n",
+              lc.apl, lc.end - lc.begin, lc.block, lc.lds, lc.tile, nsp[0], nsp[1], nsp[2],
+              nsp[3]);
+    }
   }
   return GD_OK;
 }
@@ -904,8 +985,9 @@ const void *mc_kernel_fn(int apl, int da, bool tiled) {
 // Launch every class of a pack set, the classes concurrently on side streams
 // forked from and joined to `stream`. With `da`, the delayed-acceptance
 // instances run; `da_scratch` adds the per-wave LDS scratch of the direct
-// network evaluation (a class whose LDS would overflow then runs with one
-// lane group per pack: same packs, same state).
+// network evaluation (a class whose LDS would overflow then runs each pack as
+// one lane group: the same packs and decisions' inputs, with the pair sums in
+// one group's order).
 // Per-class kernel arguments of a launch (persistent grids when `queue`).
 struct ClassLaunch {
   const void *f;
@@ -929,9 +1011,9 @@ hipError_t plan_classes(gd_ctx *ctx, const std::vector<LaunchClass> &classes, co
     if (da && da_scratch) {
       P.off_da = (int)((cl.lds + 15) & ~(size_t)15);
       cl.lds = P.off_da + (size_t)(cl.block / 64) * 6208;
-      if (cl.lds > ctx->max_lds && P.n_split > 1) {
-        P.n_split = 1;
-        cl.block = P.half;
+      if (cl.lds > ctx->max_lds && lc.half_max < cl.block) {
+        P.unsplit = 1;
+        cl.block = lc.half_max;
         cl.lds = P.off_da + (size_t)(cl.block / 64) * 6208;
       }
       if (cl.lds > ctx->max_lds) return hipErrorInvalidValue;
@@ -1350,27 +1432,34 @@ static gd_status eval_poses(gd_ctx *ctx, int cid, int n, const int *idx_lig, con
       return fail(ctx, GD_ERR_INVALID, "pose %d: conformer index out of range", i);
   HIPCHK(hipSetDevice(ctx->device));
   if (!no_alloc) TRY(ensure_ref(ctx));
-  PackSet ps;
-  const int per = std::max(1, 256 / h.lpr);
-  for (int p = 0; p < h.d.n_prt; ++p) {
-    std::vector<int> mine;
-    for (int i = 0; i < n; ++i)
-      if (idx_prt[i] == p) mine.push_back(i);
-    for (size_t b = 0; b < mine.size(); b += per) {
-      Pack pk;
-      pk.cx = cid;
-      pk.prt = p;
-      pk.n_slots = (int)std::min<size_t>(per, mine.size() - b);
-      pk.n_groups = 0;
-      pk.slot_off = (int)ps.slots.size();
-      for (int j = 0; j < pk.n_slots; ++j) ps.slots.push_back(mine[b + j]);
-      ps.packs.push_back(pk);
-    }
+  // each pose is evaluated with the receptor split of its ligand conformer's
+  // MC packs, so a pose's energy has the bits an MC step gives it
+  std::vector<int> lig_ns = h.lig_ns;
+  if (lig_ns.size() != (size_t)h.d.n_lig) {
+    lig_ns.assign(h.d.n_lig, 1);
+    for (const LadderGroup &lg : ladder_groups(ctx, h, auto_gmax(ctx)))
+      for (int g = 0; g < lg.ng; ++g) lig_ns[lg.l0 + g] = lg.ns;
   }
-  // pose packs use the MC launch's receptor split (or the option's) so the
-  // same reduction order is exercised; packs hold at most 256 lanes
-  const int ns = ctx->state_alloc ? ctx->n_split : (ctx->opts.receptor_split == 2 ? 2 : 1);
-  pack_finish(ctx, ps, ns);
+  PackSet ps;
+  for (int p = 0; p < h.d.n_prt; ++p)
+    for (int ns = 1; ns <= 8; ns *= 2) {
+      std::vector<int> mine;
+      for (int i = 0; i < n; ++i)
+        if (idx_prt[i] == p && lig_ns[idx_lig[i]] == ns) mine.push_back(i);
+      const int per = std::max(1, (512 / ns) / h.lpr);
+      for (size_t b = 0; b < mine.size(); b += per) {
+        Pack pk{};
+        pk.cx = cid;
+        pk.prt = p;
+        pk.n_slots = (int)std::min<size_t>(per, mine.size() - b);
+        pk.n_groups = 0;
+        pk.ns = ns;
+        pk.slot_off = (int)ps.slots.size();
+        for (int j = 0; j < pk.n_slots; ++j) ps.slots.push_back(mine[b + j]);
+        ps.packs.push_back(pk);
+      }
+    }
+  pack_finish(ctx, ps);
   for (const LaunchClass &lc : ps.classes)
     if (lc.lds > ctx->max_lds) return fail(ctx, GD_ERR_CAPACITY, "LDS %zu > %zu", lc.lds, ctx->max_lds);
   KParams P{};

@@ -68,9 +68,13 @@ struct DevComplex {
 
 // One workgroup's work: replicas (or poses) of one complex that share one
 // protein conformer; slots are ordered ladder by ladder, temperature slot
-// fastest, so slots [g*n_tmp, (g+1)*n_tmp) form exchange ladder g.
+// fastest, so slots [g*n_tmp, (g+1)*n_tmp) form exchange ladder g. The
+// pack's receptor split: `ns` lane groups of `half` lanes (a multiple of 64)
+// each evaluate all of the pack's replicas over their share of every receptor
+// chunk.
 struct Pack {
   int cx, prt, n_slots, n_groups, slot_off;
+  int ns, half;
 };
 
 // MODE_SUREVAL: surrogate total of given poses (delayed-acceptance kernels)
@@ -106,10 +110,10 @@ struct KParams {
   uint32_t k0, k1;
   // LDS layout (bytes)
   int off_hpp, off_psp, off_par, off_kde, off_kr, off_ch, off_xe, off_xi, off_xa, off_xo;
-  // receptor split: n_split (1 or 2) groups of `half` lanes each evaluate the
-  // same replicas over their share of every receptor chunk and exchange
-  // partial sums through LDS at off_xp (two parity buffers of xp_floats)
-  int n_split, half, off_xp, xp_floats, xp_slots;
+  // receptor split (per pack): the groups exchange partial sums through LDS
+  // at off_xp (two parity buffers of xp_floats); `unsplit` runs every pack as
+  // one group of the whole workgroup
+  int off_xp, xp_floats, unsplit;
   int off_da;  // per-wave surrogate scratch (direct network evaluation)
   int tile;    // receptor points per LDS tile of tiled classes (0: receptor resident in LDS)
   int sur_tab;  // 1: surrogate from the tabulated network outputs

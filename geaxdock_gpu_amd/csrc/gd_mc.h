@@ -48,9 +48,10 @@ struct Lds {
 };
 
 // Receptor split of one workgroup: lanes [h*half, (h+1)*half) form group h
-// of n_split; every group holds the same replicas and covers its share of
-// each receptor chunk. `par` selects the LDS parity buffer of the next
-// exchange (each exchange flips it, so one barrier per exchange suffices).
+// of ns (1, 2, 4 or 8, per pack); every group holds the same replicas and
+// covers its share of each receptor chunk. `par` selects the LDS parity
+// buffer of the next exchange (each exchange flips it, so one barrier per
+// exchange suffices).
 struct Split {
   int h, ns, half, ltid, slot, slots;
   int xp_floats;
@@ -294,11 +295,11 @@ __device__ __forceinline__ float mcs_sum(const DevComplex &C, int lpr, const flo
 // CombineEnergy_d + CalcRmsd_d + CalcMcc_d). Every lane of the replica ends
 // with the same bit pattern in E.
 //
-// With a receptor split (sp.ns == 2) every thread of the workgroup must call
+// With a receptor split (sp.ns > 1) every thread of the workgroup must call
 // this together: `live` (wave-uniform) says whether the wave has a pose to
 // evaluate; the groups exchange their partial sums through LDS behind one
-// workgroup barrier. Two-operand float sums are order-free, so both groups
-// end with identical bits.
+// workgroup barrier and add them in group order, so every group ends with
+// identical bits (with two groups this is the order-free g0 + g1).
 //
 // TILE: the receptor does not stay resident in LDS; its sorted points stream
 // through two LDS tile buffers of S.tile points. While a tile is evaluated
@@ -483,35 +484,49 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
   t_cnt = seg_sum_i(t_cnt, lpr);
   if (sp.ns > 1) {
     // exchange: per-atom HPC sums (the restraint is nonlinear in them) and
-    // the replica's linear sums
+    // the replica's linear sums. Every group adds the ns partials in group
+    // order, so all groups end with the same bits.
     float *buf = S.xp + sp.par * sp.xp_floats;
-    float *hb = buf;                                // [n_split*half][4]
-    float *rb = buf + (size_t)sp.ns * sp.half * 4;  // [n_split][slots][8]
+    float *hb = buf;                                // [ns*half][4]
+    float *rb = buf + (size_t)sp.ns * sp.half * 4;  // [ns][slots][8]
     if (live) {
+      float hg[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int k = 0; k < APL; ++k) hb[threadIdx.x * 4 + k] = hpc[k];
+      for (int k = 0; k < APL; ++k) hg[k] = hpc[k];
+      *reinterpret_cast<float4 *>(hb + (size_t)threadIdx.x * 4) = make_float4(hg[0], hg[1], hg[2], hg[3]);
       if ((sp.ltid - sp.slot * lpr) == 0) {
-        float *r = rb + ((size_t)sp.h * sp.slots + sp.slot) * 8;
-        r[0] = t_vdw;
-        r[1] = t_ele;
-        r[2] = t_pmf;
-        r[3] = t_psp;
-        r[4] = t_hdb;
-        r[5] = __int_as_float(t_cnt);
+        float4 *r = reinterpret_cast<float4 *>(rb + ((size_t)sp.h * sp.slots + sp.slot) * 8);
+        r[0] = make_float4(t_vdw, t_ele, t_pmf, t_psp);
+        r[1] = make_float4(t_hdb, __int_as_float(t_cnt), 0.f, 0.f);
       }
     }
     __syncthreads();
     if (live) {
-      const int other = threadIdx.x + (sp.h == 0 ? sp.half : -sp.half);
+      for (int g = 0; g < sp.ns; ++g) {
+        const float4 hv = *reinterpret_cast<const float4 *>(hb + ((size_t)g * sp.half + sp.ltid) * 4);
+        const float4 *r = reinterpret_cast<const float4 *>(rb + ((size_t)g * sp.slots + sp.slot) * 8);
+        const float4 r0 = r[0], r1 = r[1];
+        const float hg[4] = {hv.x, hv.y, hv.z, hv.w};
+        if (g == 0) {
 #pragma unroll
-      for (int k = 0; k < APL; ++k) hpc[k] += hb[other * 4 + k];
-      const float *r = rb + ((size_t)(1 - sp.h) * sp.slots + sp.slot) * 8;
-      t_vdw += r[0];
-      t_ele += r[1];
-      t_pmf += r[2];
-      t_psp += r[3];
-      t_hdb += r[4];
-      t_cnt += __float_as_int(r[5]);
+          for (int k = 0; k < APL; ++k) hpc[k] = hg[k];
+          t_vdw = r0.x;
+          t_ele = r0.y;
+          t_pmf = r0.z;
+          t_psp = r0.w;
+          t_hdb = r1.x;
+          t_cnt = __float_as_int(r1.y);
+        } else {
+#pragma unroll
+          for (int k = 0; k < APL; ++k) hpc[k] += hg[k];
+          t_vdw += r0.x;
+          t_ele += r0.y;
+          t_pmf += r0.z;
+          t_psp += r0.w;
+          t_hdb += r1.x;
+          t_cnt += __float_as_int(r1.y);
+        }
+      }
     }
     sp.par ^= 1;
   }
@@ -782,12 +797,12 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
                          int lpr, int step0) {
   const int tid = threadIdx.x;
   Split sp;
-  sp.ns = P.n_split;
-  sp.half = P.half;
+  sp.ns = P.unsplit ? 1 : pk.ns;
+  sp.half = P.unsplit ? (int)blockDim.x : pk.half;
   // half is a multiple of 64, so the group index is wave-uniform
-  sp.h = __builtin_amdgcn_readfirstlane(tid >= sp.half ? 1 : 0);
+  sp.h = __builtin_amdgcn_readfirstlane(tid / sp.half);
   sp.ltid = tid - sp.h * sp.half;
-  sp.slots = P.xp_slots;
+  sp.slots = sp.half / 8;
   sp.xp_floats = P.xp_floats;
   sp.par = 0;
   const int slot = sp.ltid / lpr;
@@ -795,7 +810,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   const int a = sp.ltid - slot * lpr;
   const bool lead = a == 0 && sp.h == 0;  // writes the replica's results
   const int n_slots = pk.n_slots;
-  const bool active = slot < n_slots;
+  const bool active = slot < n_slots && sp.h < sp.ns;
   const int sl = active ? slot : 0;
   const int sv = P.slots[pk.slot_off + sl];  // replica id or pose index
   const int lna = C.lna;

@@ -162,8 +162,11 @@ typedef struct gd_opts {
   int lanes_per_replica; /* 0 = auto; else 8, 16, 32 or 64 */
   int atoms_per_lane;    /* 0 = auto; else 1..4 */
   int verbose;
-  int receptor_split;    /* 0 = auto; 1 = one lane group per replica; 2 = two lane
-                            groups per replica, each over half of every receptor chunk */
+  int receptor_split;    /* lane groups per replica, each over its share of every
+                            receptor chunk: 1, 2, 4 or 8 for every workgroup; 0 = auto
+                            per workgroup: whole ladders first, then the split that fills
+                            the workgroup to 8 waves (ladders left over from an odd count
+                            take 4 or 8 groups) */
   int ladders_per_pack;  /* 0 = auto; else temperature ladders per workgroup (replicas
                             per workgroup = this x n_tmp), capped by 512 lanes */
   int launch;            /* GD_LAUNCH_* flags; 0 = graph-captured, non-persistent */
@@ -233,7 +236,10 @@ gd_status gd_restore(gd_ctx *ctx, const void *buf, size_t size);
 /* Parity hook: energies of n poses of one complex, through the same device
  * code as the MC kernel (CalcEnergy_d + CalcRmsd_d + CalcMcc_d,
  * src/kernel_cuda_l2_calcenergy.cu:19-360). movematrix is [n*6]. Needs the
- * complex's reference contact map (built here on first use). */
+ * complex's reference contact map (built here on first use). Each pose is
+ * summed with the receptor split of its ligand conformer's MC workgroups, so
+ * an MC step and this call give a pose the same bits (before the first
+ * gd_run: the split the complexes added so far would get). */
 gd_status gd_calc_energy(gd_ctx *ctx, int complex_id, int n, const int *idx_lig,
                          const int *idx_prt, const float *movematrix, gd_energy *out);
 

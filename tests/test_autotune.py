@@ -5,8 +5,8 @@ from geaxdock_gpu_amd import abi, autotune, synth
 def test_shape_grid_and_tile_stage():
     cxs = synth.uneven_batch(2, seed=1, lna_range=(12, 20), pnp_range=(800, 900))
     grid = autotune.candidates(cxs)
-    assert len(grid) == 2 * 5 * 2
-    assert {c["receptor_split"] for c in grid} == {1, 2}
+    assert len(grid) == 3 * 5 * 2
+    assert {c["receptor_split"] for c in grid} == {0, 1, 2}
     assert {c["launch"] for c in grid} == {0, abi.LAUNCH_PERSISTENT}
     assert all(c["receptor_tile"] == 0 for c in grid)
     assert len(autotune.candidates(cxs, full=True)) == len(grid) + 3

@@ -13,7 +13,7 @@ def test_autotune_shapes_then_tiles():
     import geaxdock_gpu_amd as gd
     cxs = synth.uneven_batch(3, seed=5, lna_range=(12, 40), pnp_range=(800, 1600))
     shapes = [dict(receptor_split=s, ladders_per_pack=0, launch=l, lanes_per_replica=0,
-                   receptor_tile=0) for s in (1, 2) for l in (0, abi.LAUNCH_PERSISTENT)]
+                   receptor_tile=0) for s in (0, 1, 2) for l in (0, abi.LAUNCH_PERSISTENT)]
     best, trials = autotune.tune(cxs, steps=8, warmup=4, budget_s=300, shapes=shapes)
     assert len(trials) == len(shapes) + len(autotune.TILES)
     tiles = [t["opts"]["receptor_tile"] for t in trials[len(shapes):]]

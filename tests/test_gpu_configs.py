@@ -94,7 +94,7 @@ def test_batch_256_full_size(gd):
     cxs = synth.uneven_batch(256, seed=2)
     n_rep = sum(c.n_rep for c in cxs)
     _say(f"\nbatch: {len(cxs)} complexes, {n_rep} replicas")
-    shape = dict(receptor_split=2, ladders_per_pack=3)
+    shape = dict(receptor_split=0, ladders_per_pack=0)
     states = {}
     for launch in (abi.LAUNCH_PERSISTENT, abi.LAUNCH_NO_GRAPH):
         recs = {c: [] for c in range(len(cxs))}
@@ -128,8 +128,8 @@ def test_batch_256_full_size(gd):
 
 def test_surrogate_training_2m_poses(gd):
     cxs = [synth.ensemble(seed=1)] + synth.uneven_batch(256, seed=2)
-    with gd.Engine(seed=11, record=False, steps_per_dump=100, receptor_split=2,
-                   ladders_per_pack=3, launch=abi.LAUNCH_PERSISTENT) as eng:
+    with gd.Engine(seed=11, record=False, steps_per_dump=100, receptor_split=0,
+                   ladders_per_pack=0, launch=abi.LAUNCH_PERSISTENT) as eng:
         for cx in cxs:
             eng.add_complex(cx)
         eng.run(40, 20)
@@ -154,7 +154,7 @@ def test_da_1024_complexes(gd):
     cxs = synth.uneven_batch(1024, seed=2)
     n_rep = sum(c.n_rep for c in cxs)
     _say(f"\nDA batch: {len(cxs)} complexes, {n_rep} replicas")
-    shape = dict(receptor_split=2, ladders_per_pack=3, launch=abi.LAUNCH_PERSISTENT)
+    shape = dict(receptor_split=0, ladders_per_pack=0, launch=abi.LAUNCH_PERSISTENT)
     params = sg.pack_params(sg.init_params(3))
     finals = []
     for dump in (100, 40):

@@ -59,7 +59,8 @@ def check_energy_parity(oracle, eng, cid, cx, mm, idx_lig, idx_prt, report):
 
 
 @pytest.mark.parametrize("lpr,apl,split", [(8, 4, 1), (8, 4, 2), (16, 2, 1), (16, 3, 1),
-                                           (16, 3, 2), (32, 1, 2), (64, 1, 1)])
+                                           (16, 3, 2), (32, 1, 2), (64, 1, 1), (16, 2, 4),
+                                           (8, 4, 8)])
 def test_energy_terms_vs_fp64(oracle, eng_mod, lpr, apl, split):
     cx = synth.single_complex(seed=3, identity_norm=True, void_radius=7.0)
     rng = np.random.default_rng(11)
@@ -202,14 +203,14 @@ def _run_gpu(eng_mod, cx, seed, n_steps, nc, **kw):
     return recs, state, st
 
 
-@pytest.mark.parametrize("split", [1, 2])
+@pytest.mark.parametrize("split", [1, 2, 4])
 def test_mc_run_tracks_cpu_path(oracle, eng_mod, split):
     """Same seed, same random stream: the GPU engine and the FP32 CPU path make
     the same accept / exchange decisions, step for step. A replica may diverge
     only where the CPU path's own margin log shows a decision of it or its
     ladder within FP32 rounding of its threshold (oracle/decisions.py); every
-    other replica's counters and temperature index must be equal. Both
-    receptor-split layouts."""
+    other replica's counters and temperature index must be equal. One, two
+    and four receptor-split groups."""
     from oracle import decisions
     cx = synth.single_complex(seed=0)
     n_steps, nc = 400, 20
@@ -274,7 +275,7 @@ def test_launch_modes_bit_identical(eng_mod):
                 assert np.array_equal(a[k], b[k]), (launch, k)
 
 
-@pytest.mark.parametrize("split,ladders", [(1, 0), (2, 1), (1, 3)])
+@pytest.mark.parametrize("split,ladders", [(1, 0), (2, 1), (1, 3), (0, 0), (4, 2), (8, 1)])
 def test_batch_tracks_cpu_path(oracle, eng_mod, split, ladders):
     """An uneven batch in one engine: every complex follows the FP32 CPU path
     of that complex (its own Philox streams, keyed by complex id), decision
