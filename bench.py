@@ -273,8 +273,10 @@ def roofline(pairs_per_step, mc_steps, t_compute, launches, kname):
             "flops_per_launch": flops / max(1, launches), "flops": flops,
             "note": "FP32 vector (VALU) bound, not MFMA; 36 flops per ligand-receptor pair "
                     "per replica-step; peak is the packed-FP32 157.3 TFLOP/s, "
-                    "frac_unpacked_fp32 uses 78.6; traffic: see profiles/ (HBM is not "
-                    "the bound: a few MB per launch)"}
+                    "frac_unpacked_fp32 uses 78.6; traffic: not counted live; the "
+                    "counter profile of the batch workload's plain launches measures "
+                    "6.3 MB of HBM traffic per MC step (0.5 GB/s), so HBM is not the "
+                    "bound"}
 
 
 def launch_options(args, cxs):
