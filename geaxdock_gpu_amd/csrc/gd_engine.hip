@@ -119,6 +119,7 @@ struct gd_ctx {
     // work buffers
     float *y = nullptr, *dS = nullptr, *sq = nullptr, *S = nullptr;
     float *p1 = nullptr, *p2 = nullptr, *pb2 = nullptr, *pw3 = nullptr;
+    float *dyr = nullptr;  // per-row output gradient of the current batch
     int64_t work_rows = 0;
     int last_groups = 0;  // backward workgroups (partials) of the last batch
   } sur;
@@ -267,7 +268,8 @@ void gd_destroy(gd_ctx *ctx) {
     for (float *g : S.tab)
       if (g) (void)hipFree(g);
     void *qs[] = {S.p, S.m, S.v, S.grad, S.W1T, S.W2T, S.W2R, S.X, S.pose_of_row,
-                  S.d_off, S.d_target, S.y, S.dS, S.sq, S.S, S.p1, S.p2, S.pb2, S.pw3};
+                  S.d_off, S.d_target, S.y, S.dS, S.sq, S.S, S.p1, S.p2, S.pb2, S.pw3,
+                  S.dyr};
     for (void *p : qs)
       if (p) (void)hipFree(p);
   }
@@ -1704,7 +1706,7 @@ gd_status sur_upload_index(gd_ctx *ctx) {
 }
 
 // Workgroups of the batch kernels: the fused backward runs one per CU (its
-// LDS tiles take 106 KB), the forward two; each walks 64-row steps.
+// LDS tiles take 124 KB), the forward two; each walks 64-row steps.
 int sur_train_groups(const gd_ctx *ctx, int64_t rows) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(rows / 64, ctx->n_cu));
 }
@@ -1718,10 +1720,11 @@ gd_status sur_work(gd_ctx *ctx, int64_t rows, int64_t n_pose_total) {
   auto &S = ctx->sur;
   rows = (rows + 63) / 64 * 64 + 64;
   if (rows > S.work_rows) {
-    void *qs[] = {S.p1, S.p2, S.pb2, S.pw3};
+    void *qs[] = {S.p1, S.p2, S.pb2, S.pw3, S.dyr};
     for (void *p : qs)
       if (p) (void)hipFree(p);
     const int64_t ng = std::max(1, ctx->n_cu);
+    HIPCHK(hipMalloc(&S.dyr, (size_t)rows * sizeof(float)));
     HIPCHK(hipMalloc(&S.p1, (size_t)ng * SUR_F * SUR_H * sizeof(float)));
     HIPCHK(hipMalloc(&S.p2, (size_t)ng * SUR_H * SUR_H * sizeof(float)));
     HIPCHK(hipMalloc(&S.pb2, (size_t)ng * SUR_H * sizeof(float)));
@@ -1757,9 +1760,10 @@ gd_status sur_batch(gd_ctx *ctx, int64_t p0, int64_t p1, bool grad, double *loss
                      S.d_off, (int)p0, (int)p1, S.d_target, S.dS, S.sq, S.S);
   if (grad) {
     const int ng = sur_train_groups(ctx, nr);
+    hipLaunchKernelGGL(gd_mlp_dyrow_kernel, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0,
+                       ctx->stream, S.pose_of_row + r0, nr, (int)p0, (int)p1, S.dS, S.dyr);
     hipLaunchKernelGGL(gd_mlp_train_kernel, dim3(ng), dim3(256), SUR_TRAIN_LDS, ctx->stream, X,
-                       nr, S.pose_of_row + r0, (int)p0, (int)p1, S.dS, W, S.p1, S.p2, S.pb2,
-                       S.pw3);
+                       nr, S.dyr, W, S.p1, S.p2, S.pb2, S.pw3);
     S.last_groups = ng;
     if (!update)
       hipLaunchKernelGGL(gd_mlp_gradsum_kernel, dim3((SUR_NPARAM + 255) / 256), dim3(256), 0,

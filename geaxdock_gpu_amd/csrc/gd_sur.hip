@@ -98,19 +98,28 @@ __device__ inline void load_colw(const SurW &W, int wv, bool with_w2r, ColW &cw,
   }
 }
 
-// X rows [r0, r0 + 64) into xs (row-major) and, when xt is given, xt
-// (transposed, [64 features][SUR_TR]). All 256 threads.
-__device__ inline void stage_x(const unsigned short *X, int64_t r0, unsigned short *xs,
-                               unsigned short *xt) {
+// X rows [r0, r0 + 64) as 512 pieces of 8 bf16, two per thread (all 256
+// threads): read into registers, then written to xs (row-major) and, when xt
+// is given, xt (transposed, [64 features][SUR_TR]).
+__device__ inline void load_x(const unsigned short *X, int64_t r0, uint4 (&v)[2]) {
   const int t = threadIdx.x;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    const int e = t + 256 * h;       // 512 pieces of 8 bf16
+    const int e = t + 256 * h;
     const int r = e >> 3, c8 = (e & 7) * 8;
-    const uint4 v = *reinterpret_cast<const uint4 *>(X + (r0 + r) * SUR_F + c8);
-    *reinterpret_cast<uint4 *>(xs + r * SUR_XS + c8) = v;
+    v[h] = *reinterpret_cast<const uint4 *>(X + (r0 + r) * SUR_F + c8);
+  }
+}
+
+__device__ inline void put_x(const uint4 (&v)[2], unsigned short *xs, unsigned short *xt) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int e = t + 256 * h;
+    const int r = e >> 3, c8 = (e & 7) * 8;
+    *reinterpret_cast<uint4 *>(xs + r * SUR_XS + c8) = v[h];
     if (xt) {
-      const unsigned int w[4] = {v.x, v.y, v.z, v.w};
+      const unsigned int w[4] = {v[h].x, v[h].y, v[h].z, v[h].w};
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         xt[(c8 + 2 * q) * SUR_TR + r] = (unsigned short)(w[q] & 0xFFFFu);
@@ -118,6 +127,13 @@ __device__ inline void stage_x(const unsigned short *X, int64_t r0, unsigned sho
       }
     }
   }
+}
+
+__device__ inline void stage_x(const unsigned short *X, int64_t r0, unsigned short *xs,
+                               unsigned short *xt) {
+  uint4 v[2];
+  load_x(X, r0, v);
+  put_x(v, xs, xt);
 }
 
 // Layer 1 of the wave's column slice: h1[rt][c] = C tile (rows 16 rt.., cols
@@ -279,6 +295,17 @@ extern "C" __global__ void __launch_bounds__(256) gd_sur_tab_kernel(const float 
     }
 }
 
+// Output gradient of each row of a batch: dyr[i] = dS of row i's pose when
+// that pose lies in [p0, p1), else 0 (rows of the 64-row padding that belong
+// to the next poses). One thread per row.
+extern "C" __global__ void gd_mlp_dyrow_kernel(const int *pose_of_row, int64_t n_rows, int p0,
+                                               int p1, const float *dS, float *dyr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_rows) return;
+  const int p = pose_of_row[i];
+  dyr[i] = (p >= p0 && p < p1) ? dS[p] : 0.0f;
+}
+
 // Pose sums over poses [p0, p1): S = sum of row outputs, residual against the
 // target, dS = 2 (S - target) / (p1 - p0), squared residuals for the loss.
 // Row offsets are absolute; y is indexed by absolute row.
@@ -299,7 +326,7 @@ extern "C" __global__ void gd_mlp_pose_kernel(const float *y, const int64_t *off
 // gd_mlp_fwd_kernel. Workgroup b takes the contiguous 64-row steps
 // [b S / G, (b + 1) S / G) (S steps, G = gridDim.x). Per step, wave w
 // recomputes H1 and H2 of its 32 columns, forms
-//   dy = dS of the row's pose,  dH2 = dy w3 * silu'(H2),
+//   dy = dyr of the row (gd_mlp_dyrow_kernel),  dH2 = dy w3 * silu'(H2),
 //   dA1 = dH2 W2^T (all of dH2 from LDS, W2^T slice in registers),
 //   dH1 = dA1 * silu'(H1)
 // and accumulates its 32 columns of
@@ -308,14 +335,16 @@ extern "C" __global__ void gd_mlp_pose_kernel(const float *y, const int64_t *off
 // in MFMA accumulators, with b2, w3 and b3. Only fixed-order partials go to
 // HBM: p1g[b][64][128], p2g[b][128][128], pb2[b][128], pw3[b][129]
 // (summed in a fixed order by gd_mlp_update_kernel or gd_mlp_gradsum_kernel,
-// so training is deterministic).
+// so training is deterministic). The next step's X rows are read into
+// registers while a step is evaluated, and a step's dy rows are read at its
+// start, so neither load stalls the step.
 extern "C" __global__ void __launch_bounds__(256) gd_mlp_train_kernel(
-    const unsigned short *X, int64_t n_rows, const int *pose_of_row, int p0, int p1,
-    const float *dS, SurW W, float *p1g, float *p2g, float *pb2, float *pw3) {
+    const unsigned short *X, int64_t n_rows, const float *dyr, SurW W, float *p1g, float *p2g,
+    float *pb2, float *pw3) {
   extern __shared__ __attribute__((aligned(16))) unsigned char sm_train[];
-  unsigned short *xs = reinterpret_cast<unsigned short *>(sm_train);  // X [64][SUR_XS]
-  unsigned short *xt = xs + SUR_STEP * SUR_XS;                          // X^T [64][SUR_TR]
-  unsigned short *a1s = xt + SUR_F * SUR_TR;                            // A1 [64][SUR_AS]
+  // X [64][SUR_XS] and X^T [64][SUR_TR], two of each (even and odd steps)
+  unsigned short *xb = reinterpret_cast<unsigned short *>(sm_train);
+  unsigned short *a1s = xb + 2 * (SUR_STEP * SUR_XS + SUR_F * SUR_TR);  // A1 [64][SUR_AS]
   unsigned short *a1t = a1s + SUR_STEP * SUR_AS;                        // A1^T [128][SUR_TR]
   unsigned short *d2s = a1t + SUR_H * SUR_TR;                           // dH2 [64][SUR_AS]
   unsigned short *d2t = d2s + SUR_STEP * SUR_AS;                        // dH2^T [128][SUR_TR]
@@ -336,9 +365,19 @@ extern "C" __global__ void __launch_bounds__(256) gd_mlp_train_kernel(
   const int64_t n_steps = n_rows / SUR_STEP;
   const int64_t s_beg = n_steps * blockIdx.x / gridDim.x;
   const int64_t s_end = n_steps * (blockIdx.x + 1) / gridDim.x;
+  uint4 xv[2];
+  if (s_beg < s_end) load_x(X, s_beg * SUR_STEP, xv);
   for (int64_t st = s_beg; st < s_end; ++st) {
     const int64_t r0 = st * SUR_STEP;
-    stage_x(X, r0, xs, xt);
+    unsigned short *xs = xb + (st & 1) * (SUR_STEP * SUR_XS + SUR_F * SUR_TR);
+    unsigned short *xt = xs + SUR_STEP * SUR_XS;
+    // dy of the rows 16 rt + 4 lq + i
+    float4 dyv[4];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+      dyv[rt] = *reinterpret_cast<const float4 *>(dyr + r0 + 16 * rt + 4 * lq);
+    put_x(xv, xs, xt);
+    if (st + 1 < s_end) load_x(X, r0 + SUR_STEP, xv);
     __syncthreads();
     f32x4_t h1[4][2], h2[4][2];
     col_layer1(xs, cw, h1);
@@ -357,12 +396,12 @@ extern "C" __global__ void __launch_bounds__(256) gd_mlp_train_kernel(
     // dy of the rows 16 rt + 4 lq + i; w3, b2, b3 gradients
     float dy[4][4];
 #pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int p = pose_of_row[r0 + 16 * rt + 4 * lq + i];
-        dy[rt][i] = (p >= p0 && p < p1) ? dS[p] : 0.0f;
-      }
+    for (int rt = 0; rt < 4; ++rt) {
+      dy[rt][0] = dyv[rt].x;
+      dy[rt][1] = dyv[rt].y;
+      dy[rt][2] = dyv[rt].z;
+      dy[rt][3] = dyv[rt].w;
+    }
     if (wv == 0 && lr == 0)
 #pragma unroll
       for (int rt = 0; rt < 4; ++rt) gb3 += (dy[rt][0] + dy[rt][1]) + (dy[rt][2] + dy[rt][3]);
@@ -429,7 +468,8 @@ extern "C" __global__ void __launch_bounds__(256) gd_mlp_train_kernel(
         g1[t][1] = mfma32(a, b1v[1], g1[t][1]);
       }
     }
-    __syncthreads();  // the tiles are rewritten by the next step
+    // no barrier here: the next step writes the other X buffer, and its
+    // first barrier precedes every write to the shared A1 and dH2 tiles
   }
 
   // partials of this workgroup: its 4 waves own disjoint columns
@@ -467,7 +507,7 @@ extern "C" __global__ void __launch_bounds__(256) gd_mlp_train_kernel(
 
 // Dynamic LDS bytes of gd_mlp_train_kernel.
 constexpr int SUR_TRAIN_LDS =
-    (SUR_STEP * SUR_XS + SUR_F * SUR_TR + 2 * SUR_STEP * SUR_AS + 3 * SUR_H * SUR_TR) * 2;
+    (2 * (SUR_STEP * SUR_XS + SUR_F * SUR_TR) + 2 * SUR_STEP * SUR_AS + 3 * SUR_H * SUR_TR) * 2;
 
 // Sum of the squared residuals of poses [0, n) into *out: one workgroup,
 // strided partial sums then a fixed-shape tree (deterministic).
