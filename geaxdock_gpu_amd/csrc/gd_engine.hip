@@ -1034,9 +1034,14 @@ hipError_t plan_classes(gd_ctx *ctx, const std::vector<LaunchClass> &classes, co
       P.n_queue = n_pk;
     }
     cl.f = mc_kernel_fn(lc.apl, da, lc.tile > 0);
-    const hipError_t e = hipFuncSetAttribute(cl.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cl.lds);
-    if (e != hipSuccess) return e;
     out.push_back(cl);
+  }
+  // the dynamic LDS limit of every instance used is the device's: classes
+  // (and pose-evaluation launches) of different sizes share instances
+  for (const ClassLaunch &cl : out) {
+    const hipError_t e =
+        hipFuncSetAttribute(cl.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->max_lds);
+    if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }

@@ -133,6 +133,37 @@ def test_start_pose_mcc_is_one(oracle, eng_mod):
     assert float(out[0]["rmsd"]) == 0.0
 
 
+def test_mcs_templates_on_the_start_pose(oracle, eng_mod):
+    """Every MCS template atom placed exactly on the start pose (its FP32
+    coordinates): the MCS term is log(0) = -inf, as in the reference
+    (src/kernel_cuda_l2_calcenergy.cu:341), and so is the weighted total, on the
+    device as on the FP32 CPU path, which places the pose with the same FP32
+    additions. (The FP64 oracle places it exactly and sees the FP32 rounding
+    of the template, a finite log.) A replica holding -inf accepts no move
+    (exp of -inf is 0), so an MC run from the start poses keeps every replica
+    there."""
+    cx = synth.single_complex(seed=4)
+    start = np.stack([cx.lig_x[:cx.lna], cx.lig_y[:cx.lna], cx.lig_z[:cx.lna]], 1) + \
+        np.asarray(cx.pocket_center, np.float32)
+    for arr, i in ((cx.mcs_x, 0), (cx.mcs_y, 1), (cx.mcs_z, 2)):
+        t = arr.reshape(cx.pos, cx.n_mcs)
+        valid = t != abi.MCS_INVALID_COORD
+        for m in range(cx.pos):
+            for a in np.nonzero(valid[m, 1:])[0]:
+                t[m, a + 1] = start[a, i]
+    d = cx.desc()
+    want = oracle.energy_f32(d, oracle.refmap_f32(d, cx), 0, 0, np.zeros(6, np.float32))
+    assert want.e[7] == -np.inf and want.e[9] == -np.inf
+    with eng_mod.Engine(seed=3, record=False) as eng:
+        cid = eng.add_complex(cx)
+        out = eng.calc_energy(cid, [0], [0], np.zeros((1, 6), np.float32))
+        assert out[0]["e"][7] == -np.inf and out[0]["e"][9] == -np.inf
+        eng.run(40, 20)
+        st = eng.state(cid)
+    assert np.all(st["acs_mc"] == 0)
+    assert np.all(st["movematrix"] == 0) and np.all(st["energy"]["e"][:, 9] == -np.inf)
+
+
 def test_philox_matches_oracle(oracle, eng_mod):
     rng = np.random.default_rng(0)
     ctr = rng.integers(0, 2**32, size=(257, 4), dtype=np.uint64).astype(np.uint32)

@@ -312,6 +312,20 @@ def test_mcs_uniform_offset(oracle):
     assert e64(oracle, cx).raw[7] == pytest.approx(want, rel=1e-6)
 
 
+def test_mcs_template_on_the_pose_gives_log_zero(oracle):
+    """Every template atom exactly on the pose: the reference's log of the
+    mean template distance is log(0) = -inf (src/kernel_cuda_l2_calcenergy.cu:341),
+    kept as is; with a positive weight the total is -inf too."""
+    lig = [[0, 0, 0], [1.5, 0, 0], [1.5, 1.5, 0]]
+    pts = {a: np.array(lig[a]) for a in range(3)}
+    cx = tiny_complex(lig, [[20, 0, 0]], mcs=[(0.6, pts), (0.2, {1: np.array(lig[1])})])
+    o = e64(oracle, cx)
+    assert o.raw[7] == -math.inf and o.e[7] == -math.inf and o.e[9] == -math.inf
+    d = cx.desc()
+    f = oracle.energy_f32(d, oracle.refmap_f32(d, cx), 0, 0, np.zeros(6, np.float32))
+    assert f.e[7] == -math.inf and f.e[9] == -math.inf
+
+
 def test_dst_rmsd_and_rigid_placement(oracle):
     rng = np.random.default_rng(2)
     lig = rng.normal(0, 2, (6, 3))
