@@ -337,6 +337,49 @@ def test_batch_tracks_cpu_path(oracle, eng_mod, split, ladders):
     print(f"\nsplit {split} ladders {ladders}: {same}/{total} replicas with identical decisions")
 
 
+@pytest.mark.parametrize("split,ladders", [(0, 0), (0, 4), (1, 0)])
+def test_odd_ladder_shapes_track_cpu_path(oracle, eng_mod, split, ladders):
+    """Ladders of 1 to 5 waves (3, 6, 12, 20 and 40 temperatures; 8 or 16 lanes
+    per replica) and odd ligand-conformer counts, so the automatic split makes
+    packs of 1, 2 and 4 ladders at splits 1 to 8, with 3- and 5-wave groups:
+    every complex follows the FP32 CPU path decision for decision (up to
+    explained rounding splits), records equal gd_calc_energy of their poses bit
+    for bit, and graph-captured persistent and eager launches agree."""
+    from oracle import decisions
+    shapes = [(3, 20, 5), (6, 40, 3), (12, 36, 3), (20, 14, 2), (40, 12, 1)]
+    cxs = [synth.make_complex(50 + i, lna=lna, pnp=700 + 150 * i, n_lig=n_lig, n_prt=1,
+                              n_tmp=n_tmp)
+           for i, (n_tmp, lna, n_lig) in enumerate(shapes)]
+    n_steps, nc = 120, 20
+    states = {}
+    for launch in (abi.LAUNCH_PERSISTENT, abi.LAUNCH_NO_GRAPH):
+        recs = {c: [] for c in range(len(cxs))}
+        with eng_mod.Engine(seed=9, receptor_split=split, ladders_per_pack=ladders,
+                            launch=launch, steps_per_dump=60) as eng:
+            for cx in cxs:
+                eng.add_complex(cx)
+            eng.run(n_steps, nc, sink=lambda c, r: recs[c].append(r))
+            states[launch] = [eng.state(c) for c in range(len(cxs))]
+            if launch != abi.LAUNCH_PERSISTENT:
+                continue
+            for c, cx in enumerate(cxs):
+                r = np.concatenate(recs[c])
+                ex = eng.calc_energy(c, r["replica"][:, 3], r["replica"][:, 1], r["movematrix"])
+                assert np.array_equal(ex["e"].view(np.uint32), r["energy"]["e"].view(np.uint32))
+                d = cx.desc()
+                cpu = oracle.McState(cx, rec_cap=n_steps + 1)
+                cpu.enable_log(0, n_steps, n_steps // nc)
+                oracle.mc_run(d, cpu, seed=9, cid=c, init=True, step0=0, n_steps=0, nc=nc)
+                oracle.mc_run(d, cpu, seed=9, cid=c, init=False, step0=0, n_steps=n_steps, nc=nc)
+                out = decisions.compare(cx, r, cpu, nc, gpu_state=states[launch][c])
+                print(f"\nsplit {split} ladders {ladders} complex {c} (n_tmp {cx.n_tmp}, "
+                      f"lna {cx.lna}): "
+                      f"{out['identical']}/{cx.n_rep} identical, explained {out['explained']}")
+    for a, b in zip(states[abi.LAUNCH_PERSISTENT], states[abi.LAUNCH_NO_GRAPH]):
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+
+
 def test_statistics_match_cpu_path_independent_seeds(oracle, eng_mod):
     """Independent chains (different seeds) of the GPU engine and the FP32 CPU
     path on one complex, exchanges off: at every temperature the per-replica
