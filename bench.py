@@ -88,6 +88,9 @@ def parse(argv=None):
     ap.add_argument("--mode", default="da", choices=["plain", "da", "post"],
                     help="plain vs delayed acceptance (default), plain engine only, or post-MC")
     ap.add_argument("--no-record", action="store_true", help="plain mode: keep no records")
+    ap.add_argument("--drain", action="store_true",
+                    help="plain mode: copy every window's records to a host sink (counted), so "
+                         "the rate includes the device-to-host transfers")
     ap.add_argument("--k-sub", type=int, default=4, help="surrogate steps per exact evaluation")
     ap.add_argument("--log-snapshots", type=int, default=32,
                     help="state snapshots taken over the warm-up for training")
@@ -625,13 +628,22 @@ def plain_bench(args, cxs, desc):
     opts, tuned = launch_options(args, cxs)
     eng = gd.Engine(seed=0, record=not args.no_record, steps_per_dump=args.steps_per_dump,
                     verbose=args.verbose, **opts)
+    t0 = time.perf_counter()
     for cx in cxs:
         eng.add_complex(cx)
     n_rep = sum(c.n_rep for c in cxs)
     if mc_warm > 0:
         eng.run(mc_warm, args.nc)
+    t_setup = time.perf_counter() - t0  # host-to-device upload, packing, warm-up
+    drained = [0, 0]
+
+    def count(cid, recs):
+        drained[0] += len(recs)
+        drained[1] += recs.nbytes
+
+    sink = count if (args.drain and not args.no_record) else None
     t0 = time.perf_counter()
-    st = eng.run(mc_steps, args.nc)
+    st = eng.run(mc_steps, args.nc, sink=sink)
     dt = time.perf_counter() - t0
     pairs_per_step = sum(c.n_rep * c.lna * c.pnp for c in cxs)
     roof = roofline(pairs_per_step, mc_steps, st["t_compute"], st["kernel_launches"],
@@ -659,7 +671,12 @@ def plain_bench(args, cxs, desc):
         "exchange_accepts": st["exchanges_accepted"],
         "t_compute_s": st["t_compute"],
         "kernel_replica_steps_per_s": n_rep * mc_steps / st["t_compute"],
+        "setup_s": t_setup,
     }
+    if sink is not None:
+        out["drained"] = {"records": drained[0], "bytes": drained[1],
+                          "note": "every window's records copied to the host inside the timed "
+                                  "region (value includes the transfers)"}
     eng.close()
     return out
 

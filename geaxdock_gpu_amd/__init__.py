@@ -122,6 +122,14 @@ def _u(a):
     return a.ctypes.data_as(C.POINTER(C.c_uint32))
 
 
+def _records(ptr, n):
+    """Copy of n gd_record entries at ptr (the library's buffer is reused after
+    the sink returns)."""
+    raw = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_uint8)),
+                                shape=(n * abi.RECORD_DTYPE.itemsize,))
+    return raw.view(abi.RECORD_DTYPE).copy()
+
+
 class Engine:
     """One engine context on one device (gd_create ... gd_destroy)."""
 
@@ -188,8 +196,7 @@ class Engine:
             cb = abi.RECORD_SINK()
         else:
             def _cb(user, cid, recs, n):
-                buf = C.cast(recs, C.POINTER(C.c_char * (n * abi.RECORD_DTYPE.itemsize))).contents
-                sink(cid, np.frombuffer(bytes(buf), dtype=abi.RECORD_DTYPE).copy())
+                sink(cid, _records(recs, n))
             cb = abi.RECORD_SINK(_cb)
         self._check(self.lib.gd_run(self._ctx, int(n_steps), int(steps_per_exchange), cb, None,
                                     C.byref(st)))
@@ -202,8 +209,7 @@ class Engine:
             cb = abi.RECORD_SINK()
         else:
             def _cb(user, cid, recs, n):
-                buf = C.cast(recs, C.POINTER(C.c_char * (n * abi.RECORD_DTYPE.itemsize))).contents
-                sink(cid, np.frombuffer(bytes(buf), dtype=abi.RECORD_DTYPE).copy())
+                sink(cid, _records(recs, n))
             cb = abi.RECORD_SINK(_cb)
         self._check(self.lib.gd_run_da(self._ctx, int(n_steps), int(steps_per_exchange),
                                        int(k_sub), float(sur_scale), cb, None, C.byref(st)))

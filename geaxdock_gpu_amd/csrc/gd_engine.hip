@@ -82,6 +82,11 @@ struct gd_ctx {
   gd_record *d_rec = nullptr;
   int *d_rec_cnt = nullptr;
   int rec_cap = 0;
+  // record drains: device offsets and compacted window, and the pinned host
+  // staging buffer the sink reads, grown as needed and reused across windows
+  int64_t *d_drain_off = nullptr;
+  gd_record *d_drain = nullptr, *h_drain = nullptr;
+  int64_t drain_cap = 0;
   // packs of the MC launch
   Pack *d_packs = nullptr;
   int *d_slots = nullptr;
@@ -257,7 +262,9 @@ void gd_destroy(gd_ctx *ctx) {
     for (void *p : h.bufs) (void)hipFree(p);
   void *ps[] = {ctx->d_cx, ctx->d_overflow, ctx->d_mm, ctx->d_eold, ctx->d_idx_tmp,
                 ctx->d_acs_mc, ctx->d_acs_x, ctx->d_rep_flat, ctx->d_rec, ctx->d_rec_cnt,
-                ctx->d_packs, ctx->d_slots, ctx->d_best_e, ctx->d_best_mm, ctx->d_ctrl};
+                ctx->d_packs, ctx->d_slots, ctx->d_best_e, ctx->d_best_mm, ctx->d_ctrl,
+                ctx->d_drain_off, ctx->d_drain};
+  if (ctx->h_drain) (void)hipHostFree(ctx->h_drain);
   if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
   for (void *p : ps)
     if (p) (void)hipFree(p);
@@ -1085,30 +1092,37 @@ hipError_t launch_classes(gd_ctx *ctx, const std::vector<LaunchClass> &classes, 
 gd_status drain_records(gd_ctx *ctx, gd_record_sink sink, void *user) {
   if (!ctx->d_rec) return GD_OK;
   const int n = ctx->n_rep;
-  std::vector<int> cnt(n);
-  HIPCHK(hipMemcpyAsync(cnt.data(), ctx->d_rec_cnt, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
   if (sink) {
+    std::vector<int> cnt(n);
+    HIPCHK(hipMemcpyAsync(cnt.data(), ctx->d_rec_cnt, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     std::vector<int64_t> off(n + 1, 0);
     for (int i = 0; i < n; ++i) off[i + 1] = off[i] + cnt[i];
     const int64_t tot = off[n];
     if (tot > 0) {
-      int64_t *d_off = nullptr;
-      gd_record *d_out = nullptr;
-      HIPCHK(hipMalloc(&d_off, (n + 1) * sizeof(int64_t)));
-      HIPCHK(hipMalloc(&d_out, tot * sizeof(gd_record)));
-      HIPCHK(hipMemcpyAsync(d_off, off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+      if (tot > ctx->drain_cap) {
+        if (ctx->d_drain) (void)hipFree(ctx->d_drain);
+        if (ctx->h_drain) (void)hipHostFree(ctx->h_drain);
+        ctx->d_drain = nullptr;
+        ctx->h_drain = nullptr;
+        ctx->drain_cap = 0;
+        const int64_t cap = tot + tot / 8;
+        HIPCHK(hipMalloc(&ctx->d_drain, cap * sizeof(gd_record)));
+        HIPCHK(hipHostMalloc(&ctx->h_drain, cap * sizeof(gd_record), hipHostMallocDefault));
+        ctx->drain_cap = cap;
+      }
+      if (!ctx->d_drain_off) HIPCHK(hipMalloc(&ctx->d_drain_off, (n + 1) * sizeof(int64_t)));
+      HIPCHK(hipMemcpyAsync(ctx->d_drain_off, off.data(), (n + 1) * sizeof(int64_t),
+                            hipMemcpyHostToDevice, ctx->stream));
       hipLaunchKernelGGL(gd_compact_kernel, dim3(n), dim3(256), 0, ctx->stream, n, ctx->d_rec,
-                         ctx->d_rec_cnt, d_off, ctx->rec_cap, d_out);
-      std::vector<gd_record> host(tot);
-      HIPCHK(hipMemcpyAsync(host.data(), d_out, tot * sizeof(gd_record), hipMemcpyDeviceToHost, ctx->stream));
+                         ctx->d_rec_cnt, ctx->d_drain_off, ctx->rec_cap, ctx->d_drain);
+      HIPCHK(hipMemcpyAsync(ctx->h_drain, ctx->d_drain, tot * sizeof(gd_record),
+                            hipMemcpyDeviceToHost, ctx->stream));
       HIPCHK(hipStreamSynchronize(ctx->stream));
-      (void)hipFree(d_off);
-      (void)hipFree(d_out);
       for (int c = 0; c < (int)ctx->cx.size(); ++c) {
         const HostComplex &h = ctx->cx[c];
         const int64_t b = off[h.rep_base], e = off[h.rep_base + h.n_rep];
-        if (e > b) sink(user, c, host.data() + b, e - b);
+        if (e > b) sink(user, c, ctx->h_drain + b, e - b);
       }
     }
   }

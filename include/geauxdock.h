@@ -195,7 +195,8 @@ typedef struct gd_run_stats {
 } gd_run_stats;
 
 /* Record sink: called once per dump window and complex with that window's
- * accepted-state records, ordered by replica then step. */
+ * accepted-state records, ordered by replica then step. recs is valid until
+ * the sink returns (the library reuses its pinned staging buffer). */
 typedef void (*gd_record_sink)(void *user, int complex_id, const gd_record *recs, int64_t n);
 
 typedef struct gd_ctx gd_ctx;
