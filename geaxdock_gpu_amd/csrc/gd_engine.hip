@@ -79,9 +79,16 @@ struct gd_ctx {
   gd_energy *d_eold = nullptr;
   float *d_best_e = nullptr, *d_best_mm = nullptr;
   int *d_idx_tmp = nullptr, *d_acs_mc = nullptr, *d_acs_x = nullptr, *d_rep_flat = nullptr;
-  gd_record *d_rec = nullptr;
+  // record rings: the MC kernels append accepted states to ring `ring`; with
+  // a sink, windows alternate between two rings so one window's records are
+  // drained while the next window runs (ring 1 is allocated on first use)
+  gd_record *d_rec = nullptr;  // ring 0
   int *d_rec_cnt = nullptr;
+  gd_record *d_ring[2] = {nullptr, nullptr};
+  int *d_ring_cnt[2] = {nullptr, nullptr};
+  int ring = 0;
   int rec_cap = 0;
+  hipStream_t drain_stream = nullptr;
   // record drains: device offsets and compacted window, and the pinned host
   // staging buffer the sink reads, grown as needed and reused across windows
   int64_t *d_drain_off = nullptr;
@@ -96,11 +103,13 @@ struct gd_ctx {
   // counter of class i in persistent launches
   int *d_ctrl = nullptr;
   int h_step = 0;  // host source of the step upload (outlives the async copy)
-  // captured MC launch (one graph per window shape)
-  hipGraphExec_t graph = nullptr;
-  bool graph_valid = false;
-  int g_mode = -1, g_win = 0, g_nc = 0, g_da = 0, g_tab = 0, g_persistent = 0;
-  float g_scale = 0.0f;
+  // captured MC launch per record ring (one graph per window shape)
+  struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    bool valid = false;
+    int mode = -1, win = 0, nc = 0, da = 0, tab = 0, persistent = 0;
+    float scale = 0.0f;
+  } gslot[2];
   // learned surrogate
   struct Sur {
     std::vector<float *> grid;  // per complex [n_prt][n^3][32]
@@ -240,6 +249,7 @@ gd_status gd_create(gd_ctx **out, const gd_opts *opts) {
             prop.gcnArchName, prop.multiProcessorCount, (size_t)prop.sharedMemPerBlock,
             (size_t)prop.maxSharedMemoryPerMultiProcessor, prop.regsPerBlock);
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&ctx->drain_stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&ctx->ev0));
   HIPCHK(hipEventCreate(&ctx->ev1));
   HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
@@ -263,9 +273,10 @@ void gd_destroy(gd_ctx *ctx) {
   void *ps[] = {ctx->d_cx, ctx->d_overflow, ctx->d_mm, ctx->d_eold, ctx->d_idx_tmp,
                 ctx->d_acs_mc, ctx->d_acs_x, ctx->d_rep_flat, ctx->d_rec, ctx->d_rec_cnt,
                 ctx->d_packs, ctx->d_slots, ctx->d_best_e, ctx->d_best_mm, ctx->d_ctrl,
-                ctx->d_drain_off, ctx->d_drain};
+                ctx->d_drain_off, ctx->d_drain, ctx->d_ring[1], ctx->d_ring_cnt[1]};
   if (ctx->h_drain) (void)hipHostFree(ctx->h_drain);
-  if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+  for (auto &g : ctx->gslot)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   for (void *p : ps)
     if (p) (void)hipFree(p);
   {
@@ -287,6 +298,7 @@ void gd_destroy(gd_ctx *ctx) {
     if (ctx->ev_join[i]) (void)hipEventDestroy(ctx->ev_join[i]);
     if (ctx->aux[i]) (void)hipStreamDestroy(ctx->aux[i]);
   }
+  if (ctx->drain_stream) (void)hipStreamDestroy(ctx->drain_stream);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -590,7 +602,7 @@ gd_status sync_complexes(gd_ctx *ctx) {
   if (!ctx->cx_dirty) return GD_OK;
   if (ctx->d_cx) HIPCHK(hipFree(ctx->d_cx));
   ctx->d_cx = nullptr;
-  ctx->graph_valid = false;  // captured launches hold the old pointer
+  for (auto &g : ctx->gslot) g.valid = false;  // captured launches hold the old pointer
   std::vector<DevComplex> v;
   for (auto &h : ctx->cx) v.push_back(h.dev);
   HIPCHK(hipMalloc(&ctx->d_cx, std::max<size_t>(v.size(), 1) * sizeof(DevComplex)));
@@ -895,6 +907,7 @@ gd_status alloc_state(gd_ctx *ctx) {
   HIPCHK(hipMalloc(&ctx->d_acs_x, (size_t)n * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_rep_flat, (size_t)n * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_rec_cnt, (size_t)n * sizeof(int)));
+  ctx->d_ring_cnt[0] = ctx->d_rec_cnt;
   std::vector<int> flat(n), itmp(n);
   for (auto &h : ctx->cx)
     for (int f = 0; f < h.n_rep; ++f) {
@@ -911,6 +924,7 @@ gd_status alloc_state(gd_ctx *ctx) {
   if (ctx->opts.record) {
     ctx->rec_cap = ctx->opts.steps_per_dump + 1;
     HIPCHK(hipMalloc(&ctx->d_rec, (size_t)n * ctx->rec_cap * sizeof(gd_record)));
+    ctx->d_ring[0] = ctx->d_rec;
   }
   // receptor split: automatic (0, per pack) or fixed
   const int ns = ctx->opts.receptor_split;
@@ -972,8 +986,8 @@ KParams base_params(gd_ctx *ctx) {
   P.acs_mc = ctx->d_acs_mc;
   P.acs_x = ctx->d_acs_x;
   P.rep_flat = ctx->d_rep_flat;
-  P.rec = ctx->d_rec;
-  P.rec_cnt = ctx->d_rec_cnt;
+  P.rec = ctx->d_ring[ctx->ring];
+  P.rec_cnt = ctx->d_ring_cnt[ctx->ring];
   P.rec_cap = ctx->rec_cap;
   P.k0 = (uint32_t)ctx->opts.seed;
   P.k1 = (uint32_t)(ctx->opts.seed >> 32);
@@ -1089,13 +1103,18 @@ hipError_t launch_classes(gd_ctx *ctx, const std::vector<LaunchClass> &classes, 
   return e;
 }
 
-gd_status drain_records(gd_ctx *ctx, gd_record_sink sink, void *user) {
-  if (!ctx->d_rec) return GD_OK;
+// Drain record ring r: its accepted-state records, compacted on the device
+// into one window, copied to the pinned staging buffer and handed to the
+// sink per complex; then the ring's counts restart. All on `stream`, which
+// is synchronized before returning.
+gd_status drain_ring(gd_ctx *ctx, int r, gd_record_sink sink, void *user, hipStream_t stream) {
+  if (!ctx->d_ring[r]) return GD_OK;
   const int n = ctx->n_rep;
+  int *cnt_d = ctx->d_ring_cnt[r];
   if (sink) {
     std::vector<int> cnt(n);
-    HIPCHK(hipMemcpyAsync(cnt.data(), ctx->d_rec_cnt, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpyAsync(cnt.data(), cnt_d, n * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
     std::vector<int64_t> off(n + 1, 0);
     for (int i = 0; i < n; ++i) off[i + 1] = off[i] + cnt[i];
     const int64_t tot = off[n];
@@ -1113,12 +1132,13 @@ gd_status drain_records(gd_ctx *ctx, gd_record_sink sink, void *user) {
       }
       if (!ctx->d_drain_off) HIPCHK(hipMalloc(&ctx->d_drain_off, (n + 1) * sizeof(int64_t)));
       HIPCHK(hipMemcpyAsync(ctx->d_drain_off, off.data(), (n + 1) * sizeof(int64_t),
-                            hipMemcpyHostToDevice, ctx->stream));
-      hipLaunchKernelGGL(gd_compact_kernel, dim3(n), dim3(256), 0, ctx->stream, n, ctx->d_rec,
-                         ctx->d_rec_cnt, ctx->d_drain_off, ctx->rec_cap, ctx->d_drain);
+                            hipMemcpyHostToDevice, stream));
+      hipLaunchKernelGGL(gd_compact_kernel, dim3(n), dim3(256), 0, stream, n, ctx->d_ring[r],
+                         cnt_d, ctx->d_drain_off, ctx->rec_cap, ctx->d_drain);
+      HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(ctx->h_drain, ctx->d_drain, tot * sizeof(gd_record),
-                            hipMemcpyDeviceToHost, ctx->stream));
-      HIPCHK(hipStreamSynchronize(ctx->stream));
+                            hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
       for (int c = 0; c < (int)ctx->cx.size(); ++c) {
         const HostComplex &h = ctx->cx[c];
         const int64_t b = off[h.rep_base], e = off[h.rep_base + h.n_rep];
@@ -1126,12 +1146,30 @@ gd_status drain_records(gd_ctx *ctx, gd_record_sink sink, void *user) {
       }
     }
   }
-  HIPCHK(hipMemsetAsync(ctx->d_rec_cnt, 0, n * sizeof(int), ctx->stream));
+  HIPCHK(hipMemsetAsync(cnt_d, 0, n * sizeof(int), stream));
+  HIPCHK(hipStreamSynchronize(stream));
   return GD_OK;
 }
 
-gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float *ms,
-                    int da_k = 0, float sur_scale = 1.0f) {
+gd_status drain_records(gd_ctx *ctx, gd_record_sink sink, void *user) {
+  return drain_ring(ctx, ctx->ring, sink, user, ctx->stream);
+}
+
+// Second record ring, allocated the first time a run drains to a sink.
+gd_status ensure_ring1(gd_ctx *ctx) {
+  if (ctx->d_ring[1] || !ctx->d_rec) return GD_OK;
+  const int n = ctx->n_rep;
+  HIPCHK(hipMalloc(&ctx->d_ring[1], (size_t)n * ctx->rec_cap * sizeof(gd_record)));
+  HIPCHK(hipMalloc(&ctx->d_ring_cnt[1], (size_t)n * sizeof(int)));
+  HIPCHK(hipMemsetAsync(ctx->d_ring_cnt[1], 0, (size_t)n * sizeof(int), ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GD_OK;
+}
+
+// Enqueue one MC launch (mode, window of n_steps from step0) on ctx->stream
+// between the timing events ev0 and ev1; launch_mc_wait waits for it.
+gd_status launch_mc_enqueue(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, int da_k = 0,
+                            float sur_scale = 1.0f) {
   KParams P = base_params(ctx);
   P.mode = mode;
   P.step0 = step0;
@@ -1157,13 +1195,15 @@ gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float
   if (use_graph) {
     // the first step comes from device memory so one graph serves every window
     P.step_dev = ctx->d_ctrl;
-    const bool same = ctx->graph_valid && ctx->g_mode == mode && ctx->g_win == n_steps &&
-                      ctx->g_nc == nc && ctx->g_da == P.da_k && ctx->g_tab == P.sur_tab &&
-                      ctx->g_scale == P.sur_scale && ctx->g_persistent == (int)persistent;
+    // one graph slot per record ring (the ring's pointers are kernel arguments)
+    auto &G = ctx->gslot[ctx->ring];
+    const bool same = G.valid && G.mode == mode && G.win == n_steps && G.nc == nc &&
+                      G.da == P.da_k && G.tab == P.sur_tab && G.scale == P.sur_scale &&
+                      G.persistent == (int)persistent;
     if (!same) {
-      if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
-      ctx->graph = nullptr;
-      ctx->graph_valid = false;
+      if (G.exec) (void)hipGraphExecDestroy(G.exec);
+      G.exec = nullptr;
+      G.valid = false;
       std::vector<ClassLaunch> cls;
       hipError_t e = plan_classes(ctx, ctx->mc_classes, P, da, scratch, queue, cls);
       if (e == hipErrorInvalidValue)
@@ -1175,22 +1215,22 @@ gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float
       const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
       HIPCHK(e);
       HIPCHK(e2);
-      e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+      e = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       HIPCHK(e);
-      ctx->graph_valid = true;
-      ctx->g_mode = mode;
-      ctx->g_win = n_steps;
-      ctx->g_nc = nc;
-      ctx->g_da = P.da_k;
-      ctx->g_tab = P.sur_tab;
-      ctx->g_scale = P.sur_scale;
-      ctx->g_persistent = (int)persistent;
+      G.valid = true;
+      G.mode = mode;
+      G.win = n_steps;
+      G.nc = nc;
+      G.da = P.da_k;
+      G.tab = P.sur_tab;
+      G.scale = P.sur_scale;
+      G.persistent = (int)persistent;
     }
     ctx->h_step = step0;
     HIPCHK(hipMemcpyAsync(ctx->d_ctrl, &ctx->h_step, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(hipGraphLaunch(ctx->graph, ctx->stream));
+    HIPCHK(hipGraphLaunch(G.exec, ctx->stream));
   } else {
     std::vector<ClassLaunch> cls;
     hipError_t e = plan_classes(ctx, ctx->mc_classes, P, da, scratch, queue, cls);
@@ -1202,9 +1242,19 @@ gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  return GD_OK;
+}
+
+gd_status launch_mc_wait(gd_ctx *ctx, float *ms) {
   HIPCHK(hipEventSynchronize(ctx->ev1));
   if (ms) HIPCHK(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
   return GD_OK;
+}
+
+gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float *ms,
+                    int da_k = 0, float sur_scale = 1.0f) {
+  TRY(launch_mc_enqueue(ctx, mode, step0, n_steps, nc, da_k, sur_scale));
+  return launch_mc_wait(ctx, ms);
 }
 
 }  // namespace
@@ -1252,17 +1302,31 @@ static gd_status run_impl(gd_ctx *ctx, int64_t n_steps, int steps_per_exchange,
   }
   int64_t left = n_steps;
   double t_ms = 0.0;
+  // with records and a sink, windows alternate between the two rings: the
+  // previous window's records are drained (on the drain stream, sinks called
+  // here) while the next window runs
+  const bool overlap = sink && ctx->d_rec && n_steps > 0;
+  if (overlap) TRY(ensure_ring1(ctx));
+  int pending = -1;
   while (left > 0) {
     int win = (int)std::min<int64_t>(left, ctx->opts.steps_per_dump);
     if (da_k > 0) win = std::max(da_k, win / da_k * da_k);
     float ms = 0.f;
-    TRY(launch_mc(ctx, MODE_MC, (int)ctx->step, win, steps_per_exchange, &ms, da_k, sur_scale));
+    TRY(launch_mc_enqueue(ctx, MODE_MC, (int)ctx->step, win, steps_per_exchange, da_k, sur_scale));
+    if (pending >= 0) TRY(drain_ring(ctx, pending, sink, user, ctx->drain_stream));
+    TRY(launch_mc_wait(ctx, &ms));
     t_ms += ms;
     st.kernel_launches += 1;
     ctx->step += win;
     left -= win;
-    TRY(drain_records(ctx, sink, user));
+    if (overlap) {
+      pending = ctx->ring;
+      ctx->ring ^= 1;
+    } else {
+      TRY(drain_records(ctx, sink, user));
+    }
   }
+  if (pending >= 0) TRY(drain_ring(ctx, pending, sink, user, ctx->stream));
   if (n_steps == 0) TRY(drain_records(ctx, sink, user));
   std::vector<int> acs1(ctx->n_rep), acx1(ctx->n_rep);
   HIPCHK(hipMemcpyAsync(acs1.data(), ctx->d_acs_mc, ctx->n_rep * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -1395,7 +1459,8 @@ gd_status gd_restore(gd_ctx *ctx, const void *buf, size_t size) {
   HIPCHK(hipMemcpyAsync(ctx->d_best_e, o, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   o += n * sizeof(float);
   HIPCHK(hipMemcpyAsync(ctx->d_best_mm, o, n * 6 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(hipMemsetAsync(ctx->d_rec_cnt, 0, n * sizeof(int), ctx->stream));
+  for (int r = 0; r < 2; ++r)
+    if (ctx->d_ring_cnt[r]) HIPCHK(hipMemsetAsync(ctx->d_ring_cnt[r], 0, n * sizeof(int), ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->initialized = hdr[2] != 0;
   ctx->step = (int64_t)hdr[4] | ((int64_t)hdr[5] << 32);
