@@ -1313,12 +1313,15 @@ static gd_status run_impl(gd_ctx *ctx, int64_t n_steps, int steps_per_exchange,
     if (da_k > 0) win = std::max(da_k, win / da_k * da_k);
     float ms = 0.f;
     TRY(launch_mc_enqueue(ctx, MODE_MC, (int)ctx->step, win, steps_per_exchange, da_k, sur_scale));
-    if (pending >= 0) TRY(drain_ring(ctx, pending, sink, user, ctx->drain_stream));
+    // a failed drain is reported after the window in flight has finished
+    const gd_status ds =
+        pending >= 0 ? drain_ring(ctx, pending, sink, user, ctx->drain_stream) : GD_OK;
     TRY(launch_mc_wait(ctx, &ms));
     t_ms += ms;
     st.kernel_launches += 1;
     ctx->step += win;
     left -= win;
+    if (ds != GD_OK) return ds;
     if (overlap) {
       pending = ctx->ring;
       ctx->ring ^= 1;
