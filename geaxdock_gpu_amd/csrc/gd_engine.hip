@@ -1793,12 +1793,13 @@ gd_status sur_upload_index(gd_ctx *ctx) {
 }
 
 // Workgroups of the batch kernels: the fused backward runs one per CU (its
-// LDS tiles take 124 KB), the forward two; each walks 64-row steps.
+// LDS tiles take 124 KB), the forward three (its registers allow three waves
+// per SIMD); each walks 64-row steps.
 int sur_train_groups(const gd_ctx *ctx, int64_t rows) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(rows / 64, ctx->n_cu));
 }
 int sur_fwd_groups(const gd_ctx *ctx, int64_t rows) {
-  return (int)std::max<int64_t>(1, std::min<int64_t>(rows / 64, 2 * (int64_t)ctx->n_cu));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(rows / 64, 3 * (int64_t)ctx->n_cu));
 }
 
 // work buffers for batches of up to `rows` rows (rounded up to 64): the

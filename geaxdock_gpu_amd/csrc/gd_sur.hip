@@ -197,7 +197,8 @@ __device__ inline void put_tiles(const float (&v)[4][2][4], int wv, unsigned sho
 }
 
 // Per-row outputs of a batch: y[r] = w3 . silu(H2[r]) + b3. Workgroup b
-// takes 64-row steps b, b + G, ... (G = gridDim.x).
+// takes 64-row steps b, b + G, ... (G = gridDim.x); the next step's X rows
+// are read into registers while a step is evaluated.
 extern "C" __global__ void __launch_bounds__(256) gd_mlp_fwd_kernel(const unsigned short *X,
                                                                     int64_t n_rows, SurW W,
                                                                     float *y) {
@@ -210,9 +211,12 @@ extern "C" __global__ void __launch_bounds__(256) gd_mlp_fwd_kernel(const unsign
   load_colw(W, wv, false, cw, unused);
   const float b3 = W.b3[0];
   const int64_t n_steps = n_rows / SUR_STEP;
+  uint4 xv[2];
+  if (blockIdx.x < n_steps) load_x(X, (int64_t)blockIdx.x * SUR_STEP, xv);
   for (int64_t st = blockIdx.x; st < n_steps; st += gridDim.x) {
     const int64_t r0 = st * SUR_STEP;
-    stage_x(X, r0, xs, nullptr);
+    put_x(xv, xs, nullptr);
+    if (st + gridDim.x < n_steps) load_x(X, (st + gridDim.x) * SUR_STEP, xv);
     __syncthreads();
     f32x4_t h1[4][2], h2[4][2];
     col_layer1(xs, cw, h1);
