@@ -34,6 +34,9 @@ per ligand-atom x receptor-point pair per replica-step (distance 10, soft LJ
 flops by its summed launch durations (HIP events on the engine's stream around
 each window's launch, all per-APL kernel classes of the window included).
 Peak: MI355X FP32 vector 157.3 TFLOP/s (78.6 without packed FP32).
+`traffic` is the HBM bytes per launch of the plain kernel from the workload's
+FETCH_SIZE/WRITE_SIZE counter profile (PLAIN_TRAFFIC: a per-launch part plus a
+per-MC-step part), null for workloads and launch shapes not profiled.
 
 cpu_baseline: the FP32 CPU restatement (oracle/, OpenMP over replicas) on a
 bounded sample of the same workload, on this host's cores.
@@ -55,6 +58,17 @@ PEAK_BF16 = 2.5e15
 SUR_FWD_FLOPS_PER_ROW = 2.0 * (64 * 128 + 128 * 128 + 128)
 SUR_TRAIN_FLOPS_PER_ROW = SUR_FWD_FLOPS_PER_ROW + 2.0 * (128 * 128 + 128 * 128 + 64 * 128)
 METRIC = "MC replica-steps/s, plain vs surrogate-DA, at matched best-pose energy"
+# HBM bytes of one plain MC launch: (read per launch, read per MC step,
+# written per launch, written per MC step). From rocprofv3 FETCH_SIZE and
+# WRITE_SIZE of eager 260- and 500-step launches of the same workload, each
+# scaled by what the counter reads for a copy kernel of known size (2.000 KiB
+# per FETCH_SIZE unit, 1.000 KiB per WRITE_SIZE unit). The per-launch part is
+# every workgroup staging its receptor, tables and KDE points into LDS; the
+# per-step part is L2 misses of MCS templates, contact lists and PSP rows.
+PLAIN_TRAFFIC = {
+    ("batch", 1024): (8.292e8, 4.132e5, 1.576e7, 246.6),
+    ("ensemble", None): (7.90e6, 0.0, 8.01e5, 0.0),
+}
 
 
 def parse(argv=None):
@@ -263,23 +277,40 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def roofline(pairs_per_step, mc_steps, t_compute, launches, kname):
+def traffic_model(args):
+    """The measured HBM traffic of this workload's plain launches, if profiled."""
+    key = (args.config, args.batch if args.config == "batch" else None)
+    if args.config == "single" or args.tile or args.split > 0 or args.ladders > 0:
+        return None
+    return PLAIN_TRAFFIC.get(key)
+
+
+def roofline(pairs_per_step, mc_steps, t_compute, launches, kname, traffic=None):
     """FP32 VALU roofline of an MC run: 36 flops per ligand-atom x
-    receptor-point pair per replica-step, over the summed launch durations."""
+    receptor-point pair per replica-step, over the summed launch durations.
+    traffic: PLAIN_TRAFFIC's model of the launches' HBM bytes, or None."""
     flops = FLOPS_PER_PAIR * pairs_per_step * mc_steps
     ach = flops / t_compute if t_compute > 0 else 0.0
-    return {"bound": "valu", "unit": "TFLOP/s", "achieved": ach / 1e12,
-            "peak": PEAK_FP32 / 1e12, "frac": ach / PEAK_FP32,
-            "frac_unpacked_fp32": ach / PEAK_FP32_UNPACKED, "traffic": None,
-            "kernel": kname, "launches": launches, "mc_steps": mc_steps,
-            "kernel_ms_avg": t_compute * 1e3 / max(1, launches),
-            "flops_per_launch": flops / max(1, launches), "flops": flops,
-            "note": "FP32 vector (VALU) bound, not MFMA; 36 flops per ligand-receptor pair "
-                    "per replica-step; peak is the packed-FP32 157.3 TFLOP/s, "
-                    "frac_unpacked_fp32 uses 78.6; traffic: not counted live; the "
-                    "counter profile of the batch workload's plain launches measures "
-                    "6.3 MB of HBM traffic per MC step (0.5 GB/s), so HBM is not the "
-                    "bound"}
+    n = max(1, launches)
+    out = {"bound": "valu", "unit": "TFLOP/s", "achieved": ach / 1e12,
+           "peak": PEAK_FP32 / 1e12, "frac": ach / PEAK_FP32,
+           "frac_unpacked_fp32": ach / PEAK_FP32_UNPACKED, "traffic": None,
+           "kernel": kname, "launches": launches, "mc_steps": mc_steps,
+           "kernel_ms_avg": t_compute * 1e3 / n,
+           "flops_per_launch": flops / n, "flops": flops,
+           "note": "FP32 vector (VALU) bound, not MFMA; 36 flops per ligand-receptor pair "
+                   "per replica-step; peak is the packed-FP32 157.3 TFLOP/s, "
+                   "frac_unpacked_fp32 uses 78.6"}
+    if traffic is not None:
+        rf, rs, wf, ws = traffic
+        total = n * (rf + wf) + mc_steps * (rs + ws)
+        out["traffic"] = total / n
+        out["hbm_gbs"] = total / t_compute / 1e9 if t_compute > 0 else 0.0
+        out["traffic_note"] = ("HBM bytes per launch from the workload's FETCH_SIZE/WRITE_SIZE "
+                               "profile (%.3g B per launch + %.3g B per MC step), "
+                               "copy-calibrated; far below 8 TB/s, so HBM is not the bound"
+                               % (rf + wf, rs + ws))
+    return out
 
 
 def launch_options(args, cxs):
@@ -486,7 +517,7 @@ def da_compare(args, cxs, desc):
     d_steps = cmp["equal_steps"]["complex_mean"]
     d_time = cmp["equal_wall_time"]["complex_mean"]
     roof = roofline(pairs_per_step, mc_steps, out["plain"]["kernel_seconds"],
-                    out["plain"]["launches"], kname)
+                    out["plain"]["launches"], kname, traffic_model(args))
     # the DA kernel's exact share: one pose evaluation per k surrogate steps
     roof_da = roofline(pairs_per_step / k, mc_steps, out["da"]["kernel_seconds"],
                        out["da"]["launches"], kname_da)
@@ -647,7 +678,7 @@ def plain_bench(args, cxs, desc):
     dt = time.perf_counter() - t0
     pairs_per_step = sum(c.n_rep * c.lna * c.pnp for c in cxs)
     roof = roofline(pairs_per_step, mc_steps, st["t_compute"], st["kernel_launches"],
-                    kernel_names(eng))
+                    kernel_names(eng), traffic_model(args))
     out = {
         "metric": "MC replica-steps/s (plain REMC engine)",
         "value": n_rep * mc_steps / dt,

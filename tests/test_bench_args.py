@@ -54,3 +54,22 @@ def test_exchange_interval_must_hold_whole_groups():
     a = bench.parse(["--k-sub", "8", "--nc", "40"])
     timed, warm, unit = bench.step_plan(a)
     assert unit == 40 and timed % 40 == 0
+
+
+def test_roofline_traffic_only_for_profiled_workloads():
+    """The roofline's HBM traffic comes from the counter profile of the same
+    workload and launch shape; any other workload reports null."""
+    a = bench.parse(["--gpus", "1", "--steps", "20", "--warmup", "5"])
+    model = bench.traffic_model(a)
+    assert model is not None
+    rf, rs, wf, ws = model
+    # 7 windows of 5,000 steps: per launch = the fixed part + steps/launch x per step
+    r = bench.roofline(91744 * 122880, 5000, 60.0, 7, "k", model)
+    assert math.isclose(r["traffic"], rf + wf + 5000 / 7 * (rs + ws), rel_tol=1e-12)
+    assert math.isclose(r["hbm_gbs"], 7 * r["traffic"] / 60.0 / 1e9, rel_tol=1e-12)
+    assert r["bound"] == "valu" and r["hbm_gbs"] < 1.0  # GB/s, against 8 TB/s
+    for argv in (["--config", "batch", "--batch", "256"], ["--config", "single"],
+                 ["--split", "2"], ["--tile", "256"], ["--ladders", "1"]):
+        assert bench.traffic_model(bench.parse(argv)) is None
+        assert bench.roofline(1, 1, 1.0, 1, "k", None)["traffic"] is None
+    assert bench.traffic_model(bench.parse(["--config", "ensemble"])) is not None
