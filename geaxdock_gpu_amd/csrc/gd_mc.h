@@ -70,7 +70,8 @@ __device__ __forceinline__ int seg_sum_i(int v, int lpr) {
 // HPC switching polynomial (src/kernel_cuda_l2_calcenergy.cu:116-121):
 // 1 - (3.5 x - 4.5 x^2 + 2.5 x^3 - 0.5 x^4), x = d^2 / 81, which factors as
 // (1 - x)^3 (1 - x/2) = u^3 (1 + u) / 2 with u = 1 - x. Clamping u at 0 makes
-// it exactly 0 beyond d = 9 (the reference's d^2 <= 81 test), and the factored
+// it exactly 0 beyond d = 9 (the reference's d^2 <= 81 test; u never exceeds
+// 1, so the clamp to [0, 1] of the fma's output modifier does it), and the factored
 // form has no cancellation near the cut-off. The 1/2 is folded into the
 // receptor's hpp values when they are staged into LDS (an exact scaling).
 constexpr float INV81 = 1.0f / 81.0f;
@@ -163,9 +164,11 @@ __device__ __forceinline__ void pair_group(const Lds &S, int p, const float4 (&P
       // whose parameters let it reach 60; elsewhere the clamp is a no-op
       const float wv0 = fmaf(pa, q2, fmaf(cp[k].pb, d, cp[k].pc));
       const float wv = CLAMP ? fminf(wv0, 64.0f) : wv0;
-      const float e1 = 1.0f + fast_exp2(wv);
-      const float r = fast_rcp(den[j][k] * e1);
-      vdw[k] = fmaf(num[j][k] * e1, r, vdw[k]);
+      // den (1 + e) and num (1 + e) as den + den e and num + num e: one
+      // fma each, one rounding each
+      const float e = fast_exp2(wv);
+      const float r = fast_rcp(fmaf(den[j][k], e, den[j][k]));
+      vdw[k] = fmaf(fmaf(num[j][k], e, num[j][k]), r, vdw[k]);
       pmf[k] = fmaf(den[j][k], r, pmf[k]);
       // H-bond: only in chunks where some lane's atom has an H-bond record
       // with this receptor type (elsewhere every term would be 0 * exp2)
@@ -174,7 +177,8 @@ __device__ __forceinline__ void pair_group(const Lds &S, int p, const float4 (&P
         hdb[k] = fmaf(cp[k].hc, fast_exp2(-(h * h)), hdb[k]);
       }
       if (KIND >= KIND_HPC) {
-        const float u = fmaxf(fmaf(q2, -INV81, 1.0f), 0.0f);
+        // u <= 1, so clamping to [0, 1] is max(u, 0): the fma's own clamp
+        const float u = __builtin_amdgcn_fmed3f(fmaf(q2, -INV81, 1.0f), 0.0f, 1.0f);
         const float u3 = u * u * u;
         hpc[k] = fmaf(hp[j], fmaf(u3, u, u3), hpc[k]);
       }

@@ -78,3 +78,51 @@ def test_hbond_gaussian_in_base_2():
         g_ref = np.exp(-((d - hdb0) * hdb1) ** 2 / 2.0)
         g_ker = np.exp2(-(d * ha + hb) ** 2)
         np.testing.assert_allclose(g_ker, g_ref, rtol=1e-9, atol=1e-300)
+
+
+def _fma32(a, b, c):
+    """FP32 fused multiply-add: the exact FP64 product of two FP32 values plus
+    an FP32 addend, rounded once to FP32 (the FP64 sum's own rounding can
+    only matter for addends far below the product's last bit)."""
+    return np.float32(np.float64(a) * np.float64(b) + np.float64(c))
+
+
+def test_shared_reciprocal_with_fused_products():
+    """vdW (num/den) and the PMF factor 1/(1 + e) share one reciprocal
+    r = 1/(den (1 + e)): vdW = num (1 + e) r and 1/(1 + e) = den r. The kernel
+    forms den (1 + e) and num (1 + e) as fma(den, e, den) and fma(num, e, num),
+    one rounding each. Checked in FP32 against FP64 over the range of d
+    (0.5 to 60 A: den up to 60^9 = 1e16) and of the exponent (e up to 2^64,
+    the clamp), with no overflow."""
+    rng = np.random.default_rng(2)
+    f32 = np.float32
+    worst_v, worst_p = 0.0, 0.0
+    for _ in range(20000):
+        d = rng.uniform(0.5, 60.0)
+        p1a, p2a = rng.uniform(1e2, 1e7), rng.uniform(1e1, 1e5)
+        a0, a1 = p1a * rng.uniform(0.5, 1.0), p1a * rng.uniform(0.25, 1.0)
+        e = 2.0 ** rng.uniform(-150.0, 64.0)
+        num = f32(p1a - p2a * d ** 3)
+        den = f32(a0 + a1 * d * d + d ** 9)
+        e32 = f32(e)
+        dene = _fma32(den, e32, den)
+        nume = _fma32(num, e32, num)
+        assert np.isfinite(dene) and np.isfinite(nume)
+        r = f32(1.0) / dene
+        vdw = np.float64(nume) * np.float64(r)
+        pmf = np.float64(den) * np.float64(r)
+        ref_v = np.float64(num) / np.float64(den)
+        ref_p = 1.0 / (1.0 + np.float64(e32))
+        worst_v = max(worst_v, abs(vdw - ref_v) / abs(ref_v))
+        worst_p = max(worst_p, abs(pmf - ref_p) / ref_p)
+    # two FP32 roundings in each form (the fma, the reciprocal): a few ulp
+    assert worst_v < 4 * 2.0 ** -23 and worst_p < 4 * 2.0 ** -23
+
+
+def test_hpc_clamp_is_the_fma_output_clamp():
+    """u = 1 - d^2/81 never exceeds 1 (d^2 >= 0), so clamping it to [0, 1],
+    as the fma's output modifier does, is the switch's max(u, 0)."""
+    q2 = np.concatenate([np.linspace(0.0, 200.0, 20001), [0.0, 81.0, 1e-30, 1e30]])
+    u = 1.0 - q2 / 81.0
+    assert np.all(u <= 1.0)
+    np.testing.assert_array_equal(np.clip(u, 0.0, 1.0), np.maximum(u, 0.0))
