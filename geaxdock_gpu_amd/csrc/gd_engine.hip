@@ -683,7 +683,9 @@ void layout(KParams &P, size_t *bytes, int pnp_max, int pnk_max, int nch_max, in
   o = al(o + (size_t)slots_max * 4);
   P.off_xo = (int)o;
   o = al(o + (size_t)slots_max * 4);
-  P.xp_floats = split ? 5 * block : 0;
+  // split exchange: 4 floats of per-atom HPC sums per lane, 12 floats of
+  // partial sums per replica (at most block/8 replicas): 5.5 floats per lane
+  P.xp_floats = split ? (11 * block) / 2 : 0;
   P.off_xp = (int)o;
   o = al(o + (size_t)2 * P.xp_floats * 4);
   P.off_q = (int)o;

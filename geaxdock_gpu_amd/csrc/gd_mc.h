@@ -255,18 +255,19 @@ __device__ __forceinline__ void chunk_pairs(const Lds &S, int beg, int end, cons
   for (int k = 0; k < APL; ++k) pmf[k] = fmaf(cp[k].pmf1, pmfc[k], pmf[k]);
 }
 
-// Sum over MCS templates of tcc_m * sqrt(mean squared distance of the
-// template's atoms), src/kernel_cuda_l2_calcenergy.cu:274-313. Templates go
+// Sum over MCS templates [m_begin, m_end) of tcc_m * sqrt(mean squared
+// distance of the template's atoms), src/kernel_cuda_l2_calcenergy.cu:274-313. Templates go
 // four at a time: their loads are issued together and their lane sums share
 // each butterfly step, so the per-template latencies overlap. Per template
 // the additions are the same as one template at a time.
 template <int APL>
 __device__ __forceinline__ float mcs_sum(const DevComplex &C, int lpr, const float (&x)[APL],
                                          const float (&y)[APL], const float (&z)[APL],
-                                         const int (&li)[APL], const bool (&val)[APL]) {
+                                         const int (&li)[APL], const bool (&val)[APL],
+                                         int m_begin, int m_end) {
   float elhm = 0.0f;
-  const int pos = C.pos;
-  for (int m0 = 0; m0 < pos; m0 += 4) {
+  const int pos = m_end;
+  for (int m0 = m_begin; m0 < pos; m0 += 4) {
     float s[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -466,10 +467,49 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
     }
   }
 
+  // Per-atom lists that the groups of a receptor split share out the way they
+  // share the receptor's chunks: each group takes its part of the KDE points
+  // of every atom's type, of every atom's reference contacts and of the MCS
+  // templates, and the partial sums join the exchange below. With one group
+  // the parts are the whole lists.
+  const float kde2l = C.kde2l;
+  float t_kde = 0.f, elhm = 0.f;
+  int t_tp = 0;
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < APL; ++k) {
+      if (!val[k]) continue;
+      // KDE: mean over same-type points, src/kernel_cuda_l2_calcenergy.cu:224-253
+      const int2 kr = S.kr[lt[k]];
+      const int kl = kr.y - kr.x;
+      const int kb = kr.x + (kl * sp.h) / sp.ns, ke = kr.x + (kl * (sp.h + 1)) / sp.ns;
+      float ks = 0.0f;
+      for (int j = kb; j < ke; ++j) {
+        const float4 Q = S.kde[j];
+        ks += fast_exp2(kde2l * pair_d2(x[k], y[k], z[k], Q.x, Q.y, Q.z));
+      }
+      if (kl > 0) t_kde += ks / (float)kl;
+      // true positives against the reference contact list
+      const int nr = C.ref_cnt[li[k]];
+      const int2 *rl = C.ref + (size_t)li[k] * KMAX;
+      const int rb0 = (nr * sp.h) / sp.ns, re0 = (nr * (sp.h + 1)) / sp.ns;
+      for (int j = rb0; j < re0; ++j) {
+        const int2 r = rl[j];
+        const float4 P = TILE ? S.gx[r.x] : S.xyze[r.x];
+        const float d2 = pair_d2(x[k], y[k], z[k], P.x, P.y, P.z);
+        const float d = d2 * fast_rsq(d2);
+        t_tp += (d <= __int_as_float(r.y)) ? 1 : 0;
+      }
+    }
+    // MCS restraints, src/kernel_cuda_l2_calcenergy.cu:274-313
+    elhm = mcs_sum<APL>(C, lpr, x, y, z, li, val, (C.pos * sp.h) / sp.ns,
+                        (C.pos * (sp.h + 1)) / sp.ns);
+  }
+
   // pair sums that are linear in the receptor points
   float t_vdw = 0.f, t_ele = 0.f, t_pmf = 0.f, t_psp = 0.f, t_hdb = 0.f, t_hpc = 0.f;
-  float t_kde = 0.f, t_rms = 0.f;
-  int t_cnt = 0, t_tp = 0;
+  float t_rms = 0.f;
+  int t_cnt = 0;
 #pragma unroll
   for (int k = 0; k < APL; ++k) {
     if (!val[k]) continue;
@@ -486,30 +526,33 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
   t_psp = seg_sum(t_psp, lpr);
   t_hdb = seg_sum(t_hdb, lpr);
   t_cnt = seg_sum_i(t_cnt, lpr);
+  t_kde = seg_sum(t_kde, lpr);
+  t_tp = seg_sum_i(t_tp, lpr);
   if (sp.ns > 1) {
     // exchange: per-atom HPC sums (the restraint is nonlinear in them) and
     // the replica's linear sums. Every group adds the ns partials in group
     // order, so all groups end with the same bits.
     float *buf = S.xp + sp.par * sp.xp_floats;
     float *hb = buf;                                // [ns*half][4]
-    float *rb = buf + (size_t)sp.ns * sp.half * 4;  // [ns][slots][8]
+    float *rb = buf + (size_t)sp.ns * sp.half * 4;  // [ns][slots][12]
     if (live) {
       float hg[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int k = 0; k < APL; ++k) hg[k] = hpc[k];
       *reinterpret_cast<float4 *>(hb + (size_t)threadIdx.x * 4) = make_float4(hg[0], hg[1], hg[2], hg[3]);
       if ((sp.ltid - sp.slot * lpr) == 0) {
-        float4 *r = reinterpret_cast<float4 *>(rb + ((size_t)sp.h * sp.slots + sp.slot) * 8);
+        float4 *r = reinterpret_cast<float4 *>(rb + ((size_t)sp.h * sp.slots + sp.slot) * 12);
         r[0] = make_float4(t_vdw, t_ele, t_pmf, t_psp);
-        r[1] = make_float4(t_hdb, __int_as_float(t_cnt), 0.f, 0.f);
+        r[1] = make_float4(t_hdb, __int_as_float(t_cnt), t_kde, __int_as_float(t_tp));
+        r[2] = make_float4(elhm, 0.f, 0.f, 0.f);
       }
     }
     __syncthreads();
     if (live) {
       for (int g = 0; g < sp.ns; ++g) {
         const float4 hv = *reinterpret_cast<const float4 *>(hb + ((size_t)g * sp.half + sp.ltid) * 4);
-        const float4 *r = reinterpret_cast<const float4 *>(rb + ((size_t)g * sp.slots + sp.slot) * 8);
-        const float4 r0 = r[0], r1 = r[1];
+        const float4 *r = reinterpret_cast<const float4 *>(rb + ((size_t)g * sp.slots + sp.slot) * 12);
+        const float4 r0 = r[0], r1 = r[1], r2 = r[2];
         const float hg[4] = {hv.x, hv.y, hv.z, hv.w};
         if (g == 0) {
 #pragma unroll
@@ -520,6 +563,9 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
           t_psp = r0.w;
           t_hdb = r1.x;
           t_cnt = __float_as_int(r1.y);
+          t_kde = r1.z;
+          t_tp = __float_as_int(r1.w);
+          elhm = r2.x;
         } else {
 #pragma unroll
           for (int k = 0; k < APL; ++k) hpc[k] += hg[k];
@@ -529,6 +575,9 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
           t_psp += r0.w;
           t_hdb += r1.x;
           t_cnt += __float_as_int(r1.y);
+          t_kde += r1.z;
+          t_tp += __float_as_int(r1.w);
+          elhm += r2.x;
         }
       }
     }
@@ -537,46 +586,22 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
   if (!live) return;
 
   // per-atom terms
-  const float kde2l = C.kde2l;
 #pragma unroll
   for (int k = 0; k < APL; ++k) {
     if (!val[k]) continue;
     // hydrophobic restraint, src/kernel_cuda_l2_calcenergy.cu:200-205
     const float h2 = (hpc[k] - C.hpl0[lt[k]]) / C.hpl1[lt[k]];
     t_hpc += 0.5f * h2 * h2 - C.hpl2[lt[k]];
-    // KDE: mean over same-type points, src/kernel_cuda_l2_calcenergy.cu:224-253
-    const int2 kr = S.kr[lt[k]];
-    float ks = 0.0f;
-    for (int j = kr.x; j < kr.y; ++j) {
-      const float4 Q = S.kde[j];
-      ks += fast_exp2(kde2l * pair_d2(x[k], y[k], z[k], Q.x, Q.y, Q.z));
-    }
-    if (kr.y > kr.x) t_kde += ks / (float)(kr.y - kr.x);
-    // true positives against the reference contact list
-    const int nr = C.ref_cnt[li[k]];
-    const int2 *rl = C.ref + (size_t)li[k] * KMAX;
-    for (int j = 0; j < nr; ++j) {
-      const int2 r = rl[j];
-      const float4 P = TILE ? S.gx[r.x] : S.xyze[r.x];
-      const float d2 = pair_d2(x[k], y[k], z[k], P.x, P.y, P.z);
-      const float d = d2 * fast_rsq(d2);
-      t_tp += (d <= __int_as_float(r.y)) ? 1 : 0;
-    }
     // RMSD against the start pose
     const float dx = x[k] - (ox[k] + C.pocket[0]);
     const float dy = y[k] - (oy[k] + C.pocket[1]);
     const float dz = z[k] - (oz[k] + C.pocket[2]);
     t_rms += dx * dx + dy * dy + dz * dz;
   }
-
-  // MCS restraints, src/kernel_cuda_l2_calcenergy.cu:274-313
   const int pos = C.pos;
-  const float elhm = mcs_sum<APL>(C, lpr, x, y, z, li, val);
 
   t_hpc = seg_sum(t_hpc, lpr);
-  t_kde = seg_sum(t_kde, lpr);
   t_rms = seg_sum(t_rms, lpr);
-  t_tp = seg_sum_i(t_tp, lpr);
 
   // src/kernel_cuda_l2_calcenergy.cu:334-359, src/kernel_cuda_l3_combineenergy.cu:28-37
   const float lna = (float)C.lna;

@@ -66,8 +66,14 @@ def _poses(rng, n, t_sigma):
     return mm
 
 
-@pytest.mark.parametrize("which", ["1a07C1", "10gsA00"])
-def test_real_complex_energy_terms_vs_fp64(oracle, gd, which):
+@pytest.mark.parametrize("which,split", [("1a07C1", 0), ("1a07C1", 8), ("10gsA00", 0),
+                                         ("10gsA00", 1), ("10gsA00", 2), ("10gsA00", 4),
+                                         ("10gsA00", 8)])
+def test_real_complex_energy_terms_vs_fp64(oracle, gd, which, split):
+    """Every term against FP64 on the reference's complexes. The receptor
+    split's groups share out the receptor points and also each atom's KDE
+    points (4,408 in 10gsA00), reference contacts and the MCS templates (196
+    in 10gsA00), so the KDE, MCS and MCC terms are checked at every split."""
     from test_gpu_parity import check_energy_parity
     files, cid_name = (A07, "1a07C1") if which == "1a07C1" else (GS10, "10gsA00")
     cx = io.load_complex(*files, PARAS, complex_id=cid_name)
@@ -77,11 +83,11 @@ def test_real_complex_energy_terms_vs_fp64(oracle, gd, which):
     il = rng.integers(0, cx.n_lig, size=n).astype(np.int32)
     ip = rng.integers(0, cx.n_prt, size=n).astype(np.int32)
     il[0] = ip[0] = 0
-    with gd.Engine(seed=2) as eng:
+    with gd.Engine(seed=2, receptor_split=split) as eng:
         cid = eng.add_complex(cx)
         rep = {}
         check_energy_parity(oracle, eng, cid, cx, mm, il, ip, rep)
-    print(f"\n{which} worst (rel, cond):",
+    print(f"\n{which} split {split} worst (rel, cond):",
           {k: (f"{v[0]:.2e}", f"{v[1]:.2e}") for k, v in rep.items()})
 
 
