@@ -67,6 +67,33 @@ __device__ __forceinline__ int seg_sum_i(int v, int lpr) {
   return v;
 }
 
+// KDE sum of one atom over points [b, e) of its type's range:
+// sum_j exp2(kde2l |x - q_j|^2) (src/kernel_cuda_l2_calcenergy.cu:232-246).
+// Four points are read and exponentiated ahead of their additions, which stay
+// in point order, so the sum has the bits of the one-point-at-a-time loop
+// while the LDS read latency of the next points overlaps the arithmetic.
+__device__ __forceinline__ float kde_sum(const float4 *kde, float kde2l, float x, float y,
+                                         float z, int b, int e) {
+  float ks = 0.0f;
+  int j = b;
+  for (; j + 4 <= e; j += 4) {
+    const float4 Q0 = kde[j], Q1 = kde[j + 1], Q2 = kde[j + 2], Q3 = kde[j + 3];
+    const float e0 = fast_exp2(kde2l * pair_d2(x, y, z, Q0.x, Q0.y, Q0.z));
+    const float e1 = fast_exp2(kde2l * pair_d2(x, y, z, Q1.x, Q1.y, Q1.z));
+    const float e2 = fast_exp2(kde2l * pair_d2(x, y, z, Q2.x, Q2.y, Q2.z));
+    const float e3 = fast_exp2(kde2l * pair_d2(x, y, z, Q3.x, Q3.y, Q3.z));
+    ks += e0;
+    ks += e1;
+    ks += e2;
+    ks += e3;
+  }
+  for (; j < e; ++j) {
+    const float4 Q = kde[j];
+    ks += fast_exp2(kde2l * pair_d2(x, y, z, Q.x, Q.y, Q.z));
+  }
+  return ks;
+}
+
 // HPC switching polynomial (src/kernel_cuda_l2_calcenergy.cu:116-121):
 // 1 - (3.5 x - 4.5 x^2 + 2.5 x^3 - 0.5 x^4), x = d^2 / 81, which factors as
 // (1 - x)^3 (1 - x/2) = u^3 (1 + u) / 2 with u = 1 - x. Clamping u at 0 makes
@@ -483,17 +510,30 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
       const int2 kr = S.kr[lt[k]];
       const int kl = kr.y - kr.x;
       const int kb = kr.x + (kl * sp.h) / sp.ns, ke = kr.x + (kl * (sp.h + 1)) / sp.ns;
-      float ks = 0.0f;
-      for (int j = kb; j < ke; ++j) {
-        const float4 Q = S.kde[j];
-        ks += fast_exp2(kde2l * pair_d2(x[k], y[k], z[k], Q.x, Q.y, Q.z));
-      }
+      const float ks = kde_sum(S.kde, kde2l, x[k], y[k], z[k], kb, ke);
       if (kl > 0) t_kde += ks / (float)kl;
       // true positives against the reference contact list
       const int nr = C.ref_cnt[li[k]];
       const int2 *rl = C.ref + (size_t)li[k] * KMAX;
       const int rb0 = (nr * sp.h) / sp.ns, re0 = (nr * (sp.h + 1)) / sp.ns;
-      for (int j = rb0; j < re0; ++j) {
+      int j = rb0;
+      // four contacts at a time: their list entries, then their points, are
+      // read together (the count is order-free)
+      for (; j + 4 <= re0; j += 4) {
+        int2 r[4];
+        float4 P[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) r[u] = rl[j + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) P[u] = TILE ? S.gx[r[u].x] : S.xyze[r[u].x];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float d2 = pair_d2(x[k], y[k], z[k], P[u].x, P[u].y, P[u].z);
+          const float d = d2 * fast_rsq(d2);
+          t_tp += (d <= __int_as_float(r[u].y)) ? 1 : 0;
+        }
+      }
+      for (; j < re0; ++j) {
         const int2 r = rl[j];
         const float4 P = TILE ? S.gx[r.x] : S.xyze[r.x];
         const float d2 = pair_d2(x[k], y[k], z[k], P.x, P.y, P.z);
@@ -651,11 +691,7 @@ __device__ __forceinline__ float cheap_terms(const DevComplex &C, const Lds &S, 
   for (int k = 0; k < APL; ++k) {
     if (!val[k]) continue;
     const int2 kr = S.kr[lt[k]];
-    float ks = 0.0f;
-    for (int j = kr.x; j < kr.y; ++j) {
-      const float4 Q = S.kde[j];
-      ks += fast_exp2(kde2l * pair_d2(x[k], y[k], z[k], Q.x, Q.y, Q.z));
-    }
+    const float ks = kde_sum(S.kde, kde2l, x[k], y[k], z[k], kr.x, kr.y);
     if (kr.y > kr.x) t_kde += ks / (float)(kr.y - kr.x);
   }
   // MCS restraints in closed form from the conformer's template moments:
