@@ -280,7 +280,8 @@ def progress(msg):
 def traffic_model(args):
     """The measured HBM traffic of this workload's plain launches, if profiled."""
     key = (args.config, args.batch if args.config == "batch" else None)
-    if args.config == "single" or args.tile or args.split > 0 or args.ladders > 0:
+    if (args.config == "single" or args.tile or args.split > 0 or args.ladders > 0 or args.lpr
+            or args.apl):
         return None
     return PLAIN_TRAFFIC.get(key)
 

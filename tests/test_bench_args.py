@@ -69,7 +69,8 @@ def test_roofline_traffic_only_for_profiled_workloads():
     assert math.isclose(r["hbm_gbs"], 7 * r["traffic"] / 60.0 / 1e9, rel_tol=1e-12)
     assert r["bound"] == "valu" and r["hbm_gbs"] < 1.0  # GB/s, against 8 TB/s
     for argv in (["--config", "batch", "--batch", "256"], ["--config", "single"],
-                 ["--split", "2"], ["--tile", "256"], ["--ladders", "1"]):
+                 ["--split", "2"], ["--tile", "256"], ["--ladders", "1"], ["--lpr", "16"],
+                 ["--apl", "2"]):
         assert bench.traffic_model(bench.parse(argv)) is None
         assert bench.roofline(1, 1, 1.0, 1, "k", None)["traffic"] is None
     assert bench.traffic_model(bench.parse(["--config", "ensemble"])) is not None
