@@ -163,7 +163,9 @@ typedef struct gd_opts {
   int atoms_per_lane;    /* 0 = auto; else 1..4 */
   int verbose;
   int receptor_split;    /* lane groups per replica, each over its share of every
-                            receptor chunk: 1, 2, 4 or 8 for every workgroup; 0 = auto
+                            receptor chunk and of every atom's KDE points and reference
+                            contacts and of the MCS templates: 1, 2, 4 or 8 for every
+                            workgroup; 0 = auto
                             per workgroup: whole ladders first, then the split that fills
                             the workgroup to 8 waves (ladders left over from an odd count
                             take 4 or 8 groups) */
