@@ -96,6 +96,8 @@ NOALLOC_MAX_POSES = 32768        # GD_NOALLOC_MAX_POSES
 
 LAUNCH_NO_GRAPH = 1
 LAUNCH_PERSISTENT = 2
+LAUNCH_SPREAD = 4
+LAUNCH_NO_SPREAD = 8
 
 
 class RunStats(C.Structure):

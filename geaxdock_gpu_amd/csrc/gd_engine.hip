@@ -98,6 +98,10 @@ struct gd_ctx {
   Pack *d_packs = nullptr;
   int *d_slots = nullptr;
   int n_packs = 0, ladders_per_pack = 0;
+  // spread packing: ladders exchanged between launches, (complex, ladder,
+  // replica of slot 0, replica stride)
+  int4 *d_ladders = nullptr;
+  int n_ladders = 0;
   std::vector<LaunchClass> mc_classes;  // kernel launches of the MC pack set
   // device control words: [0] first step of a graph replay, [1 + i] pack
   // counter of class i in persistent launches
@@ -273,7 +277,8 @@ void gd_destroy(gd_ctx *ctx) {
   void *ps[] = {ctx->d_cx, ctx->d_overflow, ctx->d_mm, ctx->d_eold, ctx->d_idx_tmp,
                 ctx->d_acs_mc, ctx->d_acs_x, ctx->d_rep_flat, ctx->d_rec, ctx->d_rec_cnt,
                 ctx->d_packs, ctx->d_slots, ctx->d_best_e, ctx->d_best_mm, ctx->d_ctrl,
-                ctx->d_drain_off, ctx->d_drain, ctx->d_ring[1], ctx->d_ring_cnt[1]};
+                ctx->d_drain_off, ctx->d_drain, ctx->d_ring[1], ctx->d_ring_cnt[1],
+                ctx->d_ladders};
   if (ctx->h_drain) (void)hipHostFree(ctx->h_drain);
   for (auto &g : ctx->gslot)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -860,8 +865,62 @@ int auto_gmax(const gd_ctx *ctx) {
   return g;
 }
 
+// Spread packing: on request, or by default when the workload has at most
+// one temperature ladder per two CUs (ladder-per-workgroup packing would
+// leave most CUs idle) and no fixed split or ladders per pack is set.
+bool spread_packing(const gd_ctx *ctx) {
+  const int L = ctx->opts.launch;
+  if (L & GD_LAUNCH_SPREAD) return true;
+  if ((L & GD_LAUNCH_NO_SPREAD) || ctx->opts.receptor_split > 0 || ctx->opts.ladders_per_pack > 0)
+    return false;
+  int64_t ladders = 0;
+  for (const HostComplex &h : ctx->cx) ladders += (int64_t)h.d.n_prt * h.d.n_lig;
+  return ladders > 0 && 2 * ladders <= ctx->n_cu;
+}
+
+// Receptor split of each ligand conformer's MC packs. The split sets the
+// summation order, so a pose's energy bits follow it; gd_calc_energy uses it.
+std::vector<int> lig_splits(const gd_ctx *ctx, const HostComplex &h) {
+  std::vector<int> ns(h.d.n_lig, spread_packing(ctx) ? 8 : 1);
+  if (spread_packing(ctx)) return ns;
+  for (const LadderGroup &lg : ladder_groups(ctx, h, auto_gmax(ctx)))
+    for (int g = 0; g < lg.ng; ++g) ns[lg.l0 + g] = lg.ns;
+  return ns;
+}
+
+// Spread packing: each pack holds one wave of replicas of one ladder
+// (64 / lanes-per-replica temperature slots, in slot order) at a receptor
+// split of 8, so each workgroup is 8 waves; a ladder of more slots spans
+// several packs, and its exchanges run between launches.
+PackSet build_spread_packs(gd_ctx *ctx) {
+  PackSet ps;
+  for (int c = 0; c < (int)ctx->cx.size(); ++c) {
+    HostComplex &h = ctx->cx[c];
+    const int n_tmp = h.d.n_tmp, n_lig = h.d.n_lig;
+    const int r = std::max(1, 64 / h.lpr);
+    h.lig_ns.assign(n_lig, 8);
+    for (int p = 0; p < h.d.n_prt; ++p)
+      for (int l = 0; l < n_lig; ++l)
+        for (int t0 = 0; t0 < n_tmp; t0 += r) {
+          Pack pk{};
+          pk.cx = c;
+          pk.prt = p;
+          pk.n_groups = 0;
+          pk.n_slots = std::min(r, n_tmp - t0);
+          pk.ns = 8;
+          pk.slot_off = (int)ps.slots.size();
+          for (int t = t0; t < t0 + pk.n_slots; ++t)
+            ps.slots.push_back(h.rep_base + n_tmp * n_lig * p + n_lig * t + l);
+          ps.packs.push_back(pk);
+        }
+  }
+  pack_finish(ctx, ps);
+  return ps;
+}
+
 // MC packs: the ladder groups of every (complex, protein conformer).
 PackSet build_mc_packs(gd_ctx *ctx) {
+  if (spread_packing(ctx)) return build_spread_packs(ctx);
   PackSet ps;
   const int gmax = auto_gmax(ctx);
   for (int c = 0; c < (int)ctx->cx.size(); ++c) {
@@ -937,7 +996,24 @@ gd_status alloc_state(gd_ctx *ctx) {
       return fail(ctx, GD_ERR_CAPACITY,
                   "receptor split %d needs ladders of at most %d lanes (a ladder has %d)", ns,
                   512 / ns, h.d.n_tmp * h.lpr);
+  if ((ctx->opts.launch & GD_LAUNCH_SPREAD) && (ns > 0 || ctx->opts.ladders_per_pack > 0))
+    return fail(ctx, GD_ERR_INVALID,
+                "GD_LAUNCH_SPREAD sets its own packs: leave receptor_split and ladders_per_pack 0");
   PackSet ps = build_mc_packs(ctx);
+  if (spread_packing(ctx)) {
+    std::vector<int4> lad;
+    for (int c = 0; c < (int)ctx->cx.size(); ++c) {
+      const HostComplex &h = ctx->cx[c];
+      const int n_tmp = h.d.n_tmp, n_lig = h.d.n_lig;
+      for (int p = 0; p < h.d.n_prt; ++p)
+        for (int l = 0; l < n_lig; ++l)
+          lad.push_back(make_int4(c, p * n_lig + l, h.rep_base + n_tmp * n_lig * p + l, n_lig));
+    }
+    ctx->n_ladders = (int)lad.size();
+    HIPCHK(hipMalloc(&ctx->d_ladders, lad.size() * sizeof(int4)));
+    HIPCHK(hipMemcpyAsync(ctx->d_ladders, lad.data(), lad.size() * sizeof(int4),
+                          hipMemcpyHostToDevice, ctx->stream));
+  }
   HIPCHK(hipMalloc(&ctx->d_packs, ps.packs.size() * sizeof(Pack)));
   HIPCHK(hipMalloc(&ctx->d_slots, ps.slots.size() * sizeof(int)));
   HIPCHK(hipMemcpyAsync(ctx->d_packs, ps.packs.data(), ps.packs.size() * sizeof(Pack),
@@ -1259,6 +1335,35 @@ gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float
   return launch_mc_wait(ctx, ms);
 }
 
+// One window of spread packing: MC launches that end on exchange steps, each
+// followed by the ladder exchange kernel (timed with its launch). Advances
+// ctx->step by win.
+gd_status spread_window(gd_ctx *ctx, int win, int nc, int da_k, float sur_scale, double *ms_sum,
+                        int *launches) {
+  const int s0 = (int)ctx->step;
+  int done = 0;
+  while (done < win) {
+    int sub = win - done;
+    if (nc > 0) sub = std::min(sub, nc - (s0 + done) % nc);
+    TRY(launch_mc_enqueue(ctx, MODE_MC, s0 + done, sub, 0, da_k, sur_scale));
+    done += sub;
+    if (nc > 0 && (s0 + done) % nc == 0) {
+      const KParams P = base_params(ctx);
+      hipLaunchKernelGGL(gd_ladder_exchange_kernel, dim3((ctx->n_ladders + 63) / 64), dim3(64),
+                         0, ctx->stream, ctx->d_cx, ctx->d_ladders, ctx->n_ladders, s0 + done, nc,
+                         P.k0, P.k1, ctx->d_eold, ctx->d_idx_tmp, ctx->d_acs_x);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    }
+    float ms = 0.f;
+    TRY(launch_mc_wait(ctx, &ms));
+    *ms_sum += ms;
+    *launches += 1;
+  }
+  ctx->step += win;
+  return GD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1307,12 +1412,19 @@ static gd_status run_impl(gd_ctx *ctx, int64_t n_steps, int steps_per_exchange,
   // with records and a sink, windows alternate between the two rings: the
   // previous window's records are drained (on the drain stream, sinks called
   // here) while the next window runs
-  const bool overlap = sink && ctx->d_rec && n_steps > 0;
+  const bool spread = spread_packing(ctx);
+  const bool overlap = sink && ctx->d_rec && n_steps > 0 && !spread;
   if (overlap) TRY(ensure_ring1(ctx));
   int pending = -1;
   while (left > 0) {
     int win = (int)std::min<int64_t>(left, ctx->opts.steps_per_dump);
     if (da_k > 0) win = std::max(da_k, win / da_k * da_k);
+    if (spread) {
+      TRY(spread_window(ctx, win, steps_per_exchange, da_k, sur_scale, &t_ms, &st.kernel_launches));
+      left -= win;
+      TRY(drain_records(ctx, sink, user));
+      continue;
+    }
     float ms = 0.f;
     TRY(launch_mc_enqueue(ctx, MODE_MC, (int)ctx->step, win, steps_per_exchange, da_k, sur_scale));
     // a failed drain is reported after the window in flight has finished
@@ -1526,11 +1638,7 @@ static gd_status eval_poses(gd_ctx *ctx, int cid, int n, const int *idx_lig, con
   // each pose is evaluated with the receptor split of its ligand conformer's
   // MC packs, so a pose's energy has the bits an MC step gives it
   std::vector<int> lig_ns = h.lig_ns;
-  if (lig_ns.size() != (size_t)h.d.n_lig) {
-    lig_ns.assign(h.d.n_lig, 1);
-    for (const LadderGroup &lg : ladder_groups(ctx, h, auto_gmax(ctx)))
-      for (int g = 0; g < lg.ng; ++g) lig_ns[lg.l0 + g] = lg.ns;
-  }
+  if (lig_ns.size() != (size_t)h.d.n_lig) lig_ns = lig_splits(ctx, h);
   PackSet ps;
   for (int p = 0; p < h.d.n_prt; ++p)
     for (int ns = 1; ns <= 8; ns *= 2) {

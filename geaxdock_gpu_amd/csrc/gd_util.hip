@@ -75,6 +75,42 @@ extern "C" __global__ void gd_exchange_kernel(int n_groups, int n_tmp, int mode_
                   acs + (size_t)g * n_tmp, order, [&](int pair) { return ug[pair]; });
 }
 
+// Replica exchange of whole temperature ladders between MC launches (spread
+// packing, where a ladder's replicas sit in several workgroups): the in-kernel
+// exchange of run_pack with the same energies, temperature map, counters and
+// uniforms. One thread per ladder: ladders[g] = (complex, ladder id within the
+// complex, replica of temperature slot 0, replica stride between slots).
+// The exchange follows the MC step that ends at step_end (a multiple of nc).
+extern "C" __global__ void gd_ladder_exchange_kernel(const DevComplex *cxs, const int4 *ladders,
+                                                     int n_ladders, int step_end, int nc,
+                                                     uint32_t k0, uint32_t k1,
+                                                     const gd_energy *e_old, int *idx_tmp,
+                                                     int *acs_x) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_ladders) return;
+  const int4 L = ladders[g];
+  const DevComplex &C = cxs[L.x];
+  const int n_tmp = C.n_tmp;
+  const int seg = step_end / nc - 1;
+  const int mode_t = (seg % 2 == 0) ? 1 : 0;
+  float et[GD_MAX_NTMP];
+  int temps[GD_MAX_NTMP], acs[GD_MAX_NTMP], order[GD_MAX_NTMP];
+  for (int t = 0; t < n_tmp; ++t) {
+    const int r = L.z + L.w * t;
+    et[t] = e_old[r].e[GD_MAXWEI - 1];
+    temps[t] = idx_tmp[r];
+    acs[t] = acs_x[r];
+  }
+  const int cid = C.cid, group = L.y;
+  exchange_ladder(n_tmp, mode_t, et, C.minus_beta, temps, acs, order,
+                  [&](int pair) { return exchange_uniform(k0, k1, cid, group, seg, pair); });
+  for (int t = 0; t < n_tmp; ++t) {
+    const int r = L.z + L.w * t;
+    idx_tmp[r] = temps[t];
+    acs_x[r] = acs[t];
+  }
+}
+
 extern "C" __global__ void gd_philox_kernel(int n, const uint32_t *ctr, const uint32_t *key,
                                             uint32_t *out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

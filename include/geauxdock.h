@@ -181,9 +181,20 @@ typedef struct gd_opts {
  * launch classes, forked over side streams) and replayed; the first step of
  * a window is read from device memory. PERSISTENT launches each class with
  * as many workgroups as stay resident, taking packs from a device counter in
- * longest-first order. */
+ * longest-first order. SPREAD packing, for workloads with few temperature
+ * ladders: every workgroup holds one wave of replicas (a ladder may span
+ * several workgroups) at a receptor split of 8, and the replica exchanges run
+ * in a kernel of their own between launches of steps_per_exchange steps, so
+ * a small workload occupies more CUs. Energies then carry the bits of the
+ * split of 8 (gd_calc_energy follows). It is automatic when the complexes
+ * hold at most one ladder per two CUs and receptor_split and
+ * ladders_per_pack are 0; SPREAD asks for it (not with a fixed
+ * receptor_split or ladders_per_pack), NO_SPREAD keeps every ladder in one
+ * workgroup. */
 #define GD_LAUNCH_NO_GRAPH 1
 #define GD_LAUNCH_PERSISTENT 2
+#define GD_LAUNCH_SPREAD 4
+#define GD_LAUNCH_NO_SPREAD 8
 
 typedef struct gd_run_stats {
   double t_compute; /* seconds in MC kernels (HIP events), the reference's McLog.t0 */
