@@ -1337,20 +1337,19 @@ gd_status launch_mc(gd_ctx *ctx, int mode, int step0, int n_steps, int nc, float
 
 // One window of spread packing: MC launches that end on exchange steps, each
 // followed by the ladder exchange kernel (timed with its launch). Advances
-// ctx->step by win.
+// ctx->step by each launch's steps as it completes.
 gd_status spread_window(gd_ctx *ctx, int win, int nc, int da_k, float sur_scale, double *ms_sum,
                         int *launches) {
-  const int s0 = (int)ctx->step;
   int done = 0;
   while (done < win) {
+    const int s = (int)ctx->step;
     int sub = win - done;
-    if (nc > 0) sub = std::min(sub, nc - (s0 + done) % nc);
-    TRY(launch_mc_enqueue(ctx, MODE_MC, s0 + done, sub, 0, da_k, sur_scale));
-    done += sub;
-    if (nc > 0 && (s0 + done) % nc == 0) {
+    if (nc > 0) sub = std::min(sub, nc - s % nc);
+    TRY(launch_mc_enqueue(ctx, MODE_MC, s, sub, 0, da_k, sur_scale));
+    if (nc > 0 && (s + sub) % nc == 0) {
       const KParams P = base_params(ctx);
       hipLaunchKernelGGL(gd_ladder_exchange_kernel, dim3((ctx->n_ladders + 63) / 64), dim3(64),
-                         0, ctx->stream, ctx->d_cx, ctx->d_ladders, ctx->n_ladders, s0 + done, nc,
+                         0, ctx->stream, ctx->d_cx, ctx->d_ladders, ctx->n_ladders, s + sub, nc,
                          P.k0, P.k1, ctx->d_eold, ctx->d_idx_tmp, ctx->d_acs_x);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1359,8 +1358,9 @@ gd_status spread_window(gd_ctx *ctx, int win, int nc, int da_k, float sur_scale,
     TRY(launch_mc_wait(ctx, &ms));
     *ms_sum += ms;
     *launches += 1;
+    ctx->step += sub;
+    done += sub;
   }
-  ctx->step += win;
   return GD_OK;
 }
 
