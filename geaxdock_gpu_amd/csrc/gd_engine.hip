@@ -427,6 +427,14 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
       r[P_HA] = on ? h1 * hsc : 0.0f;
       r[P_HB] = on ? -h0 * h1 * hsc : 0.0f;
       r[P_HC] = on ? sqrt_2_pi_m1 * h1 : 0.0f;
+      // the per-chunk constants of the pair loop, with its FP32 expressions
+      const float p1a = r[P_P1A], pmf0 = r[P_PMF0];
+      const float rn = fmaxf(pmf0, 1.0f / ep.el1) * 1.0001f;
+      r[P_NR2] = rn * rn;
+      r[P_A0] = p1a * ep.lj0;
+      r[P_A1] = p1a * (ep.lj0 * ep.lj1);
+      r[P_PB] = LOG2E * fmaf(0.5f, pmf0, 6.0f);
+      r[P_PC] = -6.0f * LOG2E * pmf0;
     }
 
   // ligand conformers [n_lig][3][lna]

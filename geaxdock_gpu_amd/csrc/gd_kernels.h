@@ -10,12 +10,15 @@ namespace gd {
 
 constexpr int NLT = GD_MAXTP2;            // ligand atom types
 constexpr int NPT = GD_MAXTP1;            // receptor point types
-constexpr int NPAR = 8;                   // per (receptor type, ligand type) record
+constexpr int NPAR = 12;                  // per (receptor type, ligand type) record
 constexpr int PAR_FLOATS = NPT * NLT * NPAR;
 constexpr int KMAX = GD_MAX_REFCONTACT;
 
 // per (receptor type, ligand type) parameter record
-enum { P_P1A = 0, P_P2A, P_PMF0, P_PMF1, P_HA, P_HB, P_HC, P_PAD };
+// the record: the pair parameters, then values the pair loop derives from them
+// (the contact radius squared, the LJ denominator's a0 and a1, the PMF
+// exponent's B and C), precomputed with the pair loop's FP32 expressions
+enum { P_P1A = 0, P_P2A, P_PMF0, P_PMF1, P_HA, P_HB, P_HC, P_NR2, P_A0, P_A1, P_PB, P_PC };
 
 // Receptor point kinds after sorting: which optional terms a point can
 // contribute. 0: neither, 1: hydrophobic contact (c0_and_d12_or_c2), 2: also

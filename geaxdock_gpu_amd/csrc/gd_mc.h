@@ -380,8 +380,15 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
 #pragma unroll
     for (int k = 0; k < APL; ++k) {
       const float *r = S.par + (type * NLT + lt[k]) * NPAR;
+      // the record holds the chunk constants precomputed on the host:
+      // a0 = p1a lj0, a1 = p1a lj0 lj1; (6 - d/2)(d - pmf0) log2(e) =
+      // A d^2 + B d + C with B = log2(e)(6 + pmf0/2), C = -6 log2(e) pmf0;
+      // contacts (d <= pmf0) and the near electrostatic form (d < 1/el1) lie
+      // within sqrt(nr2), nr2 = (1.0001 max(pmf0, 1/el1))^2 (the margin covers
+      // the rounding of d)
       const float4 a = *reinterpret_cast<const float4 *>(r);
       const float4 b = *reinterpret_cast<const float4 *>(r + 4);
+      const float4 c = *reinterpret_cast<const float4 *>(r + 8);
       cp[k].p1a = a.x;
       cp[k].p2a = a.y;
       cp[k].pmf0 = a.z;
@@ -389,15 +396,11 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
       cp[k].ha = b.x;
       cp[k].hb = b.y;
       cp[k].hc = b.z;
-      cp[k].a0 = a.x * K.lj0;
-      cp[k].a1 = a.x * K.lj01;
-      // (6 - d/2)(d - pmf0) log2(e) = A d^2 + B d + C
-      cp[k].pb = LOG2E * fmaf(0.5f, a.z, 6.0f);
-      cp[k].pc = -6.0f * LOG2E * a.z;
-      // contacts (d <= pmf0) and the near electrostatic form (d < 1/el1)
-      // lie within this radius; the margin covers the rounding of d
-      const float rn = fmaxf(a.z, C.inv_el1) * 1.0001f;
-      cp[k].nr2 = rn * rn;
+      cp[k].nr2 = b.w;
+      cp[k].a0 = c.x;
+      cp[k].a1 = c.y;
+      cp[k].pb = c.z;
+      cp[k].pc = c.w;
     }
     // the PMF exponent (6 - d/2)(d - pmf0) log2(e) peaks at (pmf0 - 12)^2 / 8
     // log2(e); only a chunk where some lane's peak passes 60 needs the clamp
