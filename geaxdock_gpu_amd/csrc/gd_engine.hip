@@ -213,8 +213,11 @@ size_t lds_need(const HostComplex &h, int slots, int tile) {
   const gd_complex_desc &d = h.d;
   const size_t pts = tile > 0 ? (size_t)2 * tile : (size_t)d.pnp;
   const size_t kde = tile > 0 ? 0 : (size_t)d.pnk * 16;
+  // the receptor split's two exchange buffers of a 512-lane workgroup (any
+  // pack of the complex may be split)
+  const size_t xp = (size_t)2 * (11 * 512 / 2) * 4;
   return pts * 24 + PAR_FLOATS * 4 + kde + NLT * 8 + (size_t)h.n_chunks * 16 +
-         (size_t)slots * 16 + 256;
+         (size_t)slots * 16 + xp + 256;
 }
 
 // Tile length of a complex's packs, 0 when its receptor stays resident.

@@ -121,3 +121,22 @@ def test_zero_poses_and_zero_steps(gd):
         # the start poses were evaluated and recorded; no move was made
         assert np.all(s0["movematrix"] == 0) and np.all(s0["acs_mc"] == 0)
         assert sum(len(r) for r in recs) == cx.n_rep
+
+
+@pytest.mark.parametrize("pnp", [4200, 4600, 5600])
+@pytest.mark.parametrize("split", [0, 8])
+def test_receptors_near_the_lds_limit(gd, pnp, split):
+    """Receptors around the size where the resident LDS image (receptor,
+    type-pair records, the split's exchange buffers) stops fitting: each
+    either stays resident or streams through tiles, at any split, with terms
+    against FP64 and exact records."""
+    from oracle import parity
+    cx = synth.make_complex(80 + pnp // 100, lna=24, pnp=pnp, pnk=100, n_lig=2, n_prt=1,
+                            n_tmp=4)
+    rng = np.random.default_rng(pnp)
+    il, ip, mm = _poses(rng, cx, 12)
+    with gd.Engine(seed=6, receptor_split=split) as eng:
+        cid = eng.add_complex(cx)
+        out = eng.calc_energy(cid, il, ip, mm)
+        parity.check(out, _fp64(cx, ip, il, mm), label=f"pnp {pnp} split {split}")
+        _run_exact(eng, cid, 40, 20)
