@@ -538,11 +538,8 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
   D.cid = cid;
   for (int i = 0; i < 3; ++i) D.pocket[i] = d->pocket_center[i];
   for (int i = 0; i < 6; ++i) D.move_scale[i] = d->move_scale[i];
-  D.lj0 = ep.lj0;
-  D.lj01 = ep.lj0 * ep.lj1;
   D.el0 = ep.el0;
   D.el1 = ep.el1;
-  D.inv_el1 = 1.0f / ep.el1;
   D.a1 = ep.a1;
   D.b1 = ep.b1;
   D.kde2l = ep.kde2 * 1.44269504088896340736f;

@@ -34,7 +34,7 @@ struct DevComplex {
   int cid;        // complex id (random stream id)
   float pocket[3];
   float move_scale[6];
-  float lj0, lj01, el0, el1, inv_el1, a1, b1, kde2l, kde3;
+  float el0, el1, a1, b1, kde2l, kde3;
   float w[GD_MAXWEI], a_para[GD_MAXWEI], b_para[GD_MAXWEI];
   float hpl0[NLT], hpl1[NLT], hpl2[NLT];
   float minus_beta[GD_MAX_NTMP];

@@ -105,7 +105,7 @@ constexpr float INV81 = 1.0f / 81.0f;
 constexpr float LOG2E = 1.44269504088896340736f;
 
 struct PairConst {
-  float lj0, lj01, el0, el1, a1, b1;
+  float el0, el1, a1, b1;
 };
 
 // Per-lane parameters of one chunk (one receptor type) for one ligand atom.
@@ -350,8 +350,6 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
                             const bool (&val)[APL], const float mm[6], gd_energy &E) {
   const int n_chunks = __builtin_amdgcn_readfirstlane(C.n_chunks);
   PairConst K;
-  K.lj0 = C.lj0;
-  K.lj01 = C.lj01;
   K.el0 = C.el0;
   K.el1 = C.el1;
   K.a1 = C.a1;
