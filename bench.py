@@ -66,8 +66,8 @@ METRIC = "MC replica-steps/s, plain vs surrogate-DA, at matched best-pose energy
 # every workgroup staging its receptor, tables and KDE points into LDS; the
 # per-step part is L2 misses of MCS templates, contact lists and PSP rows.
 PLAIN_TRAFFIC = {
-    ("batch", 1024): (8.292e8, 4.132e5, 1.576e7, 246.6),
-    ("ensemble", None): (7.90e6, 0.0, 8.01e5, 0.0),
+    ("batch", 1024): (9.111e8, 4.164e5, 1.576e7, 238.5),
+    ("ensemble", None): (8.04e6, 0.0, 8.01e5, 0.0),
 }
 
 
