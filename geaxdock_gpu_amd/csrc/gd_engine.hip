@@ -46,6 +46,7 @@ struct HostComplex {
   int lpr = 0, apl = 0, n_chunks = 0;
   bool tiled = false;  // receptor streamed through LDS tiles
   bool ref_built = false;
+  bool shape_final = false;  // lane shape fixed (the complex's energies have bits)
   std::vector<int> lig_t;  // ligand atom types (host copy)
   std::vector<int> lig_ns;  // receptor split of each ligand conformer's MC packs
   DevComplex dev{};
@@ -611,8 +612,68 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
 
 namespace {
 
+bool spread_packing(const gd_ctx *ctx);
+
+// Packs of one wave of replicas (spread packing) for a complex at lpr lanes
+// per replica.
+int64_t spread_pack_count(const HostComplex &h, int lpr) {
+  const int r = std::max(1, 64 / lpr);
+  return (int64_t)h.d.n_prt * h.d.n_lig * ((h.d.n_tmp + r - 1) / r);
+}
+
+// Lane shapes under spread packing, fixed when a complex is first put on the
+// device. Spread packs hold one wave of replicas, so more lanes per replica
+// make more packs: a complex takes the shape with the fewest rounds of packs
+// over the CUs times atoms per lane (the fewest packs on a tie), among the
+// shapes whose ladder fits a workgroup. Forced shapes are kept.
+gd_status widen_shapes(gd_ctx *ctx) {
+  const bool widen = spread_packing(ctx) && ctx->opts.lanes_per_replica == 0 &&
+                     ctx->opts.atoms_per_lane == 0;
+  int64_t packs = 0;
+  for (const HostComplex &h : ctx->cx)
+    if (h.shape_final) packs += spread_pack_count(h, h.lpr);
+  for (HostComplex &h : ctx->cx) {
+    if (h.shape_final) continue;
+    h.shape_final = true;
+    if (!widen) continue;
+    const int lna = h.d.lna;
+    auto cost = [&](int l, int a) {
+      return (double)((packs + spread_pack_count(h, l) + ctx->n_cu - 1) / ctx->n_cu) * a;
+    };
+    int best_l = h.lpr, best_a = h.apl;
+    double best = cost(best_l, best_a);
+    for (int l = 8; l <= 64; l *= 2) {
+      const int a = (lna + l - 1) / l;
+      if (a > 4 || l * h.d.n_tmp > 512) continue;
+      const double c = cost(l, a);
+      if (c < best || (c == best && l < best_l)) {
+        best = c;
+        best_l = l;
+        best_a = a;
+      }
+    }
+    if (best_l != h.lpr) {
+      const int ol = h.lpr, oa = h.apl;
+      h.lpr = best_l;
+      h.apl = best_a;
+      const size_t need = lds_need(h, 512 / h.lpr, h.tiled ? complex_tile(ctx, h) : 0);
+      if (need > ctx->max_lds) {  // keep the shape add_complex chose
+        h.lpr = ol;
+        h.apl = oa;
+      } else {
+        h.lds_bytes = need;
+        h.dev.lpr = h.lpr;
+        h.dev.apl = h.apl;
+      }
+    }
+    packs += spread_pack_count(h, h.lpr);
+  }
+  return GD_OK;
+}
+
 gd_status sync_complexes(gd_ctx *ctx) {
   if (!ctx->cx_dirty) return GD_OK;
+  TRY(widen_shapes(ctx));
   if (ctx->d_cx) HIPCHK(hipFree(ctx->d_cx));
   ctx->d_cx = nullptr;
   for (auto &g : ctx->gslot) g.valid = false;  // captured launches hold the old pointer

@@ -302,8 +302,12 @@ gd_status gd_get_best(gd_ctx *ctx, int complex_id, float *best_e, float *best_mo
 /* Best-so-far := each replica's current state. */
 gd_status gd_reset_best(gd_ctx *ctx);
 /* Lane shape chosen for a complex: lanes per replica and ligand atoms per
- * lane (the MC kernel instance that runs it). No reference counterpart: the
- * reference fixes its launch shape at compile time (src/Makefile:77-105). */
+ * lane (the MC kernel instance that runs it). Under spread packing, unless
+ * lanes_per_replica or atoms_per_lane is forced, the shape is widened once,
+ * when the complex first goes to the device (first gd_calc_energy or
+ * gd_run), to the one that makes the most useful one-wave packs; it is fixed
+ * from then on. No reference counterpart: the reference fixes its launch
+ * shape at compile time (src/Makefile:77-105). */
 gd_status gd_complex_shape(const gd_ctx *ctx, int complex_id, int *lanes_per_replica,
                            int *atoms_per_lane);
 

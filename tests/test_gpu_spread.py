@@ -155,3 +155,34 @@ def test_spread_is_automatic_for_small_workloads(gd):
         assert np.array_equal(auto[k], spread[k]), k
     assert not np.array_equal(auto["energy"]["e"].view(np.uint32),
                               whole["energy"]["e"].view(np.uint32))
+
+
+def test_spread_widens_lane_shapes_once(gd):
+    """Under spread packing a complex that goes to the device for the first
+    time takes the lane shape that makes the most useful packs (the single
+    complex: 32 lanes of one atom instead of 16 lanes of two); the shape then
+    stays, so energies before and after a run have the same bits. A forced
+    shape, or NO_SPREAD, keeps the shape add_complex chose."""
+    cx = synth.single_complex(seed=4)
+    rng = np.random.default_rng(8)
+    mm = np.zeros((6, 6), np.float32)
+    mm[:, :3] = rng.normal(0, 0.5, (6, 3))
+    il = np.zeros(6, np.int32)
+    with gd.Engine(seed=2) as eng:
+        cid = eng.add_complex(cx)
+        assert eng.shape(cid) == (16, 2)
+        e0 = eng.calc_energy(cid, il, il, mm)
+        assert eng.shape(cid) == (32, 1)
+        recs = []
+        eng.run(60, 20, sink=lambda c, r: recs.append(r))
+        assert eng.shape(cid) == (32, 1)
+        e1 = eng.calc_energy(cid, il, il, mm)
+        assert np.array_equal(e0["e"].view(np.uint32), e1["e"].view(np.uint32))
+        r = np.concatenate(recs)
+        ex = eng.calc_energy(cid, r["replica"][:, 3], r["replica"][:, 1], r["movematrix"])
+        assert np.array_equal(ex["e"].view(np.uint32), r["energy"]["e"].view(np.uint32))
+    for kw in (dict(lanes_per_replica=16), dict(launch=abi.LAUNCH_NO_SPREAD)):
+        with gd.Engine(seed=2, **kw) as eng:
+            cid = eng.add_complex(cx)
+            eng.calc_energy(cid, il, il, mm)
+            assert eng.shape(cid) == (16, 2), kw
