@@ -100,9 +100,12 @@ __device__ inline bool metropolis(float e_new, float e_old, float minus_beta, fl
   return u < decision_expf(prod);
 }
 
-// Six DOF offsets of one step.
+// Six DOF offsets of one step. Products and sums round one at a time, as in
+// the CPU chain: no multiply-add fuses with the caller's addition, so the
+// offsets and the moved pose have the same bits wherever they are inlined.
 __device__ inline void move_offsets(const float u[8], const float scale[6], int kind,
                                     float dm[6]) {
+#pragma clang fp contract(off)
   if (kind == GD_MOVE_GAUSSIAN) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -116,6 +119,13 @@ __device__ inline void move_offsets(const float u[8], const float scale[6], int 
 #pragma unroll
     for (int i = 0; i < 6; ++i) dm[i] = scale[i] * (2.0f * u[i] - 1.0f);
   }
+}
+
+// The moved pose mn = mo + dm, each sum rounded on its own (see above).
+__device__ inline void apply_move(const float mo[6], const float dm[6], float mn[6]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int i = 0; i < 6; ++i) mn[i] = mo[i] + dm[i];
 }
 
 // Euler rotation of src/kernel_cuda_l2_move.cu:43-51.

@@ -537,6 +537,9 @@ gd_status gd_add_complex(gd_ctx *ctx, const gd_complex_desc *d, int *complex_id)
   D.move_kind = d->move_kind;
   D.ref_total = 0;
   D.cid = cid;
+  // a surrogate step evaluates the KDE and MCS terms exactly; for large sets
+  // the receptor split's groups share them (one exchange per step)
+  D.da_share = (d->pnk >= 256 || d->pos >= 32) ? 1 : 0;
   for (int i = 0; i < 3; ++i) D.pocket[i] = d->pocket_center[i];
   for (int i = 0; i < 6; ++i) D.move_scale[i] = d->move_scale[i];
   D.el0 = ep.el0;

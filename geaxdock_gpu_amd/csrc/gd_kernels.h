@@ -32,6 +32,7 @@ struct DevComplex {
   int n_chunks, lpr, apl, move_kind;
   int ref_total;  // contacts of the reference contact map
   int cid;        // complex id (random stream id)
+  int da_share;   // 1: split groups share the surrogate's KDE and MCS sums
   float pocket[3];
   float move_scale[6];
   float el0, el1, a1, b1, kde2l, kde3;

@@ -680,30 +680,42 @@ __device__ __forceinline__ void pose_energy(const DevComplex &C, const Lds &S, i
 // Weighted KDE, MCS and pocket-distance terms (e[6..8] after normalization)
 // of one pose: the cheap part of the energy, evaluated exactly inside the
 // surrogate (src/kernel_cuda_l2_calcenergy.cu:224-326).
-template <int APL>
+//
+// SHARE (a split pack of a complex with large KDE or MCS sets): the groups of
+// the receptor split take their parts of each atom's KDE points and of the
+// MCS templates and exchange the partial sums through the split's LDS
+// buffers behind one barrier, adding them in group order. Every thread of the
+// workgroup must then call this together; `live` (wave-uniform) says whether
+// the wave has a pose. Without SHARE the lists are evaluated whole by every
+// group, and only waves with a pose call it.
+template <int APL, bool SHARE>
 __device__ __forceinline__ float cheap_terms(const DevComplex &C, const Lds &S, int lpr,
+                                             Split &sp, bool live,
                                              const float (&x)[APL], const float (&y)[APL],
                                              const float (&z)[APL], const int (&lt)[APL],
                                              const bool (&val)[APL], const float mm[6],
                                              const Rot &R, int a, int ilig) {
-  float t_kde = 0.0f;
-  const float kde2l = C.kde2l;
-#pragma unroll
-  for (int k = 0; k < APL; ++k) {
-    if (!val[k]) continue;
-    const int2 kr = S.kr[lt[k]];
-    const float ks = kde_sum(S.kde, kde2l, x[k], y[k], z[k], kr.x, kr.y);
-    if (kr.y > kr.x) t_kde += ks / (float)(kr.y - kr.x);
-  }
-  // MCS restraints in closed form from the conformer's template moments:
-  // sum |R o + t - m|^2 = A + n |R obar + t - mbar|^2 - 2 <R, M>; the
-  // replica's lanes take the templates in turn
+  const int gh = SHARE ? sp.h : 0, gn = SHARE ? sp.ns : 1;
   const int pos = C.pos;
-  float elhm = 0.0f;
-  {
+  float t_kde = 0.0f, elhm = 0.0f;
+  if (live) {
+    const float kde2l = C.kde2l;
+#pragma unroll
+    for (int k = 0; k < APL; ++k) {
+      if (!val[k]) continue;
+      const int2 kr = S.kr[lt[k]];
+      const int kl = kr.y - kr.x;
+      const float ks = kde_sum(S.kde, kde2l, x[k], y[k], z[k], kr.x + (kl * gh) / gn,
+                               kr.x + (kl * (gh + 1)) / gn);
+      if (kl > 0) t_kde += ks / (float)kl;
+    }
+    // MCS restraints in closed form from the conformer's template moments:
+    // sum |R o + t - m|^2 = A + n |R obar + t - mbar|^2 - 2 <R, M>; the
+    // replica's lanes take the group's templates in turn
     const float4 *qm = C.mcsq + (size_t)ilig * pos * 4;
     const float tx = mm[0] + C.pocket[0], ty = mm[1] + C.pocket[1], tz = mm[2] + C.pocket[2];
-    for (int m = a; m < pos; m += lpr) {
+    const int mb = (pos * gh) / gn, me = (pos * (gh + 1)) / gn;
+    for (int m = mb + a; m < me; m += lpr) {
       const float n = C.mcs_cnt[m];
       if (!(n > 0.0f)) continue;
       const float4 q0 = qm[4 * m], q1 = qm[4 * m + 1], q2 = qm[4 * m + 2], q3 = qm[4 * m + 3];
@@ -715,9 +727,25 @@ __device__ __forceinline__ float cheap_terms(const DevComplex &C, const Lds &S, 
       const float sq = fmaxf(q0.x + n * (ux * ux + uy * uy + uz * uz) - 2.0f * rm, 0.0f);
       elhm += C.mcs_tcc[m] * sqrtf(sq / n);
     }
+    elhm = seg_sum(elhm, lpr);
+    t_kde = seg_sum(t_kde, lpr);
   }
-  elhm = seg_sum(elhm, lpr);
-  t_kde = seg_sum(t_kde, lpr);
+  if constexpr (SHARE) {
+    float *rb = S.xp + sp.par * sp.xp_floats + (size_t)sp.ns * sp.half * 4;  // [ns][slots][12]
+    if (live && (sp.ltid - sp.slot * lpr) == 0)
+      *reinterpret_cast<float4 *>(rb + ((size_t)sp.h * sp.slots + sp.slot) * 12) =
+          make_float4(t_kde, elhm, 0.f, 0.f);
+    __syncthreads();
+    if (live) {
+      for (int g = 0; g < sp.ns; ++g) {
+        const float4 r = *reinterpret_cast<const float4 *>(rb + ((size_t)g * sp.slots + sp.slot) * 12);
+        t_kde = g == 0 ? r.x : t_kde + r.x;
+        elhm = g == 0 ? r.y : elhm + r.y;
+      }
+    }
+    sp.par ^= 1;
+  }
+  if (!live) return 0.0f;
   const float lna = (float)C.lna;
   const float e6 = (t_kde / C.kde3) / lna;
   const float e7 = pos > 0 ? logf(elhm / (float)pos) : 0.0f;
@@ -734,13 +762,19 @@ __device__ __forceinline__ float cheap_terms(const DevComplex &C, const Lds &S, 
 // wave's atoms, 16 per MFMA tile: tile (k, sub) holds atom k of lanes
 // 16 sub .. 16 sub + 15. `da` is the wave's scratch: X tile [16][64] bf16,
 // A1 tile [16][128] bf16, y[16].
-template <int APL, int DAM>
+template <int APL, int DAM, bool SHARE>
 __device__ __forceinline__ float sur_total(const KParams &P, const DevComplex &C, const Lds &S,
+                                           Split &sp, bool live,
                                            unsigned char *da, int lpr, int prt,
                                            const float (&x)[APL], const float (&y)[APL],
                                            const float (&z)[APL], const int (&lt)[APL],
                                            const bool (&val)[APL], const float mm[6],
                                            const Rot &R, int a, int ilig) {
+  if (!live) {  // the cheap terms' exchange only (a workgroup barrier)
+    if constexpr (SHARE)
+      (void)cheap_terms<APL, true>(C, S, lpr, sp, false, x, y, z, lt, val, mm, R, a, ilig);
+    return 0.0f;
+  }
   if constexpr (DAM == 2) {
     // tabulated network: trilinear interpolation of the atom type's output
     // grid (same grid geometry and clamping as the feature grid)
@@ -769,7 +803,7 @@ __device__ __forceinline__ float sur_total(const KParams &P, const DevComplex &C
     }
     ys = seg_sum(ys, lpr);
     return P.sur_scale * (ys / (float)C.lna) +
-           cheap_terms<APL>(C, S, lpr, x, y, z, lt, val, mm, R, a, ilig);
+           cheap_terms<APL, SHARE>(C, S, lpr, sp, true, x, y, z, lt, val, mm, R, a, ilig);
   }
   const int l = threadIdx.x & 63;
   unsigned short *Xs = reinterpret_cast<unsigned short *>(da);
@@ -824,7 +858,7 @@ __device__ __forceinline__ float sur_total(const KParams &P, const DevComplex &C
   ysum = seg_sum(ysum, lpr);
   // the network sums per-atom pair energies; the terms are per-atom means
   return P.sur_scale * (ysum / (float)C.lna) +
-         cheap_terms<APL>(C, S, lpr, x, y, z, lt, val, mm, R, a, ilig);
+         cheap_terms<APL, SHARE>(C, S, lpr, sp, true, x, y, z, lt, val, mm, R, a, ilig);
 }
 
 // ExchangeReplicas_d for one ladder (src/kernel_cuda_l1_exchangereplicas.cu:33-78):
@@ -933,8 +967,13 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
       extern __shared__ __attribute__((aligned(16))) unsigned char lds_se[];
       unsigned char *dse = lds_se + P.off_da + (threadIdx.x >> 6) * 6208;
       place(mo);
-      const float sv_s = sur_total<APL, DA>(P, C, S, dse, lpr, pk.prt, x, y, z, lt, val, mo, Rp, a,
-                                                ilig);
+      const bool live = __ballot(active) != 0;
+      const float sv_s =
+          (sp.ns > 1 && C.da_share != 0)
+              ? sur_total<APL, DA, true>(P, C, S, sp, live, dse, lpr, pk.prt, x, y, z, lt, val, mo,
+                                         Rp, a, ilig)
+              : sur_total<APL, DA, false>(P, C, S, sp, live, dse, lpr, pk.prt, x, y, z, lt, val,
+                                          mo, Rp, a, ilig);
       if (active && lead) P.ev_sur[sv] = sv_s;
       return;
     }
@@ -1011,13 +1050,20 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   const int K = DA ? P.da_k : 1;
   unsigned char *da = nullptr;
   float s_old = 0.0f;
+  // the split's groups share the surrogate's KDE and MCS sums for complexes
+  // with large KDE or MCS sets (one exchange per surrogate evaluation, so
+  // every thread evaluates it together)
+  const bool share = DA && sp.ns > 1 && C.da_share != 0;
   if constexpr (DA) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_da[];
     da = lds_da + P.off_da + (threadIdx.x >> 6) * 6208;
-    if (wave_active) {
-      place(mo);
-      s_old = sur_total<APL, DA>(P, C, S, da, lpr, pk.prt, x, y, z, lt, val, mo, Rp, a, ilig);
-    }
+    if (wave_active) place(mo);
+    if (share)
+      s_old = sur_total<APL, DA, true>(P, C, S, sp, wave_active, da, lpr, pk.prt, x, y, z, lt,
+                                       val, mo, Rp, a, ilig);
+    else if (wave_active)
+      s_old = sur_total<APL, DA, false>(P, C, S, sp, true, da, lpr, pk.prt, x, y, z, lt, val, mo,
+                                        Rp, a, ilig);
   }
 
   for (int s = step0; s < step0 + P.n_steps; s += K) {
@@ -1027,8 +1073,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
         float dm[6];
         mc_uniforms(P.k0, P.k1, cid, flat, s, u);
         move_offsets(u, scale, C.move_kind, dm);
-#pragma unroll
-        for (int i = 0; i < 6; ++i) mn[i] = mo[i] + dm[i];
+        apply_move(mo, dm, mn);
         place(mn);
       }
       gd_energy E;
@@ -1047,22 +1092,46 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
       for (int i = 0; i < 6; ++i) yv[i] = mo[i];
       float s_y = s_old, u7 = 1.0f;
       bool moved = false;
-      for (int j = 0; wave_active && j < K; ++j) {
-        float u[8], dm[6], mn[6];
-        mc_uniforms(P.k0, P.k1, cid, flat, s + j, u);
-        move_offsets(u, scale, C.move_kind, dm);
+      if (!share) {
+        for (int j = 0; wave_active && j < K; ++j) {
+          float u[8], dm[6], mn[6];
+          mc_uniforms(P.k0, P.k1, cid, flat, s + j, u);
+          move_offsets(u, scale, C.move_kind, dm);
+          apply_move(yv, dm, mn);
+          place(mn);
+          const float s_n = sur_total<APL, DA, false>(P, C, S, sp, true, da, lpr, pk.prt, x, y, z,
+                                                      lt, val, mn, Rp, a, ilig);
+          if (metropolis(s_n, s_y, mb, u[6])) {
 #pragma unroll
-        for (int i = 0; i < 6; ++i) mn[i] = yv[i] + dm[i];
-        place(mn);
-        const float s_n =
-            sur_total<APL, DA>(P, C, S, da, lpr, pk.prt, x, y, z, lt, val, mn, Rp, a, ilig);
-        if (metropolis(s_n, s_y, mb, u[6])) {
-#pragma unroll
-          for (int i = 0; i < 6; ++i) yv[i] = mn[i];
-          s_y = s_n;
-          moved = true;
+            for (int i = 0; i < 6; ++i) yv[i] = mn[i];
+            s_y = s_n;
+            moved = true;
+          }
+          u7 = u[7];
         }
-        u7 = u[7];
+      } else {
+        // every wave runs the K evaluations: their exchanges are workgroup
+        // barriers
+        for (int j = 0; j < K; ++j) {
+          float u[8], dm[6], mn[6];
+          if (wave_active) {
+            mc_uniforms(P.k0, P.k1, cid, flat, s + j, u);
+            move_offsets(u, scale, C.move_kind, dm);
+            apply_move(yv, dm, mn);
+            place(mn);
+          }
+          const float s_n = sur_total<APL, DA, true>(P, C, S, sp, wave_active, da, lpr, pk.prt, x,
+                                                     y, z, lt, val, mn, Rp, a, ilig);
+          if (wave_active) {
+            if (metropolis(s_n, s_y, mb, u[6])) {
+#pragma unroll
+              for (int i = 0; i < 6; ++i) yv[i] = mn[i];
+              s_y = s_n;
+              moved = true;
+            }
+            u7 = u[7];
+          }
+        }
       }
       const bool eval = wave_active && __ballot(moved) != 0;
       if (eval) place(yv);

@@ -37,14 +37,24 @@ def _engine(gd, cx, seed, params_seed=3, tab=False, **kw):
     return eng, cid
 
 
-def test_da_scale0_tracks_cpu_path(gd):
+# A complex with large KDE and MCS sets: the groups of a receptor split share
+# the surrogate's KDE and MCS sums (under spread packing, split 8, and under a
+# fixed split of 4).
+def _large_lists():
+    return synth.make_complex(seed=31, lna=24, pnp=900, pnk=320, pos=36, mcs_atoms=20, n_lig=2,
+                              n_prt=1, n_tmp=6)
+
+
+@pytest.mark.parametrize("case", ["single", "shared_spread", "shared_split4"])
+def test_da_scale0_tracks_cpu_path(gd, case):
     """Decision for decision against the CPU DA chain, up to explained rounding
     splits of a sub-step, composite or exchange test (oracle/decisions.py)."""
     from oracle import decisions, gdo
-    cx = synth.single_complex(seed=0)
+    cx = synth.single_complex(seed=0) if case == "single" else _large_lists()
+    kw = dict(receptor_split=4) if case == "shared_split4" else {}
     n_steps, nc, k = 400, 20, 4
     recs = []
-    eng, cid = _engine(gd, cx, 11)
+    eng, cid = _engine(gd, cx, 11, **kw)
     st = eng.run_da(n_steps, nc, k, 0.0, sink=lambda c, r: recs.append(r))
     state = eng.state(cid)
     eng.close()
@@ -224,15 +234,18 @@ def test_da_equals_plain_when_surrogate_is_exact(gd, tab):
     assert np.array_equal(p["replica"][:, 2], q["replica"][:, 2]) or same.sum() < len(same)
 
 
-@pytest.mark.parametrize("tab", [False, True])
-def test_surrogate_cheap_terms_equal_exact_terms(gd, tab):
+@pytest.mark.parametrize("tab,shared", [(False, False), (True, False), (False, True),
+                                       (True, True)])
+def test_surrogate_cheap_terms_equal_exact_terms(gd, tab, shared):
     """With scale 0 the surrogate is its cheap part: the weighted KDE, MCS and
     pocket-distance terms. Its MCS restraint comes in closed form from the
     conformer's template moments (A + n|R obar + t - mbar|^2 - 2<R, M>); it
     must equal the exact per-atom terms of gd_calc_energy to FP32 rounding,
     over many conformers, both receptor models and poses far from and near
-    the templates."""
-    cx = synth.make_complex(seed=21, lna=40, pnp=600, pnk=120, pos=7, mcs_atoms=30, n_lig=5,
+    the templates. With `shared`, a complex with large KDE and MCS sets under
+    a receptor split of 4: the split's groups add up the sums together."""
+    pnk, pos, kw = (320, 36, dict(receptor_split=4)) if shared else (120, 7, {})
+    cx = synth.make_complex(seed=21, lna=40, pnp=600, pnk=pnk, pos=pos, mcs_atoms=30, n_lig=5,
                             n_prt=2, n_tmp=2)
     rng = np.random.default_rng(4)
     n = 300
@@ -241,14 +254,14 @@ def test_surrogate_cheap_terms_equal_exact_terms(gd, tab):
     mm[:, 3:] = rng.uniform(-np.pi, np.pi, (n, 3)) * (rng.random((n, 1)) < 0.7)
     il = rng.integers(0, cx.n_lig, n).astype(np.int32)
     ip = rng.integers(0, cx.n_prt, n).astype(np.int32)
-    eng, cid = _engine(gd, cx, 3, tab=tab)
+    eng, cid = _engine(gd, cx, 3, tab=tab, **kw)
     s = eng.sur_surrogate(cid, il, ip, mm, 0.0).astype(np.float64)
     e = eng.calc_energy(cid, il, ip, mm)["e"].astype(np.float64)
     eng.close()
     w = np.array(cx.enepara_struct.w[:], np.float64)
     want = (e[:, 6:9] * w[6:9]).sum(axis=1)
     err = np.abs(s - want)
-    print(f"\ntab={tab}: worst |S - exact cheap terms| {err.max():.2e} (scale {np.abs(want).max():.2f})")
+    print(f"\ntab={tab} shared={shared}: worst |S - exact cheap terms| {err.max():.2e} (scale {np.abs(want).max():.2f})")
     assert np.all(err <= 2e-5 * np.maximum(1.0, np.abs(want))), err.max()
 
 
