@@ -25,6 +25,11 @@ explained only if, in the CPU path's margin log, one of these holds:
 tol = 4 |minus_beta|_max dE and tol_x = 4 max|mb_i - mb_j| dE, where dE is the
 largest GPU-vs-CPU total-energy difference measured over the records both
 runs share (floored at 2^-20 of the largest |E|). Anything else raises.
+
+The accepted poses before a replica's divergence must also agree: bit for
+bit with uniform moves (the reference's kind, move_scale * (2U - 1), one
+product and one sum per coordinate on both sides), within 1e-4 with Gaussian
+moves (the device's log, sqrt and sincos against libm's).
 """
 import numpy as np
 
@@ -56,6 +61,7 @@ def compare(cx, gpu_recs, cpu, nc, k_sub=1, gpu_state=None, da=False):
 
     first = {}
     d_e, e_max = 0.0, 1e-30
+    n_pose, pose_differ, pose_dmax = 0, 0, 0.0
     for r in range(n_rep):
         g, c = g_by[r], cpu.records(r)
         kg = np.stack([g["step"], g["replica"][:, 2]], 1) if len(g) else np.zeros((0, 2), int)
@@ -68,6 +74,13 @@ def compare(cx, gpu_recs, cpu, nc, k_sub=1, gpu_state=None, da=False):
             ec = c["energy"]["e"][:k, 9].astype(np.float64)
             d_e = max(d_e, float(np.abs(eg - ec).max()))
             e_max = max(e_max, float(np.abs(ec).max()))
+            # the accepted poses up to the divergence: the same uniforms and
+            # the same sums of offsets
+            gm = np.ascontiguousarray(g["movematrix"][:k], np.float32)
+            cm = np.ascontiguousarray(c["movematrix"][:k], np.float32)
+            n_pose += k
+            pose_differ += int(np.any(gm.view(np.uint32) != cm.view(np.uint32), axis=1).sum())
+            pose_dmax = max(pose_dmax, float(np.abs(gm.astype(np.float64) - cm).max()))
         if k == len(kg) == len(kc):
             continue
         s_g = int(kg[k, 0]) if k < len(kg) else np.iinfo(np.int64).max
@@ -116,6 +129,16 @@ def compare(cx, gpu_recs, cpu, nc, k_sub=1, gpu_state=None, da=False):
         explained[r] = s
         why[r] = reason
 
+    # poses: uniform moves (the reference's) are one product and one sum per
+    # coordinate on both sides, so they match bit for bit; Gaussian moves go
+    # through the device's and libm's log, sqrt and sincos and match to a few
+    # ulps
+    if getattr(cx, "move_kind", None) == 0:
+        assert pose_differ == 0, (
+            f"{pose_differ} of {n_pose} accepted poses differ from the CPU path's "
+            f"(largest difference {pose_dmax:.3g})")
+    assert pose_dmax <= 1e-4, pose_dmax
+
     # counters of undisturbed replicas and ladders
     if gpu_state is not None:
         dis_ladders = {ladder(r) for r in first}
@@ -128,4 +151,5 @@ def compare(cx, gpu_recs, cpu, nc, k_sub=1, gpu_state=None, da=False):
                 assert gpu_state["replica"][r, 2] == cpu.idx_tmp[r], r
                 assert gpu_state["acs_x"][r] == cpu.acs_x[r], r
     return {"replicas": n_rep, "identical": n_rep - len(first), "explained": why,
-            "d_e": d_e, "tol": tol, "tol_x": tol_x}
+            "d_e": d_e, "tol": tol, "tol_x": tol_x, "poses": n_pose,
+            "poses_differing": pose_differ, "pose_max_diff": pose_dmax}

@@ -257,6 +257,44 @@ def test_mc_run_tracks_cpu_path(oracle, eng_mod, split):
           state["acs_x"])
 
 
+@pytest.mark.parametrize("case", ["plain", "plain_split4", "da"])
+def test_uniform_moves_give_the_cpu_poses_bit_for_bit(oracle, eng_mod, case):
+    """With the reference's uniform moves (move_scale * (2U - 1)) every pose
+    the engine accepts, up to a replica's first explained divergence, has the
+    bits of the CPU path's: the same Philox uniforms, and one product and one
+    sum per coordinate, rounded one at a time on both sides."""
+    from oracle import decisions
+    from geaxdock_gpu_amd import surrogate as sg
+    cx = synth.make_complex(seed=5, lna=32, pnp=1200, n_lig=2, n_tmp=8,
+                            move_kind=abi.MOVE_UNIFORM)
+    n_steps, nc, k = 200, 20, 4
+    da = case == "da"
+    recs = []
+    kw = dict(receptor_split=4) if case == "plain_split4" else {}
+    with eng_mod.Engine(seed=11, **kw) as eng:
+        cid = eng.add_complex(cx)
+        if da:
+            eng.sur_build_grids(0.8, 12.0)
+            eng.sur_set_params(sg.pack_params(sg.init_params(3)))
+            eng.run_da(n_steps, nc, k, 0.0, sink=lambda c, r: recs.append(r))
+        else:
+            eng.run(n_steps, nc, sink=lambda c, r: recs.append(r))
+        state = eng.state(cid)
+    recs = np.concatenate(recs)
+    d = cx.desc()
+    cpu = oracle.McState(cx, rec_cap=n_steps + 1)
+    cpu.enable_log(0, n_steps, n_steps // nc)
+    oracle.mc_run(d, cpu, seed=11, init=True, step0=0, n_steps=0, nc=nc)
+    if da:
+        oracle.mc_run_da(d, cpu, seed=11, step0=0, n_steps=n_steps, nc=nc, k_sub=k)
+    else:
+        oracle.mc_run(d, cpu, seed=11, init=False, step0=0, n_steps=n_steps, nc=nc)
+    out = decisions.compare(cx, recs, cpu, nc, k_sub=k if da else 1, gpu_state=state, da=da)
+    print(f"\n{case}: {out['identical']}/{cx.n_rep} identical; {out['poses']} accepted poses, "
+          f"{out['poses_differing']} differing")
+    assert out["poses"] > 10 * cx.n_rep and out["poses_differing"] == 0
+
+
 def test_checkpoint_resume_bit_exact(eng_mod):
     cx = synth.single_complex(seed=1)
     with eng_mod.Engine(seed=5, record=False) as eng:
