@@ -1145,7 +1145,8 @@ KParams base_params(gd_ctx *ctx) {
 }
 
 // Kernel instance for atoms per lane `apl` and mode `da`: 0 plain,
-// 1 delayed acceptance with the network per atom, 2 with its tables.
+// 1 delayed acceptance with the network per atom, 2 with its tables, 3 and 4
+// the same with the split groups' shared surrogate sums compiled in.
 const void *mc_kernel_fn(int apl, int da, bool tiled) {
   switch (apl) {
     case 1: return mc_kernel_handle<1>(da, tiled);
@@ -1153,6 +1154,15 @@ const void *mc_kernel_fn(int apl, int da, bool tiled) {
     case 3: return mc_kernel_handle<3>(da, tiled);
     default: return mc_kernel_handle<4>(da, tiled);
   }
+}
+
+// Instance mode of a delayed-acceptance or surrogate launch (see
+// mc_kernel_fn): the shared-sum variants only when some complex's KDE or MCS
+// sets are large enough to share, so other workloads run code without them.
+int da_instance(const gd_ctx *ctx, bool tab) {
+  bool any = false;
+  for (const HostComplex &h : ctx->cx) any = any || h.dev.da_share != 0;
+  return (tab ? 2 : 1) + (any ? 2 : 0);
 }
 
 // Launch every class of a pack set, the classes concurrently on side streams
@@ -1338,7 +1348,7 @@ gd_status launch_mc_enqueue(gd_ctx *ctx, int mode, int step0, int n_steps, int n
     P.sur_tab = S.tab_valid ? 1 : 0;
     scratch = !P.sur_tab;  // the direct network evaluation needs per-wave LDS scratch
   }
-  const int da = P.da_k > 0 ? (P.sur_tab ? 2 : 1) : 0;
+  const int da = P.da_k > 0 ? da_instance(ctx, P.sur_tab != 0) : 0;
   const bool persistent = (ctx->opts.launch & GD_LAUNCH_PERSISTENT) && mode == MODE_MC;
   const bool use_graph = !(ctx->opts.launch & GD_LAUNCH_NO_GRAPH) && mode == MODE_MC;
   int *queue = persistent ? ctx->d_ctrl + 1 : nullptr;
@@ -1786,7 +1796,7 @@ static gd_status eval_poses(gd_ctx *ctx, int cid, int n, const int *idx_lig, con
   P.ev_out = d_out;
   P.ev_sur = d_sur;
   P.mode = sur ? MODE_SUREVAL : MODE_EVAL;
-  hipError_t le = launch_classes(ctx, ps.classes, P, sur ? (P.sur_tab ? 2 : 1) : 0,
+  hipError_t le = launch_classes(ctx, ps.classes, P, sur ? da_instance(ctx, P.sur_tab != 0) : 0,
                                  sur && !P.sur_tab, ctx->stream);
   if (le == hipSuccess && out)
     le = hipMemcpyAsync(out, d_out, (size_t)n * sizeof(gd_energy), hipMemcpyDeviceToHost, ctx->stream);

@@ -891,10 +891,13 @@ __device__ __forceinline__ void exchange_ladder(int n_tmp, int mode_t, const flo
 }
 
 // DA: 0 plain Metropolis; delayed acceptance with the surrogate from the
-// network evaluated per atom (1) or from its tabulated outputs (2).
+// network evaluated per atom (1) or from its tabulated outputs (2); 3 and 4
+// are 1 and 2 with the split groups' shared surrogate sums compiled in.
 template <int APL, int DA, bool TILE>
 __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const DevComplex &C, const Lds &S,
                          int lpr, int step0) {
+  constexpr int DAM = DA > 2 ? DA - 2 : DA;
+  constexpr bool SHARED = DA > 2;
   const int tid = threadIdx.x;
   Split sp;
   sp.ns = P.unsplit ? 1 : pk.ns;
@@ -968,12 +971,17 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
       unsigned char *dse = lds_se + P.off_da + (threadIdx.x >> 6) * 6208;
       place(mo);
       const bool live = __ballot(active) != 0;
-      const float sv_s =
-          (sp.ns > 1 && C.da_share != 0)
-              ? sur_total<APL, DA, true>(P, C, S, sp, live, dse, lpr, pk.prt, x, y, z, lt, val, mo,
-                                         Rp, a, ilig)
-              : sur_total<APL, DA, false>(P, C, S, sp, live, dse, lpr, pk.prt, x, y, z, lt, val,
+      float sv_s;
+      if constexpr (SHARED) {
+        sv_s = (sp.ns > 1 && C.da_share != 0)
+                   ? sur_total<APL, DAM, true>(P, C, S, sp, live, dse, lpr, pk.prt, x, y, z, lt,
+                                               val, mo, Rp, a, ilig)
+                   : sur_total<APL, DAM, false>(P, C, S, sp, live, dse, lpr, pk.prt, x, y, z, lt,
+                                                val, mo, Rp, a, ilig);
+      } else {
+        sv_s = sur_total<APL, DAM, false>(P, C, S, sp, live, dse, lpr, pk.prt, x, y, z, lt, val,
                                           mo, Rp, a, ilig);
+      }
       if (active && lead) P.ev_sur[sv] = sv_s;
       return;
     }
@@ -1053,17 +1061,19 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
   // the split's groups share the surrogate's KDE and MCS sums for complexes
   // with large KDE or MCS sets (one exchange per surrogate evaluation, so
   // every thread evaluates it together)
-  const bool share = DA && sp.ns > 1 && C.da_share != 0;
+  const bool share = SHARED && sp.ns > 1 && C.da_share != 0;
   if constexpr (DA) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_da[];
     da = lds_da + P.off_da + (threadIdx.x >> 6) * 6208;
     if (wave_active) place(mo);
-    if (share)
-      s_old = sur_total<APL, DA, true>(P, C, S, sp, wave_active, da, lpr, pk.prt, x, y, z, lt,
-                                       val, mo, Rp, a, ilig);
-    else if (wave_active)
-      s_old = sur_total<APL, DA, false>(P, C, S, sp, true, da, lpr, pk.prt, x, y, z, lt, val, mo,
-                                        Rp, a, ilig);
+    if constexpr (SHARED) {
+      if (share)
+        s_old = sur_total<APL, DAM, true>(P, C, S, sp, wave_active, da, lpr, pk.prt, x, y, z, lt,
+                                          val, mo, Rp, a, ilig);
+    }
+    if (!share && wave_active)
+      s_old = sur_total<APL, DAM, false>(P, C, S, sp, true, da, lpr, pk.prt, x, y, z, lt, val, mo,
+                                         Rp, a, ilig);
   }
 
   for (int s = step0; s < step0 + P.n_steps; s += K) {
@@ -1099,8 +1109,8 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
           move_offsets(u, scale, C.move_kind, dm);
           apply_move(yv, dm, mn);
           place(mn);
-          const float s_n = sur_total<APL, DA, false>(P, C, S, sp, true, da, lpr, pk.prt, x, y, z,
-                                                      lt, val, mn, Rp, a, ilig);
+          const float s_n = sur_total<APL, DAM, false>(P, C, S, sp, true, da, lpr, pk.prt, x, y,
+                                                       z, lt, val, mn, Rp, a, ilig);
           if (metropolis(s_n, s_y, mb, u[6])) {
 #pragma unroll
             for (int i = 0; i < 6; ++i) yv[i] = mn[i];
@@ -1109,7 +1119,7 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
           }
           u7 = u[7];
         }
-      } else {
+      } else if constexpr (SHARED) {
         // every wave runs the K evaluations: their exchanges are workgroup
         // barriers
         for (int j = 0; j < K; ++j) {
@@ -1120,8 +1130,8 @@ __device__ __forceinline__ void run_pack(const KParams &P, const Pack &pk, const
             apply_move(yv, dm, mn);
             place(mn);
           }
-          const float s_n = sur_total<APL, DA, true>(P, C, S, sp, wave_active, da, lpr, pk.prt, x,
-                                                     y, z, lt, val, mn, Rp, a, ilig);
+          const float s_n = sur_total<APL, DAM, true>(P, C, S, sp, wave_active, da, lpr, pk.prt,
+                                                      x, y, z, lt, val, mn, Rp, a, ilig);
           if (wave_active) {
             if (metropolis(s_n, s_y, mb, u[6])) {
 #pragma unroll
