@@ -347,7 +347,10 @@ gd_status gd_sur_predict(gd_ctx *ctx, int pose_begin, int pose_end, float *out);
  * move with min(1, exp(-beta [(E(y) - E(x)) -
  * (S(y) - S(x))])) from one exact evaluation, so the chain keeps the exact
  * Boltzmann distribution. n_steps and steps_per_exchange are multiples of
- * k_sub; replica-steps count surrogate steps. */
+ * k_sub; replica-steps count surrogate steps. For a complex with at least 256
+ * KDE points or 32 MCS templates, the groups of a receptor split share the
+ * surrogate's KDE and MCS sums (S then differs from the whole-list sum by
+ * summation order only). */
 gd_status gd_run_da(gd_ctx *ctx, int64_t n_steps, int steps_per_exchange, int k_sub,
                     float sur_scale, gd_record_sink sink, void *user, gd_run_stats *stats);
 /* Tabulate the network on every complex's surrogate grid: for each ligand
