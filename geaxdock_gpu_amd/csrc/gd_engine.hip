@@ -728,7 +728,7 @@ int complex_tile(const gd_ctx *ctx, const HostComplex &h) {
 
 // LDS layout for a launch covering packs with these maxima, in blocks of
 // `block` lanes; `split` adds the partial-sum exchange buffers of packs with
-// more than one lane group ([ns*half][4] HPC sums and [ns][half/8][8] replica
+// more than one lane group ([ns*half][4] HPC sums and [ns][half/8][12] replica
 // sums, ns*half = block). With tile > 0 the receptor arrays are two tiles of
 // `tile` points and KDE stays in HBM.
 void layout(KParams &P, size_t *bytes, int pnp_max, int pnk_max, int nch_max, int slots_max,
