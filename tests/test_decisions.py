@@ -4,7 +4,8 @@ Two FP32 CPU paths on receptors whose coordinates differ by N(0, 3e-3 A) stand
 in for the engine: their energies differ by a few 1e-5, so every divergence
 must be explained by a near-threshold decision. A run with a planted decision bug (one
 temperature decided with its neighbour's minus_beta, as a wrong beta after an
-exchange would) must be rejected."""
+exchange would) must be rejected, and so must accepted poses one ulp off the
+CPU path's under uniform moves."""
 import copy
 
 import numpy as np
@@ -66,3 +67,28 @@ def test_planted_decision_bug_is_rejected():
     _, recs = _run(bad, 1, 300, 20)
     with pytest.raises(AssertionError, match="diverge"):
         decisions.compare(cx, recs, cpu, 20)
+
+
+def test_planted_pose_error_is_rejected():
+    """Uniform moves must give the CPU path's poses bit for bit: a run whose
+    accepted poses are off by one ulp in one coordinate (decisions unchanged)
+    is rejected; with Gaussian moves such a difference is within bounds, and
+    one of 1e-3 is not."""
+    from geaxdock_gpu_amd import abi
+    for kind in (abi.MOVE_UNIFORM, abi.MOVE_GAUSSIAN):
+        cx = synth.make_complex(6, lna=20, pnp=600, n_lig=2, n_prt=1, n_tmp=6, move_kind=kind)
+        cpu, _ = _run(cx, 3, 200, 20, log=True)
+        other, recs = _run(cx, 3, 200, 20)
+        out = decisions.compare(cx, recs, cpu, 20, gpu_state=_state(other))
+        assert out["poses"] > 0 and out["poses_differing"] == 0
+        bad = recs.copy()
+        bad["movematrix"][:, 0] = np.nextafter(bad["movematrix"][:, 0], np.float32(np.inf))
+        if kind == abi.MOVE_UNIFORM:
+            with pytest.raises(AssertionError, match="accepted poses differ"):
+                decisions.compare(cx, bad, cpu, 20, gpu_state=_state(other))
+        else:
+            out = decisions.compare(cx, bad, cpu, 20, gpu_state=_state(other))
+            assert out["poses_differing"] > 0
+            bad["movematrix"][:, 1] += np.float32(1e-3)
+            with pytest.raises(AssertionError):
+                decisions.compare(cx, bad, cpu, 20, gpu_state=_state(other))
